@@ -1,0 +1,318 @@
+/*
+ * pt_amd.h — C ABI of the MI355X path tracer (libptamd.so).
+ *
+ * Plain C: no HIP or torch types, only pointers, sizes and POD structs.
+ * Every entry point returns a pt_status; pt_last_error() gives the message.
+ * Handles (pt_scene, pt_film) own their device memory.  The library is
+ * thread-compatible (one thread per handle at a time), not thread-safe.
+ *
+ * Each declaration cites the interface of the reference renderer
+ * (Neroued/PathTracingTutorial, paths relative to Part_2/src) that it replaces.
+ */
+#ifndef PT_AMD_H
+#define PT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_ABI_VERSION 1
+
+typedef int pt_status;
+enum {
+    PT_OK = 0,
+    PT_ERR_INVALID_ARG = 1,
+    PT_ERR_IO = 2,
+    PT_ERR_PARSE = 3,
+    PT_ERR_DEVICE = 4,
+    PT_ERR_STATE = 5,
+    PT_ERR_UNSUPPORTED = 6,
+    PT_ERR_OUT_OF_MEMORY = 7,
+    PT_ERR_LIMIT = 8
+};
+
+/* ------------------------------------------------------------------------ */
+/* Data contract: byte layouts identical to the reference's host structs so  */
+/* that HostScene arrays convert by memcpy.                                   */
+/* ------------------------------------------------------------------------ */
+
+/* vec3 with the reference's alignas(16) padding (math/vecmath.h:63). */
+typedef struct pt_vec3p { float x, y, z, _pad; } pt_vec3p;
+
+/* Vertex, 64 B (geometry/vertex.h:10-15). */
+typedef struct pt_vertex {
+    pt_vec3p position;
+    pt_vec3p normal;
+    float u, v;
+    float _pad[2];
+    float tangent[4]; /* xyz direction, w handedness sign */
+} pt_vertex;
+
+/* TriangleFace, 16 B (geometry/vertex.h:17-20). */
+typedef struct pt_face { uint32_t v0, v1, v2; int32_t material_id; } pt_face;
+
+/* Material, 256 B (materials/material.h:8-81), field for field. */
+typedef struct pt_material {
+    pt_vec3p base_color;
+    float metallic;
+    float roughness, specular_tint, anisotropic, sheen;
+    float sheen_tint, clearcoat, clearcoat_roughness, spec_trans;
+    float ior, subsurface, specular_factor, specular_color_r;
+    float specular_color_g, specular_color_b, anisotropy_rotation, normal_scale;
+    float _pad_e[3];
+    pt_vec3p emissive;
+    float emissive_strength;
+    float alpha_cutoff;
+    int32_t alpha_mode; /* 0 opaque, 1 mask, 2 blend */
+    float base_alpha;
+    int32_t double_sided;
+    float sheen_color_r, sheen_color_g, sheen_color_b, occlusion_strength;
+    float thickness_factor, attenuation_distance, att_color_r, att_color_g;
+    float att_color_b;
+    float uv_offset_x, uv_offset_y, uv_scale_x;
+    float uv_scale_y, uv_rotation, _pad1, _pad2;
+    int32_t base_color_tex_id, metallic_roughness_tex_id, normal_tex_id, emissive_tex_id;
+    int32_t occlusion_tex_id, clearcoat_tex_id, clearcoat_roughness_tex_id, clearcoat_normal_tex_id;
+    int32_t transmission_tex_id, sheen_color_tex_id, sheen_roughness_tex_id, _pad3;
+    int32_t _pad_tail[3];
+} pt_material;
+
+/* PunctualLight, 112 B (lights/punctual_light.h:14-25). type: 0 point, 1 directional, 2 spot. */
+typedef struct pt_punctual_light {
+    pt_vec3p position;
+    float range;
+    float _pad0[3];
+    pt_vec3p color;
+    float intensity;
+    float _pad1[3];
+    pt_vec3p direction;
+    float inner_cone, outer_cone; /* cosines */
+    int32_t type;
+    float _pad2[2];
+    float _tail[3];
+} pt_punctual_light;
+
+/* Camera, 80 B (camera/camera.h:12-21). fov is vertical, in degrees. */
+typedef struct pt_camera {
+    pt_vec3p position;
+    pt_vec3p look_at;
+    pt_vec3p up;
+    float fov, aperture, focal_distance;
+    int32_t width, height;
+    float _pad[3];
+} pt_camera;
+
+/* CameraFrame as built by buildCameraFrame (camera/camera.h:25-51). */
+typedef struct pt_camera_frame {
+    float position[3], forward[3], right[3], up[3];
+    float tan_half_fov, aspect, aperture, focal_distance;
+} pt_camera_frame;
+
+/* A texture: RGBA float texels, row 0 first; wrap_s/t are GL enums
+ * (10497 repeat, 33071 clamp-to-edge, 33648 mirrored repeat) as in
+ * io/image_io.h:8-17.  linear=0 applies the reference's pow(x, 2.2) decode to
+ * colour fetches (materials/disney_bsdf.cuh:81-83); linear=1 skips it. */
+typedef struct pt_texture_desc {
+    const float* rgba;
+    int32_t width, height;
+    int32_t wrap_s, wrap_t;
+    int32_t linear;
+    int32_t _pad;
+} pt_texture_desc;
+
+/* Render settings (scene/scene.h:16-22 RenderSettings) plus a global seed.
+ * sampler: 0 PCG, 1 Sobol.  seed 0 reproduces the reference's sequences. */
+typedef struct pt_render_settings {
+    int32_t sample_per_frame;
+    int32_t target_samples;
+    int32_t max_depth;
+    int32_t sampler;
+    uint32_t seed;
+    int32_t _pad[3];
+} pt_render_settings;
+
+/* Host scene descriptor: the fields of HostScene (scene/scene.h:24-35). */
+typedef struct pt_host_scene_desc {
+    const pt_vertex* vertices; uint32_t num_vertices;
+    const pt_face* faces; uint32_t num_faces;
+    const pt_material* materials; uint32_t num_materials;
+    const pt_texture_desc* textures; uint32_t num_textures;
+    const pt_punctual_light* lights; uint32_t num_lights;
+    const pt_camera* camera;
+    /* environment map, RGBA float (channel 4 as device_scene.cu:19 requires); NULL for none */
+    const float* env_rgba; int32_t env_width, env_height;
+    const pt_render_settings* settings;
+} pt_host_scene_desc;
+
+/* BVH build options.  Reference settings (accel/bvh_builder.cpp:10-13):
+ * bins 24, leaf_threshold 4 (leaf when count < 4), max_leaf 0 (no cap on the
+ * SAH early-out leaf).  max_leaf > 0 forbids SAH leaves above that size. */
+typedef struct pt_bvh_options {
+    int32_t bins;
+    int32_t leaf_threshold;
+    int32_t max_leaf;
+    int32_t builder; /* 0 = GPU binned SAH (default) */
+} pt_bvh_options;
+
+typedef struct pt_scene_info {
+    uint32_t num_vertices, num_faces, num_materials, num_textures;
+    uint32_t num_lights, num_emissive_triangles, num_bvh_nodes, bvh_depth;
+    int32_t env_width, env_height;
+    int32_t width, height;
+    int32_t sample_per_frame, target_samples, max_depth, sampler;
+    uint32_t seed, frame_count;
+    float bvh_build_ms;
+    uint32_t max_leaf_size;
+} pt_scene_info;
+
+/* Ray and hit counters of the last counted render (rays split as in-kernel). */
+typedef struct pt_counters {
+    uint64_t closest_rays;   /* camera + extension rays */
+    uint64_t shadow_rays;    /* any-hit rays */
+    uint64_t node_visits;    /* 32-B node records tested */
+    uint64_t tri_tests;      /* triangle intersection tests */
+    uint64_t paths;
+    uint64_t stack_overflows;
+} pt_counters;
+
+typedef struct pt_scene pt_scene;
+typedef struct pt_film pt_film;
+
+enum { PT_INTEGRATOR_PATH = 0, PT_INTEGRATOR_WAVEFRONT = 1 };
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                    */
+/* ------------------------------------------------------------------------ */
+int pt_abi_version(void);
+/* Message of the last failed call on this thread.  Replaces the reference's
+ * print-and-exit CUDA_CHECK (cuda/check.h:6-24). */
+const char* pt_last_error(void);
+/* Selects and initialises the HIP device. */
+pt_status pt_init(int device);
+/* Struct sizes of the data contract, for binding checks: fills out[0..7] with
+ * sizeof pt_vertex, pt_face, pt_material, pt_punctual_light, pt_camera,
+ * pt_camera_frame, pt_host_scene_desc, pt_counters. */
+pt_status pt_struct_sizes(uint32_t* out, int n);
+
+/* ------------------------------------------------------------------------ */
+/* Scene                                                                      */
+/* ------------------------------------------------------------------------ */
+/* Loads a JSON scene (scene/scene_loader.cpp:240-292 SceneLoader::load;
+ * scene/scene_manager.cpp:11 SceneManager::load).  Same schema and defaults. */
+pt_status pt_scene_load(const char* path, pt_scene** out);
+/* Builds a scene from host arrays (HostScene, scene/scene.h:24-35). */
+pt_status pt_scene_create(const pt_host_scene_desc* desc, pt_scene** out);
+/* Procedural workloads: 0 = Cornell box (36 tris, 512x512), 1 = 64 displaced
+ * icospheres (1,310,720 tris + ground, 2048x1024 sky+sun env, 1920x1080).
+ * No reference counterpart; seeds fix every value. */
+pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out);
+void pt_scene_destroy(pt_scene* scene);
+pt_status pt_scene_get_info(const pt_scene* scene, pt_scene_info* info);
+
+/* SceneManager::buildAccel (scene/scene_manager.cpp:28-41): BVH build, then
+ * emissive-triangle and environment alias tables.  opts may be NULL
+ * (reference settings). */
+pt_status pt_scene_build_accel(pt_scene* scene, const pt_bvh_options* opts);
+/* SceneManager::uploadToDevice / DeviceScene::uploadFrom
+ * (scene/device_scene.cu:8-57).  Resets frame_count to 0. */
+pt_status pt_scene_upload(pt_scene* scene);
+
+/* Setters of scene/scene_manager.h:32-36 (+ seed, samplePerFrame, camera as
+ * SceneManager::updateCamera).  They take effect at the next upload/render. */
+pt_status pt_scene_set_spp(pt_scene* scene, int spp);
+pt_status pt_scene_set_sample_per_frame(pt_scene* scene, int spf);
+pt_status pt_scene_set_max_depth(pt_scene* scene, int depth);
+pt_status pt_scene_set_sampler(pt_scene* scene, int sampler);
+pt_status pt_scene_set_seed(pt_scene* scene, uint32_t seed);
+pt_status pt_scene_set_camera(pt_scene* scene, const pt_camera* cam);
+pt_status pt_scene_get_camera(const pt_scene* scene, pt_camera* cam);
+/* buildCameraFrame (camera/camera.h:36-51) of the scene's current camera. */
+pt_status pt_scene_camera_frame(const pt_scene* scene, pt_camera_frame* frame);
+
+/* Read-only view of the scene's host arrays (HostScene fields), valid until
+ * the scene is modified or destroyed.  textures are not included. */
+typedef struct pt_host_scene_view {
+    const pt_vertex* vertices; uint32_t num_vertices;
+    const pt_face* faces; uint32_t num_faces;          /* original (load) order */
+    const pt_material* materials; uint32_t num_materials;
+    const pt_punctual_light* lights; uint32_t num_lights;
+    const float* env_rgba; int32_t env_width, env_height;
+    uint32_t num_textures;
+    pt_camera camera;
+    pt_render_settings settings;
+} pt_host_scene_view;
+pt_status pt_scene_host_view(const pt_scene* scene, pt_host_scene_view* out);
+/* Texture i of the scene: RGBA floats (pointer valid like the view). */
+pt_status pt_scene_texture(const pt_scene* scene, uint32_t i, pt_texture_desc* out);
+/* Replaces the RGBA texels of texture i (same size) and of the environment
+ * (i = 0xFFFFFFFF); takes effect at the next upload, light tables are rebuilt
+ * by the next build_accel. */
+pt_status pt_scene_set_texture_data(pt_scene* scene, uint32_t i, const float* rgba, size_t count);
+
+/* Copies of the built acceleration data, for inspection and parity tests.
+ * layout 0: the reference's depth-first order (left child = next node);
+ * layout 1: the device order (siblings adjacent).  Pass NULL to query counts.
+ * face_order[i] = original face index of BVH-ordered face i. */
+pt_status pt_scene_get_bvh(const pt_scene* scene, int layout, float* nodes8, uint32_t* num_nodes,
+                           uint32_t* face_order, uint32_t* num_faces);
+/* Emissive-triangle and environment tables (lights/light_sampler.cpp:57-149).
+ * Pass NULL to query counts.  alias entries are {float prob, uint32 alias}. */
+pt_status pt_scene_get_light_tables(const pt_scene* scene,
+                                    float* tri_pmf, uint32_t* n_tri_pmf,
+                                    float* tri_alias, uint32_t* n_tri_alias,
+                                    float* env_pmf, float* env_alias, uint32_t* n_env);
+
+/* ------------------------------------------------------------------------ */
+/* Film (film/film.cu): running-mean RGBA float, row 0 = bottom image row.   */
+/* ------------------------------------------------------------------------ */
+pt_status pt_film_create(int width, int height, pt_film** out);
+void pt_film_destroy(pt_film* film);
+/* Film::clear (film/film.cu:41-51) and Renderer::resetAccumulation
+ * (renderer.cu:71-77) when a scene is given (its frame_count returns to 0). */
+pt_status pt_film_reset(pt_film* film, pt_scene* scene);
+/* Film::downloadAccumulated (film/film.cu:53-59); which 0 = acc, 1 = new. */
+pt_status pt_film_download(const pt_film* film, int which, float* rgba_host);
+/* Device pointer of the acc (which 0) or new (which 1) surface, W*H*4 floats. */
+pt_status pt_film_device_ptr(const pt_film* film, int which, void** dev_ptr);
+
+/* ------------------------------------------------------------------------ */
+/* Rendering                                                                  */
+/* ------------------------------------------------------------------------ */
+/* Renderer::renderBatch (renderer.cu:20-45) -> Integrator::launch
+ * (integrators/integrator.h:10-20, integrators/path.cu:244-255): renders
+ * sample_per_frame samples per pixel into film, running-mean accumulates,
+ * increments frame_count.  ms_out (may be NULL) gets the HIP-event time. */
+pt_status pt_render_batch(pt_scene* scene, pt_film* film, int integrator, float* ms_out);
+/* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
+pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
+/* Counter control: when enabled, render kernels count rays, node visits and
+ * triangle tests (slower; used to price algorithmic bytes). */
+pt_status pt_counters_enable(pt_scene* scene, int enable);
+pt_status pt_counters_get(pt_scene* scene, pt_counters* out);
+pt_status pt_counters_reset(pt_scene* scene);
+
+/* ------------------------------------------------------------------------ */
+/* Probes of single hot-path stages on the device                             */
+/* ------------------------------------------------------------------------ */
+/* Sampler::get1D (samplers/sampler.cuh:25-29) for n (x, y, sample_index)
+ * triples over dims [0, num_dims); out_u32 gets the raw 32-bit value before
+ * the float conversion, out_f the float.  sampler 0 = PCG (sample_index is
+ * the frame/sample pair encoded as frame*65536+sample), 1 = Sobol. */
+pt_status pt_probe_sampler(int sampler, uint32_t seed, const int32_t* xy_idx, int n, int num_dims,
+                           uint32_t* out_u32, float* out_f);
+/* Closest-hit (intersectScene, accel/bvh_traversal.cuh:43-96) or any-hit
+ * (occludedScene, :98-133) for n rays {ox,oy,oz,dx,dy,dz,tmax,_} given as 8
+ * floats each, host buffers.  Directions are normalised as Ray's constructor
+ * does (core/ray.h:13).  t_out gets the hit distance (FLT_MAX on a miss; any-
+ * hit: 1 occluded / 0 not), face_out the ORIGINAL face index or -1,
+ * bary_out (may be NULL) 2 floats per ray. */
+pt_status pt_trace_rays(pt_scene* scene, const float* rays8, int n, int any_hit,
+                        float* t_out, int32_t* face_out, float* bary_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PT_AMD_H */

@@ -1,0 +1,883 @@
+// C ABI of libptamd (include/pt_amd.h): scene lifecycle, device upload,
+// rendering and probes.  Host stages live in csrc/host (plain C++), kernels in
+// csrc/device.  Errors are returned as status codes with a thread-local
+// message instead of the reference's print-and-exit (cuda/check.h:6-24).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device/bvh_build.h"
+#include "device/dev_scene.h"
+#include "device/render.h"
+#include "host/host_scene.h"
+#include "pt_amd.h"
+
+using ptamd::HostScene;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    pt_status code;
+    Error(pt_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(PT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+pt_status guarded(F&& f) {
+    try {
+        f();
+        return PT_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return PT_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PT_ERR_INVALID_ARG;
+    }
+}
+
+int g_device = -1;
+
+void ensure_device() {
+    if (g_device >= 0) return;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        throw Error(PT_ERR_DEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
+    hip_check(hipSetDevice(0), "hipSetDevice");
+    g_device = 0;
+}
+
+// Owning device allocation.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        reset();
+        if (count == 0) return;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) hip_check(hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+    }
+    void adopt(T* ptr, size_t count) {
+        reset();
+        p = ptr;
+        n = count;
+    }
+};
+
+int wrap_mode(int gl) {
+    if (gl == 33071) return 1;  // GL_CLAMP_TO_EDGE
+    if (gl == 33648) return 2;  // GL_MIRRORED_REPEAT
+    return 0;                   // GL_REPEAT and anything else (cuda_texture.cu:30-37)
+}
+
+}  // namespace
+
+struct pt_scene {
+    HostScene host;
+    ptamd::LightTables lights;
+    hipStream_t stream = nullptr;
+    bool accel_built = false, uploaded = false;
+    // geometry (set by build_accel)
+    DBuf<float4> vpos;
+    DBuf<uint4> faces_orig, faces_bvh;
+    DBuf<float4> tris, nodes;
+    DBuf<uint32_t> prim_order;
+    std::vector<uint32_t> prim_order_host;
+    ptd::BvhHostTopology topo;
+    uint32_t depth = 0, max_leaf = 0;
+    float build_ms = 0.0f;
+    // shading data (set by upload)
+    DBuf<float4> vnrm, vtan, texels, env;
+    DBuf<float2> vuv, tri_alias, env_alias;
+    DBuf<pt_material> materials;
+    DBuf<ptd::TexDesc> texdesc;
+    DBuf<uint4> emissive;
+    DBuf<float> tri_pmf, env_pmf;
+    DBuf<pt_punctual_light> plights;
+    int env_w = 0, env_h = 0, env_wrap_s = 0, env_wrap_t = 0;
+    // render state
+    DBuf<ptd::DevCounters> counters;
+    bool counting = false;
+    uint32_t frame_count = 0;
+
+    ~pt_scene() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct pt_film {
+    int w = 0, h = 0;
+    DBuf<float4> film_new, film_acc;
+};
+
+namespace {
+
+void validate_scene(const HostScene& s) {
+    const uint32_t nv = static_cast<uint32_t>(s.vertices.size());
+    const int nm = static_cast<int>(s.materials.size());
+    if (nm == 0) throw Error(PT_ERR_INVALID_ARG, "scene has no materials");
+    for (size_t i = 0; i < s.faces.size(); ++i) {
+        const pt_face& f = s.faces[i];
+        if (f.v0 >= nv || f.v1 >= nv || f.v2 >= nv)
+            throw Error(PT_ERR_INVALID_ARG, "face " + std::to_string(i) + " references a vertex out of range");
+        if (f.material_id < 0 || f.material_id >= nm)
+            throw Error(PT_ERR_INVALID_ARG, "face " + std::to_string(i) + " has material id " +
+                                                std::to_string(f.material_id) + " outside [0, " +
+                                                std::to_string(nm) + ")");
+    }
+    for (size_t i = 0; i < s.vertices.size(); ++i) {
+        const pt_vec3p& p = s.vertices[i].position;
+        if (!std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z))
+            throw Error(PT_ERR_INVALID_ARG, "vertex " + std::to_string(i) + " has a non-finite position");
+    }
+    for (size_t i = 0; i < s.textures.size(); ++i)
+        if (s.textures[i].empty() ||
+            s.textures[i].rgba.size() != static_cast<size_t>(s.textures[i].width) * s.textures[i].height * 4)
+            throw Error(PT_ERR_INVALID_ARG, "texture " + std::to_string(i) + " is empty or malformed");
+    if (s.camera.width <= 0 || s.camera.height <= 0 || s.camera.width > 65536 || s.camera.height > 65536)
+        throw Error(PT_ERR_INVALID_ARG, "camera resolution out of range");
+    if (s.settings.sample_per_frame <= 0) throw Error(PT_ERR_INVALID_ARG, "samplePerFrame must be positive");
+    if (s.settings.max_depth < 0) throw Error(PT_ERR_INVALID_ARG, "maxDepth must be >= 0");
+}
+
+// Host-only until a call needs the device (parsing and queries run without a GPU).
+pt_scene* new_scene() {
+    std::unique_ptr<pt_scene> sc(new pt_scene());
+    ptamd::init_scene_defaults(sc->host);
+    return sc.release();
+}
+
+void scene_device(pt_scene* sc) {
+    if (sc->stream) return;
+    ensure_device();
+    hip_check(hipStreamCreate(&sc->stream), "hipStreamCreate");
+    sc->counters.alloc(1);
+    hip_check(hipMemsetAsync(sc->counters.p, 0, sizeof(ptd::DevCounters), sc->stream), "hipMemset");
+    hip_check(hipStreamSynchronize(sc->stream), "init");
+}
+
+ptd::DevScene make_view(const pt_scene* s) {
+    ptd::DevScene v;
+    std::memset(&v, 0, sizeof(v));
+    v.nodes = s->nodes.p;
+    v.tris = s->tris.p;
+    v.faces = s->faces_bvh.p;
+    v.vpos = s->vpos.p;
+    v.vnrm = s->vnrm.p;
+    v.vuv = s->vuv.p;
+    v.vtan = s->vtan.p;
+    v.materials = s->materials.p;
+    v.textures = s->texdesc.p;
+    v.texels = s->texels.p;
+    v.emissive = s->emissive.p;
+    v.tri_pmf = s->tri_pmf.p;
+    v.tri_alias = s->tri_alias.p;
+    v.env_pmf = s->env_pmf.p;
+    v.env_alias = s->env_alias.p;
+    v.env = s->env.p;
+    v.lights = s->plights.p;
+    v.num_nodes = s->nodes.p ? static_cast<uint32_t>(s->nodes.n / 2) : 0;
+    v.num_faces = static_cast<uint32_t>(s->host.faces.size());
+    v.num_materials = static_cast<uint32_t>(s->host.materials.size());
+    v.num_textures = static_cast<uint32_t>(s->texdesc.n);
+    v.num_emissive = static_cast<uint32_t>(s->emissive.n);
+    v.num_env_texels = static_cast<uint32_t>(s->env_pmf.n);
+    v.num_lights = static_cast<uint32_t>(s->plights.n);
+    v.env_w = s->env_w;
+    v.env_h = s->env_h;
+    v.env_wrap_s = s->env_wrap_s;
+    v.env_wrap_t = s->env_wrap_t;
+    return v;
+}
+
+ptd::RenderParams make_params(const pt_scene* s) {
+    ptd::RenderParams p;
+    std::memset(&p, 0, sizeof(p));
+    pt_camera_frame f = ptamd::camera_frame(s->host.camera);
+    for (int i = 0; i < 3; ++i) {
+        p.cam_pos[i] = f.position[i];
+        p.cam_fwd[i] = f.forward[i];
+        p.cam_right[i] = f.right[i];
+        p.cam_up[i] = f.up[i];
+    }
+    p.tan_half_fov = f.tan_half_fov;
+    p.aspect = f.aspect;
+    p.aperture = f.aperture;
+    p.focal_distance = f.focal_distance;
+    p.width = s->host.camera.width;
+    p.height = s->host.camera.height;
+    p.spf = s->host.settings.sample_per_frame;
+    p.max_depth = s->host.settings.max_depth;
+    p.sampler = s->host.settings.sampler;
+    p.seed = s->host.settings.seed;
+    p.frame_count = s->frame_count;
+    return p;
+}
+
+// Depth-first (reference) node order from the build topology.
+void dfs_layout(const ptd::BvhHostTopology& t, std::vector<float>& out) {
+    out.clear();
+    if (t.num_nodes == 0) return;
+    struct Item {
+        uint32_t node, parent_slot;
+        int which;
+    };
+    std::vector<Item> st;
+    st.push_back({0, 0xFFFFFFFFu, 0});
+    std::vector<uint32_t> words;
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        uint32_t slot = static_cast<uint32_t>(out.size() / 8);
+        if (it.parent_slot != 0xFFFFFFFFu) {
+            uint32_t w = slot;
+            std::memcpy(&out[it.parent_slot * 8 + (it.which == 0 ? 3 : 7)], &w, 4);
+        }
+        out.resize(out.size() + 8);
+        float* o = &out[slot * 8];
+        uint32_t k = it.node;
+        o[0] = t.lo[3 * k]; o[1] = t.lo[3 * k + 1]; o[2] = t.lo[3 * k + 2];
+        o[4] = t.hi[3 * k]; o[5] = t.hi[3 * k + 1]; o[6] = t.hi[3 * k + 2];
+        if (t.left[k] == 0) {
+            uint32_t d0 = 0x80000000u | t.start[k], d1 = t.count[k];
+            std::memcpy(&o[3], &d0, 4);
+            std::memcpy(&o[7], &d1, 4);
+        } else {
+            // push right first so the left subtree is laid out next (left = slot + 1)
+            st.push_back({t.left[k] + 1, slot, 1});
+            st.push_back({t.left[k], slot, 0});
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int pt_abi_version(void) { return PT_ABI_VERSION; }
+
+const char* pt_last_error(void) { return g_err.c_str(); }
+
+pt_status pt_init(int device) {
+    return guarded([&] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0)
+            throw Error(PT_ERR_DEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
+        if (device < 0 || device >= n) throw Error(PT_ERR_INVALID_ARG, "device index out of range");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        g_device = device;
+    });
+}
+
+pt_status pt_struct_sizes(uint32_t* out, int n) {
+    return guarded([&] {
+        if (!out || n < 8) throw Error(PT_ERR_INVALID_ARG, "need room for 8 sizes");
+        out[0] = sizeof(pt_vertex);
+        out[1] = sizeof(pt_face);
+        out[2] = sizeof(pt_material);
+        out[3] = sizeof(pt_punctual_light);
+        out[4] = sizeof(pt_camera);
+        out[5] = sizeof(pt_camera_frame);
+        out[6] = sizeof(pt_host_scene_desc);
+        out[7] = sizeof(pt_counters);
+    });
+}
+
+pt_status pt_scene_load(const char* path, pt_scene** out) {
+    return guarded([&] {
+        if (!path || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *out = nullptr;
+        std::string p(path);
+        std::string ext = p.size() >= 5 ? p.substr(p.size() - 5) : p;
+        for (auto& c : ext) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+        if (ext != ".json") {
+            if (ext == ".gltf" || (p.size() >= 4 && p.substr(p.size() - 4) == ".glb"))
+                throw Error(PT_ERR_UNSUPPORTED, "glTF scenes are not supported by this build: " + p);
+            throw Error(PT_ERR_UNSUPPORTED, "unknown scene format: " + p);
+        }
+        std::unique_ptr<pt_scene> sc(new_scene());
+        std::string err;
+        if (!ptamd::load_scene_json(p, sc->host, &err)) {
+            pt_status code = PT_ERR_PARSE;
+            if (err.rfind("cannot open", 0) == 0) code = PT_ERR_IO;
+            throw Error(code, err);
+        }
+        validate_scene(sc->host);
+        *out = sc.release();
+    });
+}
+
+pt_status pt_scene_create(const pt_host_scene_desc* d, pt_scene** out) {
+    return guarded([&] {
+        if (!d || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *out = nullptr;
+        std::unique_ptr<pt_scene> sc(new_scene());
+        HostScene& h = sc->host;
+        if (d->num_vertices && !d->vertices) throw Error(PT_ERR_INVALID_ARG, "vertices is null");
+        if (d->num_faces && !d->faces) throw Error(PT_ERR_INVALID_ARG, "faces is null");
+        h.vertices.assign(d->vertices, d->vertices + d->num_vertices);
+        h.faces.assign(d->faces, d->faces + d->num_faces);
+        if (d->num_materials && d->materials) h.materials.assign(d->materials, d->materials + d->num_materials);
+        if (h.materials.empty()) h.materials.push_back(ptamd::default_material());
+        for (uint32_t i = 0; i < d->num_textures; ++i) {
+            const pt_texture_desc& t = d->textures[i];
+            if (!t.rgba || t.width <= 0 || t.height <= 0) throw Error(PT_ERR_INVALID_ARG, "bad texture descriptor");
+            ptamd::Image img;
+            img.width = t.width;
+            img.height = t.height;
+            img.wrap_s = t.wrap_s;
+            img.wrap_t = t.wrap_t;
+            img.linear = t.linear;
+            img.rgba.assign(t.rgba, t.rgba + static_cast<size_t>(t.width) * t.height * 4);
+            h.textures.push_back(std::move(img));
+        }
+        if (d->num_lights && d->lights) h.lights.assign(d->lights, d->lights + d->num_lights);
+        if (d->camera) h.camera = *d->camera;
+        else ptamd::auto_fit_camera(h.vertices, h.camera);
+        if (d->env_rgba && d->env_width > 0 && d->env_height > 0) {
+            h.env.width = d->env_width;
+            h.env.height = d->env_height;
+            h.env.rgba.assign(d->env_rgba, d->env_rgba + static_cast<size_t>(d->env_width) * d->env_height * 4);
+        }
+        if (d->settings) h.settings = *d->settings;
+        validate_scene(h);
+        *out = sc.release();
+    });
+}
+
+pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out) {
+    return guarded([&] {
+        if (!out) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *out = nullptr;
+        std::unique_ptr<pt_scene> sc(new_scene());
+        if (which == 0) ptamd::generate_cornell(sc->host);
+        else if (which == 1) ptamd::generate_spheres(sc->host, seed, 64, 5, 2048, 1024);
+        else if (which == 2) ptamd::generate_spheres(sc->host, seed, 8, 3, 256, 128);  // small variant for tests
+        else throw Error(PT_ERR_INVALID_ARG, "unknown workload id " + std::to_string(which));
+        validate_scene(sc->host);
+        *out = sc.release();
+    });
+}
+
+void pt_scene_destroy(pt_scene* scene) { delete scene; }
+
+pt_status pt_scene_get_info(const pt_scene* s, pt_scene_info* info) {
+    return guarded([&] {
+        if (!s || !info) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->num_vertices = static_cast<uint32_t>(s->host.vertices.size());
+        info->num_faces = static_cast<uint32_t>(s->host.faces.size());
+        info->num_materials = static_cast<uint32_t>(s->host.materials.size());
+        info->num_textures = static_cast<uint32_t>(s->host.textures.size());
+        info->num_lights = static_cast<uint32_t>(s->host.lights.size());
+        info->num_emissive_triangles = static_cast<uint32_t>(s->lights.emissive.size());
+        info->num_bvh_nodes = s->nodes.p ? static_cast<uint32_t>(s->nodes.n / 2) : 0;
+        info->bvh_depth = s->depth;
+        info->max_leaf_size = s->max_leaf;
+        info->env_width = s->host.env.width;
+        info->env_height = s->host.env.height;
+        info->width = s->host.camera.width;
+        info->height = s->host.camera.height;
+        info->sample_per_frame = s->host.settings.sample_per_frame;
+        info->target_samples = s->host.settings.target_samples;
+        info->max_depth = s->host.settings.max_depth;
+        info->sampler = s->host.settings.sampler;
+        info->seed = s->host.settings.seed;
+        info->frame_count = s->frame_count;
+        info->bvh_build_ms = s->build_ms;
+    });
+}
+
+pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        scene_device(s);
+        pt_bvh_options o = opts ? *opts : (s->host.has_bvh_opts ? s->host.bvh_opts : pt_bvh_options{24, 4, 0, 0});
+        if (o.bins == 0) o.bins = 24;
+        if (o.leaf_threshold == 0) o.leaf_threshold = 4;
+        if (o.builder != 0) throw Error(PT_ERR_UNSUPPORTED, "only the GPU builder (0) is available");
+        if (o.bins < 2 || o.bins > 32) throw Error(PT_ERR_INVALID_ARG, "bins must be in [2, 32]");
+        if (o.leaf_threshold < 1 || o.max_leaf < 0) throw Error(PT_ERR_INVALID_ARG, "bad leaf options");
+        HostScene& h = s->host;
+        const uint32_t nv = static_cast<uint32_t>(h.vertices.size()), nf = static_cast<uint32_t>(h.faces.size());
+        std::vector<float4> pos(nv);
+        for (uint32_t i = 0; i < nv; ++i)
+            pos[i] = make_float4(h.vertices[i].position.x, h.vertices[i].position.y, h.vertices[i].position.z, 0.0f);
+        std::vector<uint4> fo(nf);
+        for (uint32_t i = 0; i < nf; ++i)
+            fo[i] = make_uint4(h.faces[i].v0, h.faces[i].v1, h.faces[i].v2, static_cast<uint32_t>(h.faces[i].material_id));
+        s->accel_built = s->uploaded = false;
+        s->vpos.upload(pos.data(), nv, s->stream);
+        s->faces_orig.upload(fo.data(), nf, s->stream);
+        s->nodes.reset();
+        s->tris.reset();
+        s->faces_bvh.reset();
+        s->prim_order.reset();
+        s->topo = ptd::BvhHostTopology();
+        s->depth = s->max_leaf = 0;
+        s->build_ms = 0.0f;
+        if (nf > 0) {
+            ptd::BvhBuildOptions bo;
+            bo.bins = o.bins;
+            bo.leaf_threshold = o.leaf_threshold;
+            bo.max_leaf = o.max_leaf;
+            ptd::BvhDeviceResult r;
+            std::string err;
+            if (!ptd::gpu_build_bvh(s->vpos.p, s->faces_orig.p, nf, bo, s->stream, &r, &s->topo, &err))
+                throw Error(err.find("supports at most") != std::string::npos ? PT_ERR_LIMIT : PT_ERR_DEVICE, err);
+            s->nodes.adopt(r.nodes, static_cast<size_t>(r.num_nodes) * 2);
+            s->prim_order.adopt(r.prim_order, nf);
+            s->depth = r.depth;
+            s->max_leaf = r.max_leaf;
+            s->build_ms = r.build_ms;
+            s->tris.alloc(static_cast<size_t>(nf) * 3);
+            s->faces_bvh.alloc(nf);
+            hip_check(ptd::launch_emit_tris(s->vpos.p, s->faces_orig.p, s->prim_order.p, s->nodes.p, r.num_nodes, nf,
+                                            s->tris.p, s->faces_bvh.p, s->stream),
+                      "emit triangles");
+            s->prim_order_host = s->topo.prim_order;
+        } else {
+            s->prim_order_host.clear();
+        }
+        hip_check(hipStreamSynchronize(s->stream), "build accel");
+        // light tables over the BVH-ordered faces (scene_manager.cpp:31-34)
+        std::vector<pt_face> faces_bvh(nf);
+        for (uint32_t i = 0; i < nf; ++i) faces_bvh[i] = h.faces[s->prim_order_host[i]];
+        ptamd::build_emissive_tables(h.vertices, faces_bvh, h.materials, s->lights);
+        ptamd::build_env_tables(h.env, s->lights);
+        s->accel_built = true;
+    });
+}
+
+pt_status pt_scene_upload(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->accel_built) throw Error(PT_ERR_STATE, "pt_scene_build_accel must run before pt_scene_upload");
+        const HostScene& h = s->host;
+        const size_t nv = h.vertices.size();
+        std::vector<float4> nrm(nv), tan;
+        std::vector<float2> uv(nv);
+        for (size_t i = 0; i < nv; ++i) {
+            const pt_vertex& v = h.vertices[i];
+            nrm[i] = make_float4(v.normal.x, v.normal.y, v.normal.z, 0.0f);
+            uv[i] = make_float2(v.u, v.v);
+        }
+        bool need_tan = false;
+        for (const auto& m : h.materials) need_tan = need_tan || (m.normal_tex_id >= 0);
+        s->vnrm.upload(nrm.data(), nv, s->stream);
+        s->vuv.upload(uv.data(), nv, s->stream);
+        if (need_tan) {
+            tan.resize(nv);
+            for (size_t i = 0; i < nv; ++i)
+                tan[i] = make_float4(h.vertices[i].tangent[0], h.vertices[i].tangent[1], h.vertices[i].tangent[2],
+                                     h.vertices[i].tangent[3]);
+            s->vtan.upload(tan.data(), nv, s->stream);
+        } else {
+            s->vtan.reset();
+        }
+        s->materials.upload(h.materials.data(), h.materials.size(), s->stream);
+        // textures: one texel pool + descriptors
+        std::vector<ptd::TexDesc> td;
+        size_t total = 0;
+        for (const auto& t : h.textures) total += static_cast<size_t>(t.width) * t.height;
+        if (total > 0xFFFFFFFFull) throw Error(PT_ERR_LIMIT, "texture pool larger than 2^32 texels");
+        std::vector<float4> pool(total);
+        size_t off = 0;
+        for (const auto& t : h.textures) {
+            ptd::TexDesc d;
+            std::memset(&d, 0, sizeof(d));
+            d.offset = static_cast<uint32_t>(off);
+            d.w = t.width;
+            d.h = t.height;
+            d.wrap_s = wrap_mode(t.wrap_s);
+            d.wrap_t = wrap_mode(t.wrap_t);
+            d.linear = t.linear;
+            td.push_back(d);
+            std::memcpy(&pool[off], t.rgba.data(), sizeof(float4) * static_cast<size_t>(t.width) * t.height);
+            off += static_cast<size_t>(t.width) * t.height;
+        }
+        s->texdesc.upload(td.data(), td.size(), s->stream);
+        s->texels.upload(pool.data(), pool.size(), s->stream);
+        // environment (a 4-channel image, device_scene.cu:19-27)
+        if (!h.env.empty()) {
+            s->env.upload(reinterpret_cast<const float4*>(h.env.rgba.data()),
+                          static_cast<size_t>(h.env.width) * h.env.height, s->stream);
+            s->env_w = h.env.width;
+            s->env_h = h.env.height;
+            s->env_wrap_s = wrap_mode(h.env.wrap_s);
+            s->env_wrap_t = wrap_mode(h.env.wrap_t);
+        } else {
+            s->env.reset();
+            s->env_w = s->env_h = 0;
+        }
+        const ptamd::LightTables& L = s->lights;
+        std::vector<uint4> em(L.emissive.size());
+        for (size_t i = 0; i < em.size(); ++i) {
+            uint32_t pmf, cdf;
+            std::memcpy(&pmf, &L.emissive[i].pmf, 4);
+            std::memcpy(&cdf, &L.emissive[i].cdf, 4);
+            em[i] = make_uint4(L.emissive[i].tri, pmf, cdf, 0u);
+        }
+        s->emissive.upload(em.data(), em.size(), s->stream);
+        s->tri_pmf.upload(L.tri_pmf.data(), L.tri_pmf.size(), s->stream);
+        s->tri_alias.upload(reinterpret_cast<const float2*>(L.tri_alias.data()), L.tri_alias.size(), s->stream);
+        s->env_pmf.upload(L.env_pmf.data(), L.env_pmf.size(), s->stream);
+        s->env_alias.upload(reinterpret_cast<const float2*>(L.env_alias.data()), L.env_alias.size(), s->stream);
+        if (L.env_pmf.empty()) s->env_pmf.reset(), s->env_alias.reset();
+        s->plights.upload(h.lights.data(), h.lights.size(), s->stream);
+        hip_check(hipStreamSynchronize(s->stream), "upload");
+        s->frame_count = 0;
+        s->uploaded = true;
+    });
+}
+
+pt_status pt_scene_set_spp(pt_scene* s, int spp) {
+    return guarded([&] {
+        if (!s || spp <= 0) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        s->host.settings.target_samples = spp;
+    });
+}
+pt_status pt_scene_set_sample_per_frame(pt_scene* s, int spf) {
+    return guarded([&] {
+        if (!s || spf <= 0) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        s->host.settings.sample_per_frame = spf;
+    });
+}
+pt_status pt_scene_set_max_depth(pt_scene* s, int depth) {
+    return guarded([&] {
+        if (!s || depth < 0) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        s->host.settings.max_depth = depth;
+    });
+}
+pt_status pt_scene_set_sampler(pt_scene* s, int sampler) {
+    return guarded([&] {
+        if (!s || (sampler != 0 && sampler != 1)) throw Error(PT_ERR_INVALID_ARG, "sampler must be 0 (pcg) or 1 (sobol)");
+        s->host.settings.sampler = sampler;
+    });
+}
+pt_status pt_scene_set_seed(pt_scene* s, uint32_t seed) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->host.settings.seed = seed;
+    });
+}
+pt_status pt_scene_set_camera(pt_scene* s, const pt_camera* cam) {
+    return guarded([&] {
+        if (!s || !cam) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (cam->width <= 0 || cam->height <= 0) throw Error(PT_ERR_INVALID_ARG, "camera resolution must be positive");
+        s->host.camera = *cam;
+    });
+}
+pt_status pt_scene_get_camera(const pt_scene* s, pt_camera* cam) {
+    return guarded([&] {
+        if (!s || !cam) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *cam = s->host.camera;
+    });
+}
+pt_status pt_scene_camera_frame(const pt_scene* s, pt_camera_frame* f) {
+    return guarded([&] {
+        if (!s || !f) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *f = ptamd::camera_frame(s->host.camera);
+    });
+}
+
+pt_status pt_scene_host_view(const pt_scene* s, pt_host_scene_view* o) {
+    return guarded([&] {
+        if (!s || !o) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        std::memset(o, 0, sizeof(*o));
+        const HostScene& h = s->host;
+        o->vertices = h.vertices.data();
+        o->num_vertices = static_cast<uint32_t>(h.vertices.size());
+        o->faces = h.faces.data();
+        o->num_faces = static_cast<uint32_t>(h.faces.size());
+        o->materials = h.materials.data();
+        o->num_materials = static_cast<uint32_t>(h.materials.size());
+        o->lights = h.lights.data();
+        o->num_lights = static_cast<uint32_t>(h.lights.size());
+        o->env_rgba = h.env.empty() ? nullptr : h.env.rgba.data();
+        o->env_width = h.env.width;
+        o->env_height = h.env.height;
+        o->num_textures = static_cast<uint32_t>(h.textures.size());
+        o->camera = h.camera;
+        o->settings = h.settings;
+    });
+}
+
+pt_status pt_scene_texture(const pt_scene* s, uint32_t i, pt_texture_desc* o) {
+    return guarded([&] {
+        if (!s || !o) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (i >= s->host.textures.size()) throw Error(PT_ERR_INVALID_ARG, "texture index out of range");
+        const ptamd::Image& t = s->host.textures[i];
+        std::memset(o, 0, sizeof(*o));
+        o->rgba = t.rgba.data();
+        o->width = t.width;
+        o->height = t.height;
+        o->wrap_s = t.wrap_s;
+        o->wrap_t = t.wrap_t;
+        o->linear = t.linear;
+    });
+}
+
+pt_status pt_scene_set_texture_data(pt_scene* s, uint32_t i, const float* rgba, size_t count) {
+    return guarded([&] {
+        if (!s || !rgba) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        ptamd::Image* t = nullptr;
+        if (i == 0xFFFFFFFFu) t = &s->host.env;
+        else if (i < s->host.textures.size()) t = &s->host.textures[i];
+        else throw Error(PT_ERR_INVALID_ARG, "texture index out of range");
+        if (t->empty() || count != t->rgba.size()) throw Error(PT_ERR_INVALID_ARG, "texel count does not match");
+        std::memcpy(t->rgba.data(), rgba, count * sizeof(float));
+        s->uploaded = false;
+    });
+}
+
+pt_status pt_scene_get_bvh(const pt_scene* s, int layout, float* nodes8, uint32_t* num_nodes, uint32_t* face_order,
+                           uint32_t* num_faces) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->accel_built) throw Error(PT_ERR_STATE, "acceleration structure not built");
+        std::vector<float> out;
+        if (layout == 0) {
+            dfs_layout(s->topo, out);
+        } else if (layout == 1) {
+            out.resize(s->nodes.n * 4);
+            if (s->nodes.n)
+                hip_check(hipMemcpy(out.data(), s->nodes.p, sizeof(float4) * s->nodes.n, hipMemcpyDeviceToHost),
+                          "download nodes");
+        } else {
+            throw Error(PT_ERR_INVALID_ARG, "layout must be 0 or 1");
+        }
+        uint32_t nn = static_cast<uint32_t>(out.size() / 8);
+        if (nodes8) {
+            if (!num_nodes || *num_nodes < nn) throw Error(PT_ERR_INVALID_ARG, "node buffer too small");
+            std::memcpy(nodes8, out.data(), out.size() * sizeof(float));
+        }
+        if (num_nodes) *num_nodes = nn;
+        uint32_t nf = static_cast<uint32_t>(s->prim_order_host.size());
+        if (face_order) {
+            if (!num_faces || *num_faces < nf) throw Error(PT_ERR_INVALID_ARG, "face buffer too small");
+            std::memcpy(face_order, s->prim_order_host.data(), nf * sizeof(uint32_t));
+        }
+        if (num_faces) *num_faces = nf;
+    });
+}
+
+pt_status pt_scene_get_light_tables(const pt_scene* s, float* tri_pmf, uint32_t* n_tri_pmf, float* tri_alias,
+                                    uint32_t* n_tri_alias, float* env_pmf, float* env_alias, uint32_t* n_env) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->accel_built) throw Error(PT_ERR_STATE, "light tables not built");
+        const ptamd::LightTables& L = s->lights;
+        if (tri_pmf) {
+            if (!n_tri_pmf || *n_tri_pmf < L.tri_pmf.size()) throw Error(PT_ERR_INVALID_ARG, "tri_pmf too small");
+            std::memcpy(tri_pmf, L.tri_pmf.data(), L.tri_pmf.size() * 4);
+        }
+        if (n_tri_pmf) *n_tri_pmf = static_cast<uint32_t>(L.tri_pmf.size());
+        if (tri_alias) {
+            if (!n_tri_alias || *n_tri_alias < L.tri_alias.size()) throw Error(PT_ERR_INVALID_ARG, "tri_alias too small");
+            std::memcpy(tri_alias, L.tri_alias.data(), L.tri_alias.size() * 8);
+        }
+        if (n_tri_alias) *n_tri_alias = static_cast<uint32_t>(L.tri_alias.size());
+        if (env_pmf || env_alias) {
+            if (!n_env || *n_env < L.env_pmf.size()) throw Error(PT_ERR_INVALID_ARG, "env buffers too small");
+            if (env_pmf) std::memcpy(env_pmf, L.env_pmf.data(), L.env_pmf.size() * 4);
+            if (env_alias) std::memcpy(env_alias, L.env_alias.data(), L.env_alias.size() * 8);
+        }
+        if (n_env) *n_env = static_cast<uint32_t>(L.env_pmf.size());
+    });
+}
+
+pt_status pt_film_create(int width, int height, pt_film** out) {
+    return guarded([&] {
+        if (!out || width <= 0 || height <= 0 || width > 65536 || height > 65536)
+            throw Error(PT_ERR_INVALID_ARG, "bad film size");
+        ensure_device();
+        std::unique_ptr<pt_film> f(new pt_film());
+        f->w = width;
+        f->h = height;
+        size_t n = static_cast<size_t>(width) * height;
+        f->film_new.alloc(n);
+        f->film_acc.alloc(n);
+        hip_check(hipMemset(f->film_new.p, 0, n * sizeof(float4)), "hipMemset");
+        hip_check(hipMemset(f->film_acc.p, 0, n * sizeof(float4)), "hipMemset");
+        *out = f.release();
+    });
+}
+
+void pt_film_destroy(pt_film* film) { delete film; }
+
+pt_status pt_film_reset(pt_film* f, pt_scene* s) {
+    return guarded([&] {
+        if (!f) throw Error(PT_ERR_INVALID_ARG, "null film");
+        size_t n = static_cast<size_t>(f->w) * f->h;
+        hip_check(hipMemset(f->film_new.p, 0, n * sizeof(float4)), "hipMemset");
+        hip_check(hipMemset(f->film_acc.p, 0, n * sizeof(float4)), "hipMemset");
+        hip_check(hipDeviceSynchronize(), "film reset");
+        if (s) s->frame_count = 0;
+    });
+}
+
+pt_status pt_film_download(const pt_film* f, int which, float* rgba) {
+    return guarded([&] {
+        if (!f || !rgba) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        const float4* src = which == 1 ? f->film_new.p : f->film_acc.p;
+        hip_check(hipMemcpy(rgba, src, sizeof(float4) * static_cast<size_t>(f->w) * f->h, hipMemcpyDeviceToHost),
+                  "film download");
+    });
+}
+
+pt_status pt_film_device_ptr(const pt_film* f, int which, void** p) {
+    return guarded([&] {
+        if (!f || !p) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *p = which == 1 ? static_cast<void*>(f->film_new.p) : static_cast<void*>(f->film_acc.p);
+    });
+}
+
+pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out) {
+    return guarded([&] {
+        if (!s || !f) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        if (f->w != s->host.camera.width || f->h != s->host.camera.height)
+            throw Error(PT_ERR_INVALID_ARG, "film size does not match the camera resolution");
+        if (integrator != PT_INTEGRATOR_PATH) throw Error(PT_ERR_UNSUPPORTED, "integrator not available");
+        ptd::DevScene v = make_view(s);
+        ptd::RenderParams p = make_params(s);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
+        hipError_t le = ptd::launch_render_plain(v, p, f->film_new.p, f->film_acc.p, s->counters.p, s->counting,
+                                                 s->stream);
+        hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
+        hipError_t se = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hip_check(le, "render launch");
+        hip_check(se, "render");
+        if (ms_out) *ms_out = ms;
+        s->frame_count++;
+    });
+}
+
+pt_status pt_render_all(pt_scene* s, pt_film* f, int integrator, float* total_ms) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        float total = 0.0f;
+        while (static_cast<int64_t>(s->frame_count) * s->host.settings.sample_per_frame <
+               s->host.settings.target_samples) {
+            float ms = 0.0f;
+            pt_status st = pt_render_batch(s, f, integrator, &ms);
+            if (st != PT_OK) throw Error(st, g_err);
+            total += ms;
+        }
+        if (total_ms) *total_ms = total;
+    });
+}
+
+pt_status pt_counters_enable(pt_scene* s, int enable) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->counting = enable != 0;
+    });
+}
+
+pt_status pt_counters_get(pt_scene* s, pt_counters* out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        scene_device(s);
+        ptd::DevCounters c;
+        hip_check(hipMemcpy(&c, s->counters.p, sizeof(c), hipMemcpyDeviceToHost), "counter download");
+        out->closest_rays = c.closest_rays;
+        out->shadow_rays = c.shadow_rays;
+        out->node_visits = c.node_visits;
+        out->tri_tests = c.tri_tests;
+        out->paths = c.paths;
+        out->stack_overflows = c.stack_overflows;
+    });
+}
+
+pt_status pt_counters_reset(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        scene_device(s);
+        hip_check(hipMemset(s->counters.p, 0, sizeof(ptd::DevCounters)), "hipMemset");
+        hip_check(hipDeviceSynchronize(), "counter reset");
+    });
+}
+
+pt_status pt_probe_sampler(int sampler, uint32_t seed, const int32_t* xyi, int n, int dims, uint32_t* out_u,
+                           float* out_f) {
+    return guarded([&] {
+        if (n < 0 || dims <= 0 || (n > 0 && (!xyi || !out_u || !out_f)) || (sampler != 0 && sampler != 1))
+            throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        if (n == 0) return;
+        ensure_device();
+        DBuf<int32_t> d_xyi;
+        DBuf<uint32_t> d_u;
+        DBuf<float> d_f;
+        d_xyi.upload(xyi, static_cast<size_t>(n) * 3, nullptr);
+        d_u.alloc(static_cast<size_t>(n) * dims);
+        d_f.alloc(static_cast<size_t>(n) * dims);
+        hip_check(ptd::launch_probe_sampler(sampler, seed, d_xyi.p, n, dims, d_u.p, d_f.p, nullptr), "probe launch");
+        hip_check(hipMemcpy(out_u, d_u.p, 4 * d_u.n, hipMemcpyDeviceToHost), "probe download");
+        hip_check(hipMemcpy(out_f, d_f.p, 4 * d_f.n, hipMemcpyDeviceToHost), "probe download");
+    });
+}
+
+pt_status pt_trace_rays(pt_scene* s, const float* rays8, int n, int any_hit, float* t_out, int32_t* face_out,
+                        float* bary_out) {
+    return guarded([&] {
+        if (!s || n < 0 || (n > 0 && (!rays8 || !t_out || !face_out))) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        if (!s->accel_built) throw Error(PT_ERR_STATE, "acceleration structure not built");
+        if (n == 0) return;
+        ptd::DevScene v = make_view(s);
+        DBuf<float> d_r, d_t, d_b;
+        DBuf<int32_t> d_face;
+        d_r.upload(rays8, static_cast<size_t>(n) * 8, s->stream);
+        d_t.alloc(n);
+        d_face.alloc(n);
+        if (bary_out) d_b.alloc(static_cast<size_t>(n) * 2);
+        hip_check(ptd::launch_trace_rays(v, d_r.p, n, any_hit, d_t.p, d_face.p, d_b.p, s->stream), "trace launch");
+        hip_check(hipStreamSynchronize(s->stream), "trace");
+        hip_check(hipMemcpy(t_out, d_t.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost), "download");
+        hip_check(hipMemcpy(face_out, d_face.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost), "download");
+        if (bary_out) hip_check(hipMemcpy(bary_out, d_b.p, 8 * static_cast<size_t>(n), hipMemcpyDeviceToHost), "download");
+    });
+}
+
+}  // extern "C"

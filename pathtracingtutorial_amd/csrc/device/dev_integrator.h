@@ -1,0 +1,419 @@
+// Unidirectional path tracer with next-event estimation and MIS.
+//
+// Follows Part_2/src/integrators/path.cu:31-206 and direct_lighting.cuh:
+// camera ray with jitter (and thin lens), closest hit, environment on a miss
+// (radiance = bilinear texel * 2 pi, MIS against the env pdf after a
+// non-delta bounce), emitter hit ends the path (MIS against the triangle
+// pdf), alpha mask / stochastic blend, NEE from an alias-sampled emissive
+// triangle and an alias-sampled env texel (both when both exist), punctual
+// lights without MIS, Disney BSDF sampling, offset 1e-4 along the geometric
+// normal, Beer-Lambert, Russian roulette after depth 2 with p = min(max T,
+// 0.95), and a per-channel clamp of the path radiance at 100.
+#pragma once
+
+#include "dev_bsdf.h"
+#include "dev_traverse.h"
+
+namespace ptd {
+
+struct TraceCtx {
+    uint32_t* stk;
+    int stride;
+    TraceCounters cnt;
+    uint32_t overflow;
+    uint32_t closest, shadow;
+};
+
+// ---- environment --------------------------------------------------------
+// envmap.h:15-24
+__device__ __forceinline__ void dir_to_uv(F3 d, float& u, float& v) {
+    u = atan2f(d.z, d.x);
+    v = asinf(d.y);
+    u *= kInv2Pi;
+    v *= kInvPi;
+    u += 0.5f;
+    v += 0.5f;
+    v = 1.0f - v;
+}
+// envmap.h:27-32
+__device__ __forceinline__ F3 uv_to_dir(float u, float v) {
+    float phi = (u - 0.5f) * kTwoPi;
+    float theta = v * kPi;
+    float st = sinf(theta);
+    return normalize(f3(cosf(phi) * st, cosf(theta), sinf(phi) * st));
+}
+__device__ __forceinline__ bool has_env_texture(const DevScene& s) { return s.env != nullptr; }
+// sampleEnvironmentRadiance (envmap.h:34-45): bilinear lookup times 2 pi.
+__device__ __forceinline__ F3 env_radiance(const DevScene& s, F3 d) {
+    if (s.env == nullptr) return f3(0.0f);
+    float u, v;
+    dir_to_uv(d, u, v);
+    float4 c = tex_bilinear(s.env, s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
+    return f3(c.x, c.y, c.z) * kTwoPi;
+}
+// environmentPdf (light_pdf.cuh:12-27)
+__device__ __forceinline__ float env_pdf(const DevScene& s, F3 d) {
+    if (s.env == nullptr || s.env_pmf == nullptr || s.num_env_texels == 0 || s.env_w <= 0 || s.env_h <= 0) return 0.0f;
+    float u, v;
+    dir_to_uv(d, u, v);
+    int x = static_cast<int>(clampf(u * s.env_w, 0.0f, static_cast<float>(s.env_w - 1)));
+    int y = static_cast<int>(clampf(v * s.env_h, 0.0f, static_cast<float>(s.env_h - 1)));
+    float st = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
+    if (st <= 1e-5f) return 0.0f;
+    int idx = y * s.env_w + x;
+    return s.env_pmf[idx] * static_cast<float>(s.env_w * s.env_h) / (kTwoPi * kPi * st);
+}
+
+// ---- lights ---------------------------------------------------------------
+struct LightSample {
+    F3 wi, radiance;
+    float pdf;
+    bool valid;
+};
+
+// direct_lighting.cuh:55-59
+__device__ __forceinline__ uint32_t sample_alias(const float2* table, uint32_t n, float u1, float u2) {
+    uint32_t idx = static_cast<uint32_t>(u1 * n);
+    if (idx >= n) idx = n - 1;
+    float2 e = table[idx];
+    return u2 < e.x ? idx : __float_as_uint(e.y);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene& s, F3 p, F3 n, F3 dir, float max_dist, TraceCtx& c) {
+    // direct_lighting.cuh:36-51
+    float sign = dot(dir, n) >= 0.0f ? 1.0f : -1.0f;
+    F3 o = p + n * (1e-4f * sign);
+    F3 d = normalize(dir);
+    float tmax = (max_dist < kInf) ? (max_dist - 1e-3f) : kInf;
+    c.shadow++;
+    return trace_any<COUNT>(s, o, d, tmax, c.stk, c.stride, c.cnt, c.overflow);
+}
+
+// sampleAreaLight (direct_lighting.cuh:75-121)
+template <bool COUNT>
+__device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+    LightSample ls;
+    ls.valid = false;
+    if (s.num_emissive == 0 || s.emissive == nullptr) return ls;
+    float a1 = rng.next();
+    float a2 = rng.next();
+    uint4 ref = s.emissive[sample_alias(s.tri_alias, s.num_emissive, a1, a2)];
+    uint4 face = s.faces[ref.x];
+    float4 q0 = s.vpos[face.x], q1 = s.vpos[face.y], q2 = s.vpos[face.z];
+    F3 p0 = f3(q0), p1 = f3(q1), p2 = f3(q2);
+    float t1 = rng.next();
+    float t2 = rng.next();
+    // light_sampler.h:58-71
+    float su0 = sqrtf(t1);
+    float b0 = 1.0f - su0, b1 = t2 * su0, b2 = 1.0f - b0 - b1;
+    F3 lp = p0 * b0 + p1 * b1 + p2 * b2;
+    F3 ln = normalize(f3(s.vnrm[face.x]) * b0 + f3(s.vnrm[face.y]) * b1 + f3(s.vnrm[face.z]) * b2);
+    F3 e1 = p1 - p0, e2 = p2 - p0;
+    if (length(ln) == 0.0f) ln = normalize(cross(e1, e2));
+    F3 tl = lp - p;
+    float d2 = dot(tl, tl);
+    if (d2 <= 1e-8f) return ls;
+    float dist = sqrtf(d2);
+    F3 wi = tl / dist;
+    float cs = fmaxf(0.0f, dot(n, wi));
+    if (cs <= 0.0f) return ls;
+    float area = 0.5f * length(cross(e1, e2));
+    if (area <= 0.0f) return ls;
+    float cl = fabsf(dot(ln, -wi));
+    if (cl <= 1e-5f) return ls;
+    float pdf = (__uint_as_float(ref.y) / area) * d2 / cl;
+    if (pdf <= 0.0f) return ls;
+    if (occluded<COUNT>(s, p, n, wi, dist, c)) return ls;
+    const pt_material& lm = s.materials[face.w];
+    ls.wi = wi;
+    ls.radiance = f3(lm.emissive.x, lm.emissive.y, lm.emissive.z) * lm.emissive_strength;
+    ls.pdf = pdf;
+    ls.valid = true;
+    return ls;
+}
+
+// sampleEnvironmentLight (direct_lighting.cuh:123-154)
+template <bool COUNT>
+__device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+    LightSample ls;
+    ls.valid = false;
+    float a1 = rng.next();
+    float a2 = rng.next();
+    uint32_t texel = sample_alias(s.env_alias, s.num_env_texels, a1, a2);
+    int x = static_cast<int>(texel % static_cast<uint32_t>(s.env_w));
+    int y = static_cast<int>(texel / static_cast<uint32_t>(s.env_w));
+    float j1 = rng.next();
+    float j2 = rng.next();
+    float u = (static_cast<float>(x) + j1) / static_cast<float>(s.env_w);
+    float v = (static_cast<float>(y) + j2) / static_cast<float>(s.env_h);
+    F3 wi = uv_to_dir(u, v);
+    float cs = fmaxf(0.0f, dot(n, wi));
+    if (cs <= 0.0f) return ls;
+    float pdf = env_pdf(s, wi);
+    if (pdf <= 0.0f) return ls;
+    if (occluded<COUNT>(s, p, n, wi, kInf, c)) return ls;
+    ls.wi = wi;
+    ls.radiance = env_radiance(s, wi);
+    ls.pdf = pdf;
+    ls.valid = true;
+    return ls;
+}
+
+// evaluateDirectLight (direct_lighting.cuh:204-218)
+__device__ __forceinline__ F3 eval_direct(const LightSample& ls, const MatEval<float>& m, F3 wo, F3 n) {
+    if (!ls.valid) return f3(0.0f);
+    BsdfEvalT<float> b = eval_bsdf(m, wo, ls.wi, n, true);
+    if (b.pdf <= 0.0f) return f3(0.0f);
+    float mis = power_heuristic(ls.pdf, b.pdf);
+    return ls.radiance * f3(b.f.x, b.f.y, b.f.z) * (mis / (ls.pdf + 1e-7f));
+}
+
+// evaluatePunctualLights (direct_lighting.cuh:158-200)
+template <bool COUNT>
+__device__ __forceinline__ F3 punctual(const DevScene& s, F3 p, const MatEval<float>& m, F3 wo, F3 n, TraceCtx& c) {
+    F3 ld = f3(0.0f);
+    if (s.num_lights == 0 || s.lights == nullptr) return ld;
+    for (uint32_t i = 0; i < s.num_lights; ++i) {
+        const pt_punctual_light& l = s.lights[i];
+        F3 wi;
+        float dist = kInf;
+        F3 inten;
+        if (l.type == 0 || l.type == 2) {
+            F3 tl = f3(l.position.x, l.position.y, l.position.z) - p;
+            float d2 = dot(tl, tl);
+            if (d2 < 1e-8f) continue;
+            dist = sqrtf(d2);
+            wi = tl / dist;
+            float att = dist >= l.range ? 0.0f : 1.0f / (dist * dist + 1e-7f);  // punctual_light.h:36-39
+            if (l.type == 2) {                                                 // punctual_light.h:28-34
+                float ca = dot(normalize(-tl), f3(l.direction.x, l.direction.y, l.direction.z));
+                float sp;
+                if (ca < l.outer_cone) sp = 0.0f;
+                else if (ca > l.inner_cone) sp = 1.0f;
+                else {
+                    float t = (ca - l.outer_cone) / (l.inner_cone - l.outer_cone + 1e-7f);
+                    sp = t * t;
+                }
+                att *= sp;
+            }
+            inten = f3(l.color.x, l.color.y, l.color.z) * (l.intensity * att);
+        } else {
+            wi = -f3(l.direction.x, l.direction.y, l.direction.z);
+            inten = f3(l.color.x, l.color.y, l.color.z) * l.intensity;
+        }
+        if (dot(n, wi) <= 0.0f) continue;
+        if (occluded<COUNT>(s, p, n, wi, dist, c)) continue;
+        BsdfEvalT<float> b = eval_bsdf(m, wo, wi, n, true);
+        ld += inten * f3(b.f.x, b.f.y, b.f.z);
+    }
+    return ld;
+}
+
+// ---- surface interaction ----------------------------------------------------
+struct Surface {
+    F3 p, n;      // point, interpolated shading normal
+    float u, v;   // texture coordinates
+    float bu, bv; // barycentrics
+    uint32_t tri; // BVH slot
+    uint32_t mat;
+    float t;
+};
+
+// intersectTriangle's hit record (triangle.h:40-49).
+__device__ __forceinline__ Surface make_surface(const DevScene& s, F3 o, F3 d, const Hit& h) {
+    Surface sf;
+    uint4 f = s.faces[h.tri];
+    float w = 1.0f - h.u - h.v;
+    sf.t = h.t;
+    sf.p = o + h.t * d;
+    sf.n = normalize(f3(s.vnrm[f.x]) * w + f3(s.vnrm[f.y]) * h.u + f3(s.vnrm[f.z]) * h.v);
+    float2 t0 = s.vuv[f.x], t1 = s.vuv[f.y], t2 = s.vuv[f.z];
+    sf.u = t0.x * w + t1.x * h.u + t2.x * h.v;
+    sf.v = t0.y * w + t1.y * h.u + t2.y * h.v;
+    sf.bu = h.u;
+    sf.bv = h.v;
+    sf.tri = h.tri;
+    sf.mat = f.w;
+    return sf;
+}
+
+// emissiveTrianglePdf (light_pdf.cuh:29-54)
+__device__ __forceinline__ float emissive_pdf(const DevScene& s, uint32_t tri, float dist, F3 d) {
+    if (tri >= s.num_faces || s.tri_pmf == nullptr) return 0.0f;
+    float pmf = s.tri_pmf[tri];
+    if (pmf <= 0.0f) return 0.0f;
+    uint4 f = s.faces[tri];
+    F3 p0 = f3(s.vpos[f.x]), p1 = f3(s.vpos[f.y]), p2 = f3(s.vpos[f.z]);
+    F3 c = cross(p1 - p0, p2 - p0);
+    float area = 0.5f * length(c);
+    if (area <= 0.0f) return 0.0f;
+    F3 ln = normalize(c);
+    float cl = fabsf(dot(ln, -d));
+    if (cl <= 1e-5f) return 0.0f;
+    return pmf * dist * dist / (area * cl);
+}
+
+__device__ __forceinline__ bool mat_emissive(const pt_material& m) {
+    return m.emissive_strength > 0.0f && (m.emissive.x > 0.0f || m.emissive.y > 0.0f || m.emissive.z > 0.0f);
+}
+
+// applyNormalMap (shading_utils.cuh:43-68); interpolateTangent (:29-41).
+__device__ __forceinline__ F3 normal_map(const DevScene& s, const pt_material& m, F3 n, const Surface& sf, float u,
+                                         float v) {
+    if (!tex_valid(s, m.normal_tex_id)) return n;
+    float4 nm = tex_fetch(s, m.normal_tex_id, u, v);
+    F3 mn = f3(nm.x * 2.0f - 1.0f, nm.y * 2.0f - 1.0f, nm.z * 2.0f - 1.0f);
+    mn.x *= m.normal_scale;
+    mn.y *= m.normal_scale;
+    mn = normalize(mn);
+    F3 T = f3(0.0f);
+    float sign = 1.0f;
+    if (s.vtan != nullptr) {
+        uint4 f = s.faces[sf.tri];
+        float w = 1.0f - sf.bu - sf.bv;
+        float4 a = s.vtan[f.x], b = s.vtan[f.y], c = s.vtan[f.z];
+        T = normalize(f3(w * a.x + sf.bu * b.x + sf.bv * c.x, w * a.y + sf.bu * b.y + sf.bv * c.y,
+                         w * a.z + sf.bu * b.z + sf.bv * c.z));
+        sign = a.w >= 0.0f ? 1.0f : -1.0f;
+    }
+    if (!is_zero(T)) {
+        T = normalize(T - n * dot(n, T));
+        F3 B = cross(n, T) * sign;
+        return normalize(T * mn.x + B * mn.y + n * mn.z);
+    }
+    Frame fr = make_frame(n);
+    return normalize(to_world(fr, mn));
+}
+
+// ---- camera -------------------------------------------------------------------
+// emitRay (path.cu:31-59)
+__device__ __forceinline__ void camera_ray(const RenderParams& p, int x, int y, Sampler& rng, F3& o, F3& d) {
+    float jx = rng.next();
+    float jy = rng.next();
+    float u = (2.0f * (static_cast<float>(x) + jx) / static_cast<float>(p.width) - 1.0f) * p.tan_half_fov * p.aspect;
+    float v = (2.0f * (static_cast<float>(y) + jy) / static_cast<float>(p.height) - 1.0f) * p.tan_half_fov;
+    F3 pos = f3(p.cam_pos[0], p.cam_pos[1], p.cam_pos[2]);
+    F3 fwd = f3(p.cam_fwd[0], p.cam_fwd[1], p.cam_fwd[2]);
+    F3 rgt = f3(p.cam_right[0], p.cam_right[1], p.cam_right[2]);
+    F3 cup = f3(p.cam_up[0], p.cam_up[1], p.cam_up[2]);
+    F3 dir = normalize(u * rgt + v * cup + fwd);
+    if (p.aperture > 0.0f) {
+        F3 focal = pos + dir * (p.focal_distance / dot(dir, fwd));
+        float l1 = rng.next();
+        float l2 = rng.next();
+        float r = sqrtf(l1) * p.aperture;
+        float th = kTwoPi * l2;
+        pos = pos + (rgt * (r * cosf(th)) + cup * (r * sinf(th)));
+        dir = normalize(focal - pos);
+    }
+    o = pos;
+    d = normalize(dir);  // Ray's constructor normalizes (core/ray.h:13)
+}
+
+// ---- path ---------------------------------------------------------------------
+template <bool COUNT>
+__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
+    F3 lo = f3(0.0f);
+    F3 tp = f3(1.0f);
+    F3 o, d;
+    camera_ray(p, x, y, rng, o, d);
+    float last_pdf = 0.0f;
+    bool last_delta = true;
+    const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
+    const bool has_env = s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 &&
+                         s.env_h > 0;
+    for (int depth = 0; depth < p.max_depth; ++depth) {
+        c.closest++;
+        Hit h = trace_closest<COUNT>(s, o, d, c.stk, c.stride, c.cnt, c.overflow);
+        if (h.tri == 0xFFFFFFFFu) {
+            F3 env = env_radiance(s, d);
+            if (!is_zero(env)) {
+                float mis = 1.0f;
+                if (depth > 0 && !last_delta) mis = power_heuristic(last_pdf, env_pdf(s, d));
+                lo += tp * env * mis;
+            }
+            break;
+        }
+        Surface sf = make_surface(s, o, d, h);
+        const pt_material& mat = s.materials[sf.mat];
+        bool entering = dot(sf.n, d) < 0.0f;
+        F3 gn;
+        if (mat.double_sided && !entering) {
+            gn = -sf.n;
+            entering = true;
+        } else {
+            gn = entering ? sf.n : -sf.n;
+        }
+        F3 wo = -d;
+        float tu = sf.u, tv = sf.v;
+        apply_uv_transform(mat, tu, tv);
+        F3 n = normal_map(s, mat, gn, sf, tu, tv);
+        MatEval<float> me = fetch_material(s, mat, tu, tv);
+        if (mat_emissive(mat)) {
+            float mis = 1.0f;
+            if (depth > 0 && !last_delta) mis = power_heuristic(last_pdf, emissive_pdf(s, sf.tri, sf.t, d));
+            lo += tp * (f3(mat.emissive.x, mat.emissive.y, mat.emissive.z) * mat.emissive_strength) * mis;
+            break;
+        }
+        if (mat.alpha_mode == 1) {  // path.cu:136-146 (raw uv)
+            float alpha = 1.0f;
+            if (tex_valid(s, mat.base_color_tex_id)) alpha = tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
+            if (alpha < mat.alpha_cutoff) {
+                o = sf.p + d * 1e-4f;
+                d = normalize(d);
+                continue;
+            }
+        } else if (mat.alpha_mode == 2) {  // :147-158
+            float alpha = mat.base_alpha;
+            if (tex_valid(s, mat.base_color_tex_id)) alpha *= tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
+            if (rng.next() >= alpha) {
+                o = sf.p + d * 1e-4f;
+                d = normalize(d);
+                continue;
+            }
+        }
+        bool delta_surface = (me.metallic >= 1.0f && me.roughness < 0.01f) ||
+                             (me.spec_trans >= 1.0f && me.roughness < 0.01f);
+        if (!delta_surface) {
+            F3 ld = f3(0.0f);
+            if (has_area && has_env) {
+                ld += eval_direct(sample_area<COUNT>(s, sf.p, n, rng, c), me, wo, n);
+                ld += eval_direct(sample_env<COUNT>(s, sf.p, n, rng, c), me, wo, n);
+            } else if (has_area) {
+                ld = eval_direct(sample_area<COUNT>(s, sf.p, n, rng, c), me, wo, n);
+            } else if (has_env) {
+                ld = eval_direct(sample_env<COUNT>(s, sf.p, n, rng, c), me, wo, n);
+            }
+            lo += tp * ld;
+            lo += tp * punctual<COUNT>(s, sf.p, me, wo, n, c);
+        }
+        BsdfSample bs = sample_bsdf(me, wo, n, rng, entering);
+        if (!bs.valid) break;
+        tp *= bs.weight;
+        bool transmitted = dot(bs.wi, gn) < 0.0f;
+        F3 off = transmitted ? -gn : gn;
+        o = sf.p + off * 1e-4f;
+        d = normalize(bs.wi);
+        if (transmitted && me.thickness > 0.0f && me.attenuation_distance < 1e20f) {  // :178-187
+            float t = me.thickness, ad = me.attenuation_distance;
+            F3 ac = me.attenuation_color;
+            tp *= f3(expf(-t * fmaxf(-logf(fmaxf(ac.x, 1e-6f)), 0.0f) / ad),
+                     expf(-t * fmaxf(-logf(fmaxf(ac.y, 1e-6f)), 0.0f) / ad),
+                     expf(-t * fmaxf(-logf(fmaxf(ac.z, 1e-6f)), 0.0f) / ad));
+        }
+        last_pdf = bs.pdf;
+        last_delta = bs.delta;
+        if (depth > 2) {  // :193-198
+            float surv = fminf(max3(tp), 0.95f);
+            if (rng.next() >= surv) break;
+            tp /= surv;
+        }
+    }
+    lo.x = fminf(lo.x, 100.0f);
+    lo.y = fminf(lo.y, 100.0f);
+    lo.z = fminf(lo.z, 100.0f);
+    return lo;
+}
+
+}  // namespace ptd

@@ -1,0 +1,22 @@
+// Host-side launchers of the render and probe kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dev_scene.h"
+
+namespace ptd {
+
+hipError_t launch_render_plain(const DevScene& s, const RenderParams& p, float4* film_new, float4* film_acc,
+                               DevCounters* ctr, bool count, hipStream_t stream);
+hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, int n, int dims, uint32_t* d_u,
+                                float* d_f, hipStream_t stream);
+hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, float* d_t, int32_t* d_face,
+                             float* d_bary, hipStream_t stream);
+// Prepares the triangle records of the traversal layout (dev_scene.h) and the
+// BVH-ordered faces from the original faces and the build's slot order.
+hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, const uint32_t* d_prim_order,
+                            const float4* d_nodes, uint32_t num_nodes, uint32_t nfaces, float4* d_tris,
+                            uint4* d_faces_bvh, hipStream_t stream);
+
+}  // namespace ptd
