@@ -312,6 +312,12 @@ pt_status pt_probe_sampler(int sampler, uint32_t seed, const int32_t* xy_idx, in
 pt_status pt_trace_rays(pt_scene* scene, const float* rays8, int n, int any_hit,
                         float* t_out, int32_t* face_out, float* bary_out);
 
+/* Streams `bytes` (rounded down to 16 B) from one device buffer to another
+ * `iters` times with a plain vectorised copy kernel and reports GB/s (read +
+ * write) from HIP events.  No reference counterpart: it calibrates HBM
+ * traffic counters and states the achievable bandwidth of the box. */
+pt_status pt_probe_copy(size_t bytes, int iters, float* gbps);
+
 #ifdef __cplusplus
 }
 #endif

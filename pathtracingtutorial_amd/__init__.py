@@ -353,3 +353,10 @@ def trace_rays(scene: SceneManager, rays: np.ndarray, any_hit: bool = False):
     bary = np.zeros((n, 2), np.float32)
     check(lib().pt_trace_rays(scene.handle, fptr(rays), n, 1 if any_hit else 0, fptr(t), i32ptr(face), fptr(bary)))
     return t, face, bary
+
+
+def probe_copy(nbytes: int = 1 << 30, iters: int = 10) -> float:
+    """Achieved GB/s (read + write) of a plain device copy kernel."""
+    g = C.c_float(0.0)
+    check(lib().pt_probe_copy(int(nbytes), int(iters), C.byref(g)))
+    return float(g.value)

@@ -880,4 +880,29 @@ pt_status pt_trace_rays(pt_scene* s, const float* rays8, int n, int any_hit, flo
     });
 }
 
+pt_status pt_probe_copy(size_t bytes, int iters, float* gbps) {
+    return guarded([&] {
+        if (bytes < 16 || iters <= 0 || !gbps) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        ensure_device();
+        size_t n = bytes / 16;
+        DBuf<float4> a, b;
+        a.alloc(n);
+        b.alloc(n);
+        hip_check(hipMemset(a.p, 0, n * 16), "hipMemset");
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(ptd::launch_copy_probe(a.p, b.p, n, nullptr), "copy warmup");
+        hip_check(hipEventRecord(e0, nullptr), "record");
+        for (int i = 0; i < iters; ++i) hip_check(ptd::launch_copy_probe(a.p, b.p, n, nullptr), "copy");
+        hip_check(hipEventRecord(e1, nullptr), "record");
+        hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbps = static_cast<float>(2.0 * n * 16.0 * iters / (ms * 1e-3) / 1e9);
+    });
+}
+
 }  // extern "C"

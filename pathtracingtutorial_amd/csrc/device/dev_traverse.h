@@ -56,6 +56,9 @@ __device__ __forceinline__ float slab(const RayInv& r, float4 lo, float4 hi, flo
 // Moller-Trumbore (triangle.h:13-51 / :53-76).  Returns t, or -1 on a miss;
 // u, v are the barycentrics of p1 and p2.
 __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& u, float& v, uint32_t& word) {
+    // no FMA contraction: every product and sum rounds as in the reference's
+    // float arithmetic, so t matches it to the last bit
+#pragma clang fp contract(off)
     float4 a = tri[0], b = tri[1], c = tri[2];
     word = __float_as_uint(a.w);
     F3 e1 = f3(b.x, b.y, b.z), e2 = f3(c.x, c.y, c.z);

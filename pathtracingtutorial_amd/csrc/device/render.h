@@ -19,4 +19,6 @@ hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, con
                             const float4* d_nodes, uint32_t num_nodes, uint32_t nfaces, float4* d_tris,
                             uint4* d_faces_bvh, hipStream_t stream);
 
+hipError_t launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream);
+
 }  // namespace ptd

@@ -198,4 +198,19 @@ hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, con
     return hipGetLastError();
 }
 
+// ---- copy probe ------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    size_t n) {
+    size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream) {
+    int blocks = 256 * 8 * 4;  // 4 blocks per CU per wave slot group, grid-stride over the buffer
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 }  // namespace ptd

@@ -192,9 +192,12 @@ void generate_sky_env(Image& env, int w, int h, uint64_t seed) {
             double ang = std::acos(c);
             double sun = 1e4 * std::exp(-ang * ang / (2.0 * sigma * sigma));
             float* o = &env.rgba[(static_cast<size_t>(y) * w + x) * 4];
-            o[0] = static_cast<float>(r + sun * 1.0);
-            o[1] = static_cast<float>(g + sun * 0.93);
-            o[2] = static_cast<float>(b + sun * 0.82);
+            // the renderer scales env radiance by 2 pi (envmap.h:43-45); the sky is set
+            // so that the scaled sky stays near 1.5 while the sun peak is 1e4
+            const double sky = 0.25;
+            o[0] = static_cast<float>(sky * r + sun * 1.0);
+            o[1] = static_cast<float>(sky * g + sun * 0.93);
+            o[2] = static_cast<float>(sky * b + sun * 0.82);
             o[3] = 1.0f;
         }
     }
