@@ -6,14 +6,18 @@ Workload: 64 displaced level-5 icospheres (1,310,720 triangles) + ground,
 per leaf.  One step = one 64-spp frame through the integrator.
 
 Prints one JSON line:
-  value      traced rays (closest + shadow) per second, Mrays/s, whole job
-  roofline   algorithmic bytes of the render kernel over its HIP-event time
-             (32 B per node record tested + 36 B per triangle test + 48 B per
-             ray) against the 8 TB/s HBM3E peak
+  value      traced rays (closest + shadow) per second, Mrays/s, whole job, of
+             the wavefront integrator (the plain megakernel's figures sit in
+             "plain", rendered from the same random numbers: the same estimate)
+  roofline   the dominant traversal kernel's algorithmic bytes per launch over
+             its average HIP-event duration, against the 8 TB/s HBM3E peak:
+             k_wf_extend (32 B per node record tested + 36 B per triangle test
+             + 52 B of ray/hit/queue I/O per ray); for --integrator path,
+             k_render_plain (48 B per ray of I/O)
   cpu_baseline  the oracle (a C++ restatement of the reference integrator)
              on a bounded sample of the same frame, all host cores
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator path]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--integrator both|wavefront|path]
 """
 from __future__ import annotations
 
@@ -89,12 +93,52 @@ def cpu_baseline(sm, target_s: float = 12.0) -> dict:
                       f"{dt:.1f} s); oracle BVH build {build_s:.1f} s"}
 
 
+def run_integrator(sm, integ, steps: int, warmup: int) -> dict:
+    """Warmup, then `steps` timed frames from a cleared film, then the same
+    frames again with in-kernel counters (exact ray / node / triangle counts)."""
+    import pathtracingtutorial_amd as pt
+    r = pt.Renderer(sm, integ)
+    for i in range(warmup):
+        r.renderBatch()
+        log(f"{integ.name()} warmup step {i}: {r.lastBatchMs():.1f} ms")
+    r.resetAccumulation()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.renderBatch()
+        kernel_ms.append(r.lastBatchMs())
+    wall = time.perf_counter() - t0
+    log(f"{integ.name()} steps: " + ", ".join(f"{m:.1f}" for m in kernel_ms) + f" ms; wall {wall:.3f} s")
+    img = r.downloadCurrentImage()
+    r.resetAccumulation()
+    sm.enableCounters(True)
+    sm.resetCounters()
+    for i in range(steps):
+        r.renderBatch()
+    c = sm.counters()
+    sm.enableCounters(False)
+    return {"wall": wall, "kernel_ms": kernel_ms, "counters": c, "image": img, "renderer": r}
+
+
+def stage_times(sm, steps: int) -> dict:
+    """One more frame of the wavefront with every launch bracketed by HIP events
+    on the integrator's stream: per-stage GPU ms and launch counts."""
+    import pathtracingtutorial_amd as pt
+    cfg = sm.wavefrontConfig()
+    sm.setWavefrontConfig(time_stages=1)
+    r = pt.Renderer(sm, pt.WavefrontIntegrator())
+    r.renderBatch()
+    st = sm.wavefrontStats()
+    sm.setWavefrontConfig(**cfg)
+    return st
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--integrator", default="path", choices=["path", "wavefront"])
+    ap.add_argument("--integrator", default="both", choices=["both", "path", "wavefront"])
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -109,11 +153,10 @@ def main(argv=None) -> int:
         return 0
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the path shards by sample index across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(one process renders on one GPU; frames would shard by sample index across GPUs)", file=sys.stderr)
         return 2
 
     import pathtracingtutorial_amd as pt
-    integ = pt.PathIntegrator() if a.integrator == "path" else pt.WavefrontIntegrator()
     sm = pt.SceneManager()
     t0 = time.perf_counter()
     sm.generate(1, a.seed)
@@ -124,35 +167,44 @@ def main(argv=None) -> int:
     log(f"accel: {info.num_faces} tris, {info.num_bvh_nodes} nodes, depth {info.bvh_depth}, "
         f"GPU build {info.bvh_build_ms:.2f} ms, total {time.perf_counter() - t0:.2f} s")
     sm.uploadToDevice()
-    r = pt.Renderer(sm, integ)
-    for i in range(a.warmup):
-        r.renderBatch()
-        log(f"warmup step {i}: {r.lastBatchMs():.1f} ms")
-    # timed: frames 0..K-1 from a cleared film
-    r.resetAccumulation()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        r.renderBatch()
-        kernel_ms.append(r.lastBatchMs())
-        log(f"step {i}: {kernel_ms[-1]:.1f} ms")
-    wall = time.perf_counter() - t0
-    # the same frames again with in-kernel counters, for exact ray and node counts
-    r.resetAccumulation()
-    sm.enableCounters(True)
-    sm.resetCounters()
-    for i in range(a.steps):
-        r.renderBatch()
-    c = sm.counters()
-    sm.enableCounters(False)
+    runs = {}
+    order = ["wavefront", "path"] if a.integrator == "both" else [a.integrator]
+    for name in order:
+        integ = pt.PathIntegrator() if name == "path" else pt.WavefrontIntegrator()
+        sm.uploadToDevice()  # frame_count 0: both integrators render the same frames
+        runs[name] = run_integrator(sm, integ, a.steps, a.warmup)
+    head = order[0]
+    R = runs[head]
+    c = R["counters"]
     rays = c["closest_rays"] + c["shadow_rays"]
-    ms_step = wall * 1000.0 / a.steps
-    mrays = rays / wall / 1e6
-    avg_kernel_s = (sum(kernel_ms) / len(kernel_ms)) / 1000.0
-    bytes_per_launch = algorithmic_bytes(c) / a.steps
-    achieved = bytes_per_launch / avg_kernel_s / 1e9
-    img = r.downloadCurrentImage()
-    log(f"counters: {c}; image mean {img[..., :3].mean():.4f}; finite {bool(np.isfinite(img).all())}")
+    ms_step = R["wall"] * 1000.0 / a.steps
+    mrays = rays / R["wall"] / 1e6
+    log(f"{head} counters: {c}")
+    if head == "wavefront":
+        st = stage_times(sm, a.steps)
+        log(f"wavefront stages: {st}")
+        ext = st["extend"]
+        kernel_s = ext["ms"] / max(1, ext["launches"]) / 1000.0
+        launches = ext["launches"] * a.steps  # the counted run renders `steps` frames
+        closest_nodes = c["node_visits"] - c["shadow_node_visits"]
+        closest_tris = c["tri_tests"] - c["shadow_tri_tests"]
+        bytes_per_launch = (32 * closest_nodes + 36 * closest_tris + 52 * c["closest_rays"]) / max(1, launches)
+        kernel = "k_wf_extend"
+        per_ray = {"nodes_per_ray": round(closest_nodes / max(1, c["closest_rays"]), 2),
+                   "tris_per_ray": round(closest_tris / max(1, c["closest_rays"]), 2)}
+        extra = {"stages_ms_per_frame": {k: round(v["ms"], 2) for k, v in st.items()},
+                 "launches_per_frame": ext["launches"]}
+    else:
+        st = None
+        kernel_s = (sum(R["kernel_ms"]) / len(R["kernel_ms"])) / 1000.0
+        bytes_per_launch = algorithmic_bytes(c) / a.steps
+        kernel = "k_render_plain"
+        per_ray = {"nodes_per_ray": round(c["node_visits"] / max(1, rays), 2),
+                   "tris_per_ray": round(c["tri_tests"] / max(1, rays), 2)}
+        extra = {}
+    achieved = bytes_per_launch / kernel_s / 1e9
+    img = R["image"]
+    log(f"image mean {img[..., :3].mean():.4f}; finite {bool(np.isfinite(img).all())}")
     out = {
         "metric": "Mrays/s (1M tris, 1080p) wavefront vs plain at equal relMSE; backward Mrays/s",
         "value": round(mrays, 2),
@@ -169,17 +221,35 @@ def main(argv=None) -> int:
         "config": {"workload": WORKLOAD, "triangles": info.num_faces, "resolution": [info.width, info.height],
                    "spp": info.sample_per_frame, "max_depth": info.max_depth, "sampler": "sobol",
                    "bvh": {"bins": 32, "max_leaf": 4, "nodes": info.num_bvh_nodes, "depth": info.bvh_depth},
-                   "integrator": integ.name(), "paths_per_step": info.width * info.height * info.sample_per_frame,
+                   "integrator": "Wavefront" if head == "wavefront" else "Path",
+                   "paths_per_step": info.width * info.height * info.sample_per_frame,
                    "rays_per_step": rays // a.steps,
                    "shadow_fraction": round(c["shadow_rays"] / max(1, rays), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("k_render_plain", WORKLOAD),
-                     "kernel": "k_render_plain", "kernel_ms": round(avg_kernel_s * 1000, 3),
-                     "bytes_per_launch": int(bytes_per_launch),
-                     "nodes_per_ray": round(c["node_visits"] / max(1, rays), 2),
-                     "tris_per_ray": round(c["tri_tests"] / max(1, rays), 2)},
+                     "traffic": profiled_traffic(kernel, WORKLOAD),
+                     "kernel": kernel, "kernel_ms": round(kernel_s * 1000, 3),
+                     "bytes_per_launch": int(bytes_per_launch), **per_ray, **extra},
     }
+    if head == "wavefront":
+        out["config"]["wavefront"] = sm.wavefrontConfig()
+    if "path" in runs and head != "path":
+        P = runs["path"]
+        pc = P["counters"]
+        prays = pc["closest_rays"] + pc["shadow_rays"]
+        p_mrays = prays / P["wall"] / 1e6
+        p_kernel_s = (sum(P["kernel_ms"]) / len(P["kernel_ms"])) / 1000.0
+        p_ach = algorithmic_bytes(pc) / a.steps / p_kernel_s / 1e9
+        a_img, b_img = R["image"][..., :3].astype(np.float64), P["image"][..., :3].astype(np.float64)
+        differ = float((np.abs(a_img - b_img) > 1e-4 * (1.0 + np.abs(b_img))).mean())
+        out["plain"] = {"value": round(p_mrays, 2), "ms_per_step": round(P["wall"] * 1000.0 / a.steps, 2),
+                        "speedup_of_wavefront": round(mrays / p_mrays, 3),
+                        "roofline": {"kernel": "k_render_plain", "achieved": round(p_ach, 1),
+                                     "frac": round(p_ach / HBM_PEAK_GBS, 4),
+                                     "traffic": profiled_traffic("k_render_plain", WORKLOAD)},
+                        # same random numbers per path: the images are the same estimate
+                        "image_channels_differing_1e-4": round(differ, 6),
+                        "rays_per_step": prays // a.steps}
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(sm)

@@ -176,12 +176,37 @@ typedef struct pt_counters {
     uint64_t tri_tests;      /* triangle intersection tests */
     uint64_t paths;
     uint64_t stack_overflows;
+    uint64_t shadow_node_visits; /* the any-hit (shadow ray) share of node_visits */
+    uint64_t shadow_tri_tests;   /* the any-hit share of tri_tests */
 } pt_counters;
 
 typedef struct pt_scene pt_scene;
 typedef struct pt_film pt_film;
 
 enum { PT_INTEGRATOR_PATH = 0, PT_INTEGRATOR_WAVEFRONT = 1 };
+
+/* Launch configuration of the wavefront integrator (no reference
+ * counterpart: the reference has only the per-pixel megakernel,
+ * integrators/path.cu:209-255).  Every field changes speed only, never the
+ * image: the estimator and its random numbers are the megakernel's. */
+typedef struct pt_wavefront_config {
+    int32_t chunk_spp;        /* samples per pixel in flight (a divisor of sample_per_frame is used) */
+    int32_t extend_block;     /* threads per block of the traversal kernels: 64, 128 or 256 */
+    int32_t shade_block;      /* threads per block of the shading kernel: 64, 128 or 256 */
+    int32_t waves_per_simd;   /* persistent grid: waves per SIMD, 0 = occupancy limit */
+    int32_t sort_by_material; /* counting-sort each bounce's shading queue by material */
+    int32_t use_graph;        /* replay one captured hipGraph per chunk */
+    int32_t time_stages;      /* bracket every launch with HIP events (no graph): pt_scene_wavefront_stats */
+} pt_wavefront_config;
+
+/* GPU time (ms) and launch count per stage of the last wavefront batch
+ * rendered with time_stages set: 0 camera rays, 1 closest-hit traversal,
+ * 2 material sort, 3 shading, 4 shadow-ray traversal, 5 control (queue
+ * resets, per-chunk finalize, film). */
+typedef struct pt_wavefront_stats {
+    float stage_ms[6];
+    uint32_t launches[6];
+} pt_wavefront_stats;
 
 /* ------------------------------------------------------------------------ */
 /* Library                                                                    */
@@ -288,6 +313,16 @@ pt_status pt_film_device_ptr(const pt_film* film, int which, void** dev_ptr);
 pt_status pt_render_batch(pt_scene* scene, pt_film* film, int integrator, float* ms_out);
 /* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
 pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
+/* Wavefront configuration of a scene (defaults: 8, 128, 128, 0, 1, 1, 0). */
+pt_status pt_scene_set_wavefront_config(pt_scene* scene, const pt_wavefront_config* cfg);
+pt_status pt_scene_get_wavefront_config(const pt_scene* scene, pt_wavefront_config* cfg);
+/* Autotuner: times up to max_trials configurations (coordinate descent over
+ * chunk_spp, block sizes, sorting and grid size) on the uploaded scene with a
+ * private film, keeps the fastest as the scene's configuration and returns it
+ * with its GPU time per sample per pixel (ms).  The caller's films and
+ * frame_count are untouched. */
+pt_status pt_scene_tune_wavefront(pt_scene* scene, int max_trials, pt_wavefront_config* best, float* ms_per_spp);
+pt_status pt_scene_wavefront_stats(const pt_scene* scene, pt_wavefront_stats* out);
 /* Counter control: when enabled, render kernels count rays, node visits and
  * triangle tests (slower; used to price algorithmic bytes). */
 pt_status pt_counters_enable(pt_scene* scene, int enable);

@@ -175,6 +175,34 @@ class SceneManager:
                                               fptr(ea), C.byref(c)))
         return {"tri_pmf": tp, "tri_alias": ta, "env_pmf": ep, "env_alias": ea}
 
+    def wavefrontConfig(self) -> dict:
+        c = _capi.WavefrontConfig()
+        check(lib().pt_scene_get_wavefront_config(self.handle, C.byref(c)))
+        return {k: int(getattr(c, k)) for k, _ in _capi.WavefrontConfig._fields_}
+
+    def setWavefrontConfig(self, **fields) -> None:
+        """Changes fields of the wavefront launch configuration (speed only)."""
+        cur = self.wavefrontConfig()
+        for k, v in fields.items():
+            if k not in cur:
+                raise KeyError(f"unknown wavefront config field {k!r}")
+            cur[k] = int(v)
+        check(lib().pt_scene_set_wavefront_config(self.handle, C.byref(_capi.WavefrontConfig(**cur))))
+
+    def tuneWavefront(self, max_trials: int = 16) -> tuple[dict, float]:
+        """Autotunes the wavefront configuration; returns it and its ms per sample per pixel."""
+        c = _capi.WavefrontConfig()
+        ms = C.c_float(0.0)
+        check(lib().pt_scene_tune_wavefront(self.handle, int(max_trials), C.byref(c), C.byref(ms)))
+        return {k: int(getattr(c, k)) for k, _ in _capi.WavefrontConfig._fields_}, float(ms.value)
+
+    def wavefrontStats(self) -> dict:
+        """Per-stage GPU ms and launch counts of the last batch rendered with time_stages=1."""
+        st = _capi.WavefrontStats()
+        check(lib().pt_scene_wavefront_stats(self.handle, C.byref(st)))
+        return {name: {"ms": float(st.stage_ms[i]), "launches": int(st.launches[i])}
+                for i, name in enumerate(_capi.WAVEFRONT_STAGES)}
+
     def enableCounters(self, on: bool = True) -> None:
         check(lib().pt_counters_enable(self.handle, 1 if on else 0))
 

@@ -147,7 +147,21 @@ class SceneInfo(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
-                ("tri_tests", C.c_uint64), ("paths", C.c_uint64), ("stack_overflows", C.c_uint64)]
+                ("tri_tests", C.c_uint64), ("paths", C.c_uint64), ("stack_overflows", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64)]
+
+
+class WavefrontConfig(C.Structure):
+    _fields_ = [("chunk_spp", C.c_int32), ("extend_block", C.c_int32), ("shade_block", C.c_int32),
+                ("waves_per_simd", C.c_int32), ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
+                ("time_stages", C.c_int32)]
+
+
+class WavefrontStats(C.Structure):
+    _fields_ = [("stage_ms", C.c_float * 6), ("launches", C.c_uint32 * 6)]
+
+
+WAVEFRONT_STAGES = ("generate", "extend", "sort", "shade", "shadow", "control")
 
 
 # Every exported symbol with its signature (kept in sync with include/pt_amd.h).
@@ -188,6 +202,10 @@ SIGNATURES = {
     "pt_film_device_ptr": (C.c_int, [_P, C.c_int, _PP]),
     "pt_render_batch": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_all": (C.c_int, [_P, _P, C.c_int, _FP]),
+    "pt_scene_set_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),
+    "pt_scene_get_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),
+    "pt_scene_tune_wavefront": (C.c_int, [_P, C.c_int, C.POINTER(WavefrontConfig), _FP]),
+    "pt_scene_wavefront_stats": (C.c_int, [_P, C.POINTER(WavefrontStats)]),
     "pt_counters_enable": (C.c_int, [_P, C.c_int]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
     "pt_counters_reset": (C.c_int, [_P]),

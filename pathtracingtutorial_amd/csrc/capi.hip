@@ -16,6 +16,7 @@
 #include "device/bvh_build.h"
 #include "device/dev_scene.h"
 #include "device/render.h"
+#include "device/wavefront.h"
 #include "host/host_scene.h"
 #include "pt_amd.h"
 
@@ -129,8 +130,13 @@ struct pt_scene {
     DBuf<ptd::DevCounters> counters;
     bool counting = false;
     uint32_t frame_count = 0;
+    // wavefront integrator state (created at its first batch)
+    ptd::Wavefront* wf = nullptr;
+    ptd::WavefrontConfig wf_cfg;
+    uint64_t generation = 0;  // bumped by every upload; keys the captured graph
 
     ~pt_scene() {
+        if (wf) ptd::wavefront_destroy(wf);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -283,6 +289,64 @@ void dfs_layout(const ptd::BvhHostTopology& t, std::vector<float>& out) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+
+namespace {
+
+void check_renderable(const pt_scene* s, const pt_film* f) {
+    if (!s || !f) throw Error(PT_ERR_INVALID_ARG, "null argument");
+    if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+    if (f->w != s->host.camera.width || f->h != s->host.camera.height)
+        throw Error(PT_ERR_INVALID_ARG, "film size does not match the camera resolution");
+}
+
+float render_wavefront(pt_scene* s, pt_film* f, const ptd::WavefrontConfig& cfg, uint32_t frame_count) {
+    if (!s->wf) s->wf = ptd::wavefront_create();
+    ptd::DevScene v = make_view(s);
+    ptd::RenderParams p = make_params(s);
+    p.frame_count = frame_count;
+    float ms = 0.0f;
+    std::string err;
+    hipError_t e = ptd::wavefront_render(s->wf, v, s->generation, p, cfg, f->film_new.p, f->film_acc.p,
+                                         s->counters.p, s->counting, s->stream, &ms, &err);
+    if (e == hipErrorOutOfMemory) throw Error(PT_ERR_OUT_OF_MEMORY, err.empty() ? "wavefront: out of memory" : err);
+    if (e != hipSuccess && !err.empty()) throw Error(e == hipErrorInvalidValue ? PT_ERR_INVALID_ARG : PT_ERR_DEVICE, err);
+    hip_check(e, "wavefront render");
+    return ms;
+}
+
+void check_wavefront_config(const pt_wavefront_config& c) {
+    auto block_ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+    if (c.chunk_spp < 1 || c.chunk_spp > 4096) throw Error(PT_ERR_INVALID_ARG, "chunk_spp must be in [1, 4096]");
+    if (!block_ok(c.extend_block) || !block_ok(c.shade_block))
+        throw Error(PT_ERR_INVALID_ARG, "block sizes must be 64, 128 or 256");
+    if (c.waves_per_simd < 0 || c.waves_per_simd > 8) throw Error(PT_ERR_INVALID_ARG, "waves_per_simd must be in [0, 8]");
+}
+
+ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
+    ptd::WavefrontConfig d;
+    d.chunk_spp = c.chunk_spp;
+    d.extend_block = c.extend_block;
+    d.shade_block = c.shade_block;
+    d.waves_per_simd = c.waves_per_simd;
+    d.sort_by_material = c.sort_by_material ? 1 : 0;
+    d.use_graph = c.use_graph ? 1 : 0;
+    d.time_stages = c.time_stages ? 1 : 0;
+    return d;
+}
+
+pt_wavefront_config to_abi_config(const ptd::WavefrontConfig& d) {
+    pt_wavefront_config c;
+    c.chunk_spp = d.chunk_spp;
+    c.extend_block = d.extend_block;
+    c.shade_block = d.shade_block;
+    c.waves_per_simd = d.waves_per_simd;
+    c.sort_by_material = d.sort_by_material;
+    c.use_graph = d.use_graph;
+    c.time_stages = d.time_stages;
+    return c;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -561,6 +625,7 @@ pt_status pt_scene_upload(pt_scene* s) {
         hip_check(hipStreamSynchronize(s->stream), "upload");
         s->frame_count = 0;
         s->uploaded = true;
+        s->generation++;
     });
 }
 
@@ -766,31 +831,131 @@ pt_status pt_film_device_ptr(const pt_film* f, int which, void** p) {
     });
 }
 
+
 pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out) {
     return guarded([&] {
-        if (!s || !f) throw Error(PT_ERR_INVALID_ARG, "null argument");
-        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
-        if (f->w != s->host.camera.width || f->h != s->host.camera.height)
-            throw Error(PT_ERR_INVALID_ARG, "film size does not match the camera resolution");
-        if (integrator != PT_INTEGRATOR_PATH) throw Error(PT_ERR_UNSUPPORTED, "integrator not available");
-        ptd::DevScene v = make_view(s);
-        ptd::RenderParams p = make_params(s);
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
-        hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
-        hipError_t le = ptd::launch_render_plain(v, p, f->film_new.p, f->film_acc.p, s->counters.p, s->counting,
-                                                 s->stream);
-        hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
-        hipError_t se = hipEventSynchronize(e1);
+        check_renderable(s, f);
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        hip_check(le, "render launch");
-        hip_check(se, "render");
+        if (integrator == PT_INTEGRATOR_WAVEFRONT) {
+            ms = render_wavefront(s, f, s->wf_cfg, s->frame_count);
+        } else if (integrator == PT_INTEGRATOR_PATH) {
+            ptd::DevScene v = make_view(s);
+            ptd::RenderParams p = make_params(s);
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
+            hipError_t le = ptd::launch_render_plain(v, p, f->film_new.p, f->film_acc.p, s->counters.p, s->counting,
+                                                     s->stream);
+            hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
+            hipError_t se = hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            hip_check(le, "render launch");
+            hip_check(se, "render");
+        } else {
+            throw Error(PT_ERR_INVALID_ARG, "unknown integrator " + std::to_string(integrator));
+        }
         if (ms_out) *ms_out = ms;
         s->frame_count++;
+    });
+}
+
+pt_status pt_scene_set_wavefront_config(pt_scene* s, const pt_wavefront_config* cfg) {
+    return guarded([&] {
+        if (!s || !cfg) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        check_wavefront_config(*cfg);
+        s->wf_cfg = to_device_config(*cfg);
+    });
+}
+
+pt_status pt_scene_get_wavefront_config(const pt_scene* s, pt_wavefront_config* cfg) {
+    return guarded([&] {
+        if (!s || !cfg) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *cfg = to_abi_config(s->wf_cfg);
+    });
+}
+
+pt_status pt_scene_wavefront_stats(const pt_scene* s, pt_wavefront_stats* out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        ptd::wavefront_stage_times(s->wf, out->stage_ms, out->launches);
+    });
+}
+
+pt_status pt_scene_tune_wavefront(pt_scene* s, int max_trials, pt_wavefront_config* best, float* ms_per_spp) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        if (max_trials < 1) throw Error(PT_ERR_INVALID_ARG, "max_trials must be positive");
+        pt_film tf;
+        tf.w = s->host.camera.width;
+        tf.h = s->host.camera.height;
+        size_t n = static_cast<size_t>(tf.w) * tf.h;
+        tf.film_new.alloc(n);
+        tf.film_acc.alloc(n);
+        const bool counting = s->counting;
+        const int spf = s->host.settings.sample_per_frame;
+        s->counting = false;
+        int trials = 0;
+        // time per sample per pixel of one configuration: the batch is cut to
+        // two chunks (warm-up run first: graph capture is not timed)
+        auto measure = [&](const ptd::WavefrontConfig& c) -> float {
+            int S = std::max(1, std::min(c.chunk_spp, spf));
+            while (spf % S) --S;
+            const int saved = s->host.settings.sample_per_frame;
+            s->host.settings.sample_per_frame = std::min(spf, 2 * S);
+            float ms = 0.0f;
+            try {
+                render_wavefront(s, &tf, c, 0);
+                ms = render_wavefront(s, &tf, c, 1);
+            } catch (...) {
+                s->host.settings.sample_per_frame = saved;
+                throw;
+            }
+            const float per = ms / static_cast<float>(s->host.settings.sample_per_frame);
+            s->host.settings.sample_per_frame = saved;
+            ++trials;
+            return per;
+        };
+        ptd::WavefrontConfig cur = s->wf_cfg;
+        cur.time_stages = 0;
+        float cur_ms;
+        try {
+            cur_ms = measure(cur);
+            auto try_axis = [&](auto set, std::initializer_list<int> values) {
+                for (int v : values) {
+                    if (trials >= max_trials) return;
+                    ptd::WavefrontConfig c = cur;
+                    set(c, v);
+                    if (std::memcmp(&c, &cur, sizeof(c)) == 0) continue;
+                    float t;
+                    try {
+                        t = measure(c);
+                    } catch (const Error& e) {
+                        if (e.code == PT_ERR_OUT_OF_MEMORY) continue;  // chunk too large for device memory
+                        throw;
+                    }
+                    if (t < cur_ms) {
+                        cur_ms = t;
+                        cur = c;
+                    }
+                }
+            };
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.chunk_spp = v; }, {2, 4, 8, 16, 32});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.extend_block = v; }, {64, 128, 256});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.shade_block = v; }, {64, 128, 256});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.sort_by_material = v; }, {0, 1});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.waves_per_simd = v; }, {0, 1, 2, 4});
+        } catch (...) {
+            s->counting = counting;
+            throw;
+        }
+        s->counting = counting;
+        s->wf_cfg = cur;
+        if (best) *best = to_abi_config(cur);
+        if (ms_per_spp) *ms_per_spp = cur_ms;
     });
 }
 
@@ -828,6 +993,8 @@ pt_status pt_counters_get(pt_scene* s, pt_counters* out) {
         out->tri_tests = c.tri_tests;
         out->paths = c.paths;
         out->stack_overflows = c.stack_overflows;
+        out->shadow_node_visits = c.shadow_node_visits;
+        out->shadow_tri_tests = c.shadow_tri_tests;
     });
 }
 

@@ -19,7 +19,8 @@ namespace ptd {
 struct TraceCtx {
     uint32_t* stk;
     int stride;
-    TraceCounters cnt;
+    TraceCounters cnt;   // closest-hit queries
+    TraceCounters scnt;  // shadow (any-hit) queries
     uint32_t overflow;
     uint32_t closest, shadow;
 };
@@ -79,20 +80,30 @@ __device__ __forceinline__ uint32_t sample_alias(const float2* table, uint32_t n
     return u2 < e.x ? idx : __float_as_uint(e.y);
 }
 
-template <bool COUNT>
-__device__ __forceinline__ bool occluded(const DevScene& s, F3 p, F3 n, F3 dir, float max_dist, TraceCtx& c) {
-    // direct_lighting.cuh:36-51
+// Shadow ray of isOccluded (direct_lighting.cuh:36-51): origin pushed 1e-4
+// along the normal towards dir, tmax = distance - 1e-3 (or unbounded).
+struct ShadowRay {
+    F3 o, d;
+    float tmax;
+};
+__device__ __forceinline__ ShadowRay make_shadow(F3 p, F3 n, F3 dir, float max_dist) {
+    ShadowRay r;
     float sign = dot(dir, n) >= 0.0f ? 1.0f : -1.0f;
-    F3 o = p + n * (1e-4f * sign);
-    F3 d = normalize(dir);
-    float tmax = (max_dist < kInf) ? (max_dist - 1e-3f) : kInf;
-    c.shadow++;
-    return trace_any<COUNT>(s, o, d, tmax, c.stk, c.stride, c.cnt, c.overflow);
+    r.o = p + n * (1e-4f * sign);
+    r.d = normalize(dir);
+    r.tmax = (max_dist < kInf) ? (max_dist - 1e-3f) : kInf;
+    return r;
 }
 
-// sampleAreaLight (direct_lighting.cuh:75-121)
 template <bool COUNT>
-__device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+__device__ __forceinline__ bool occluded(const DevScene& s, const ShadowRay& r, TraceCtx& c) {
+    c.shadow++;
+    return trace_any<COUNT>(s, r.o, r.d, r.tmax, c.stk, c.stride, c.scnt, c.overflow);
+}
+
+// sampleAreaLight (direct_lighting.cuh:75-121) up to, not including, the
+// visibility test, which is its last rejection; sh gets the shadow ray.
+__device__ __forceinline__ LightSample sample_area_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
     LightSample ls;
     ls.valid = false;
     if (s.num_emissive == 0 || s.emissive == nullptr) return ls;
@@ -124,7 +135,7 @@ __device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n
     if (cl <= 1e-5f) return ls;
     float pdf = (__uint_as_float(ref.y) / area) * d2 / cl;
     if (pdf <= 0.0f) return ls;
-    if (occluded<COUNT>(s, p, n, wi, dist, c)) return ls;
+    sh = make_shadow(p, n, wi, dist);
     const pt_material& lm = s.materials[face.w];
     ls.wi = wi;
     ls.radiance = f3(lm.emissive.x, lm.emissive.y, lm.emissive.z) * lm.emissive_strength;
@@ -133,9 +144,17 @@ __device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n
     return ls;
 }
 
-// sampleEnvironmentLight (direct_lighting.cuh:123-154)
 template <bool COUNT>
-__device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+__device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+    ShadowRay sh;
+    LightSample ls = sample_area_nv(s, p, n, rng, sh);
+    if (ls.valid && occluded<COUNT>(s, sh, c)) ls.valid = false;
+    return ls;
+}
+
+// sampleEnvironmentLight (direct_lighting.cuh:123-154) without the final
+// visibility test; sh gets the shadow ray.
+__device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
     LightSample ls;
     ls.valid = false;
     float a1 = rng.next();
@@ -152,11 +171,19 @@ __device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n,
     if (cs <= 0.0f) return ls;
     float pdf = env_pdf(s, wi);
     if (pdf <= 0.0f) return ls;
-    if (occluded<COUNT>(s, p, n, wi, kInf, c)) return ls;
+    sh = make_shadow(p, n, wi, kInf);
     ls.wi = wi;
     ls.radiance = env_radiance(s, wi);
     ls.pdf = pdf;
     ls.valid = true;
+    return ls;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+    ShadowRay sh;
+    LightSample ls = sample_env_nv(s, p, n, rng, sh);
+    if (ls.valid && occluded<COUNT>(s, sh, c)) ls.valid = false;
     return ls;
 }
 
@@ -169,41 +196,49 @@ __device__ __forceinline__ F3 eval_direct(const LightSample& ls, const MatEval<f
     return ls.radiance * f3(b.f.x, b.f.y, b.f.z) * (mis / (ls.pdf + 1e-7f));
 }
 
+// One punctual light of evaluatePunctualLights (direct_lighting.cuh:158-200)
+// up to its visibility test: direction, unoccluded intensity and distance;
+// false when the light cannot contribute (too close, behind the surface).
+__device__ __forceinline__ bool punctual_light_sample(const DevScene& s, uint32_t i, F3 p, F3 n, F3& wi, F3& inten,
+                                                      float& dist) {
+    const pt_punctual_light& l = s.lights[i];
+    dist = kInf;
+    if (l.type == 0 || l.type == 2) {
+        F3 tl = f3(l.position.x, l.position.y, l.position.z) - p;
+        float d2 = dot(tl, tl);
+        if (d2 < 1e-8f) return false;
+        dist = sqrtf(d2);
+        wi = tl / dist;
+        float att = dist >= l.range ? 0.0f : 1.0f / (dist * dist + 1e-7f);  // punctual_light.h:36-39
+        if (l.type == 2) {                                                 // punctual_light.h:28-34
+            float ca = dot(normalize(-tl), f3(l.direction.x, l.direction.y, l.direction.z));
+            float sp;
+            if (ca < l.outer_cone) sp = 0.0f;
+            else if (ca > l.inner_cone) sp = 1.0f;
+            else {
+                float t = (ca - l.outer_cone) / (l.inner_cone - l.outer_cone + 1e-7f);
+                sp = t * t;
+            }
+            att *= sp;
+        }
+        inten = f3(l.color.x, l.color.y, l.color.z) * (l.intensity * att);
+    } else {
+        wi = -f3(l.direction.x, l.direction.y, l.direction.z);
+        inten = f3(l.color.x, l.color.y, l.color.z) * l.intensity;
+    }
+    return dot(n, wi) > 0.0f;
+}
+
 // evaluatePunctualLights (direct_lighting.cuh:158-200)
 template <bool COUNT>
 __device__ __forceinline__ F3 punctual(const DevScene& s, F3 p, const MatEval<float>& m, F3 wo, F3 n, TraceCtx& c) {
     F3 ld = f3(0.0f);
     if (s.num_lights == 0 || s.lights == nullptr) return ld;
     for (uint32_t i = 0; i < s.num_lights; ++i) {
-        const pt_punctual_light& l = s.lights[i];
-        F3 wi;
-        float dist = kInf;
-        F3 inten;
-        if (l.type == 0 || l.type == 2) {
-            F3 tl = f3(l.position.x, l.position.y, l.position.z) - p;
-            float d2 = dot(tl, tl);
-            if (d2 < 1e-8f) continue;
-            dist = sqrtf(d2);
-            wi = tl / dist;
-            float att = dist >= l.range ? 0.0f : 1.0f / (dist * dist + 1e-7f);  // punctual_light.h:36-39
-            if (l.type == 2) {                                                 // punctual_light.h:28-34
-                float ca = dot(normalize(-tl), f3(l.direction.x, l.direction.y, l.direction.z));
-                float sp;
-                if (ca < l.outer_cone) sp = 0.0f;
-                else if (ca > l.inner_cone) sp = 1.0f;
-                else {
-                    float t = (ca - l.outer_cone) / (l.inner_cone - l.outer_cone + 1e-7f);
-                    sp = t * t;
-                }
-                att *= sp;
-            }
-            inten = f3(l.color.x, l.color.y, l.color.z) * (l.intensity * att);
-        } else {
-            wi = -f3(l.direction.x, l.direction.y, l.direction.z);
-            inten = f3(l.color.x, l.color.y, l.color.z) * l.intensity;
-        }
-        if (dot(n, wi) <= 0.0f) continue;
-        if (occluded<COUNT>(s, p, n, wi, dist, c)) continue;
+        F3 wi, inten;
+        float dist;
+        if (!punctual_light_sample(s, i, p, n, wi, inten, dist)) continue;
+        if (occluded<COUNT>(s, make_shadow(p, n, wi, dist), c)) continue;
         BsdfEvalT<float> b = eval_bsdf(m, wo, wi, n, true);
         ld += inten * f3(b.f.x, b.f.y, b.f.z);
     }
@@ -312,108 +347,164 @@ __device__ __forceinline__ void camera_ray(const RenderParams& p, int x, int y, 
 }
 
 // ---- path ---------------------------------------------------------------------
+struct PathState {
+    F3 o, d;      // current ray
+    F3 tp;        // throughput
+    F3 lo;        // radiance gathered so far
+    float last_pdf;
+    int depth;
+    bool last_delta;
+};
+
+// Visibility policy of the plain megakernel: shadow rays are traced inline
+// and the contributions added at once (direct_lighting.cuh:220-249).
 template <bool COUNT>
-__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
-    F3 lo = f3(0.0f);
-    F3 tp = f3(1.0f);
-    F3 o, d;
-    camera_ray(p, x, y, rng, o, d);
-    float last_pdf = 0.0f;
-    bool last_delta = true;
-    const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
-    const bool has_env = s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 &&
-                         s.env_h > 0;
-    for (int depth = 0; depth < p.max_depth; ++depth) {
-        c.closest++;
-        Hit h = trace_closest<COUNT>(s, o, d, c.stk, c.stride, c.cnt, c.overflow);
-        if (h.tri == 0xFFFFFFFFu) {
-            F3 env = env_radiance(s, d);
-            if (!is_zero(env)) {
-                float mis = 1.0f;
-                if (depth > 0 && !last_delta) mis = power_heuristic(last_pdf, env_pdf(s, d));
-                lo += tp * env * mis;
-            }
-            break;
-        }
-        Surface sf = make_surface(s, o, d, h);
-        const pt_material& mat = s.materials[sf.mat];
-        bool entering = dot(sf.n, d) < 0.0f;
-        F3 gn;
-        if (mat.double_sided && !entering) {
-            gn = -sf.n;
-            entering = true;
-        } else {
-            gn = entering ? sf.n : -sf.n;
-        }
-        F3 wo = -d;
-        float tu = sf.u, tv = sf.v;
-        apply_uv_transform(mat, tu, tv);
-        F3 n = normal_map(s, mat, gn, sf, tu, tv);
-        MatEval<float> me = fetch_material(s, mat, tu, tv);
-        if (mat_emissive(mat)) {
+struct InlineNee {
+    TraceCtx& c;
+    __device__ __forceinline__ explicit InlineNee(TraceCtx& ctx) : c(ctx) {}
+    // The area term is added before the environment term, one at a time,
+    // so the wavefront schedule (one shadow queue per term) sums alike.
+    __device__ __forceinline__ void direct(const DevScene& s, PathState& st, bool va, const ShadowRay& sa, F3 ca,
+                                           bool ve, const ShadowRay& se, F3 ce) {
+        if (va && !occluded<COUNT>(s, sa, c)) st.lo += ca;
+        if (ve && !occluded<COUNT>(s, se, c)) st.lo += ce;
+    }
+    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, F3 p, const MatEval<float>& m,
+                                                    F3 wo, F3 n) {
+        st.lo += st.tp * punctual<COUNT>(s, p, m, wo, n, c);
+    }
+};
+
+// One bounce of pathTrace (path.cu:69-199) after the closest-hit query of the
+// current ray: returns true when the path continues with the new ray in st.
+// NEE contributions are premultiplied by the throughput and handed to the
+// visibility policy together with their shadow rays.
+template <class Nee>
+__device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p, const Hit& h, PathState& st,
+                                       Sampler& rng, Nee& nee) {
+    const int depth = st.depth;
+    if (h.tri == 0xFFFFFFFFu) {
+        F3 env = env_radiance(s, st.d);
+        if (!is_zero(env)) {
             float mis = 1.0f;
-            if (depth > 0 && !last_delta) mis = power_heuristic(last_pdf, emissive_pdf(s, sf.tri, sf.t, d));
-            lo += tp * (f3(mat.emissive.x, mat.emissive.y, mat.emissive.z) * mat.emissive_strength) * mis;
-            break;
+            if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf(s, st.d));
+            st.lo += st.tp * env * mis;
         }
-        if (mat.alpha_mode == 1) {  // path.cu:136-146 (raw uv)
-            float alpha = 1.0f;
-            if (tex_valid(s, mat.base_color_tex_id)) alpha = tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
-            if (alpha < mat.alpha_cutoff) {
-                o = sf.p + d * 1e-4f;
-                d = normalize(d);
-                continue;
-            }
-        } else if (mat.alpha_mode == 2) {  // :147-158
-            float alpha = mat.base_alpha;
-            if (tex_valid(s, mat.base_color_tex_id)) alpha *= tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
-            if (rng.next() >= alpha) {
-                o = sf.p + d * 1e-4f;
-                d = normalize(d);
-                continue;
-            }
+        return false;
+    }
+    Surface sf = make_surface(s, st.o, st.d, h);
+    const pt_material& mat = s.materials[sf.mat];
+    bool entering = dot(sf.n, st.d) < 0.0f;
+    F3 gn;
+    if (mat.double_sided && !entering) {
+        gn = -sf.n;
+        entering = true;
+    } else {
+        gn = entering ? sf.n : -sf.n;
+    }
+    F3 wo = -st.d;
+    float tu = sf.u, tv = sf.v;
+    apply_uv_transform(mat, tu, tv);
+    F3 n = normal_map(s, mat, gn, sf, tu, tv);
+    MatEval<float> me = fetch_material(s, mat, tu, tv);
+    if (mat_emissive(mat)) {
+        float mis = 1.0f;
+        if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, emissive_pdf(s, sf.tri, sf.t, st.d));
+        st.lo += st.tp * (f3(mat.emissive.x, mat.emissive.y, mat.emissive.z) * mat.emissive_strength) * mis;
+        return false;
+    }
+    if (mat.alpha_mode == 1) {  // path.cu:136-146 (raw uv)
+        float alpha = 1.0f;
+        if (tex_valid(s, mat.base_color_tex_id)) alpha = tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
+        if (alpha < mat.alpha_cutoff) {
+            st.o = sf.p + st.d * 1e-4f;
+            st.d = normalize(st.d);
+            return true;
         }
-        bool delta_surface = (me.metallic >= 1.0f && me.roughness < 0.01f) ||
-                             (me.spec_trans >= 1.0f && me.roughness < 0.01f);
-        if (!delta_surface) {
-            F3 ld = f3(0.0f);
-            if (has_area && has_env) {
-                ld += eval_direct(sample_area<COUNT>(s, sf.p, n, rng, c), me, wo, n);
-                ld += eval_direct(sample_env<COUNT>(s, sf.p, n, rng, c), me, wo, n);
-            } else if (has_area) {
-                ld = eval_direct(sample_area<COUNT>(s, sf.p, n, rng, c), me, wo, n);
-            } else if (has_env) {
-                ld = eval_direct(sample_env<COUNT>(s, sf.p, n, rng, c), me, wo, n);
-            }
-            lo += tp * ld;
-            lo += tp * punctual<COUNT>(s, sf.p, me, wo, n, c);
-        }
-        BsdfSample bs = sample_bsdf(me, wo, n, rng, entering);
-        if (!bs.valid) break;
-        tp *= bs.weight;
-        bool transmitted = dot(bs.wi, gn) < 0.0f;
-        F3 off = transmitted ? -gn : gn;
-        o = sf.p + off * 1e-4f;
-        d = normalize(bs.wi);
-        if (transmitted && me.thickness > 0.0f && me.attenuation_distance < 1e20f) {  // :178-187
-            float t = me.thickness, ad = me.attenuation_distance;
-            F3 ac = me.attenuation_color;
-            tp *= f3(expf(-t * fmaxf(-logf(fmaxf(ac.x, 1e-6f)), 0.0f) / ad),
-                     expf(-t * fmaxf(-logf(fmaxf(ac.y, 1e-6f)), 0.0f) / ad),
-                     expf(-t * fmaxf(-logf(fmaxf(ac.z, 1e-6f)), 0.0f) / ad));
-        }
-        last_pdf = bs.pdf;
-        last_delta = bs.delta;
-        if (depth > 2) {  // :193-198
-            float surv = fminf(max3(tp), 0.95f);
-            if (rng.next() >= surv) break;
-            tp /= surv;
+    } else if (mat.alpha_mode == 2) {  // :147-158
+        float alpha = mat.base_alpha;
+        if (tex_valid(s, mat.base_color_tex_id)) alpha *= tex_fetch(s, mat.base_color_tex_id, sf.u, sf.v).w;
+        if (rng.next() >= alpha) {
+            st.o = sf.p + st.d * 1e-4f;
+            st.d = normalize(st.d);
+            return true;
         }
     }
-    lo.x = fminf(lo.x, 100.0f);
-    lo.y = fminf(lo.y, 100.0f);
-    lo.z = fminf(lo.z, 100.0f);
-    return lo;
+    bool delta_surface =
+        (me.metallic >= 1.0f && me.roughness < 0.01f) || (me.spec_trans >= 1.0f && me.roughness < 0.01f);
+    if (!delta_surface) {  // :161-166
+        const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
+        const bool has_env =
+            s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 && s.env_h > 0;
+        ShadowRay sa, se;
+        bool va = false, ve = false;
+        F3 ca = f3(0.0f), ce = f3(0.0f);
+        if (has_area) {
+            LightSample ls = sample_area_nv(s, sf.p, n, rng, sa);
+            if (ls.valid) {
+                ca = st.tp * eval_direct(ls, me, wo, n);
+                va = !is_zero(ca);
+            }
+        }
+        if (has_env) {
+            LightSample ls = sample_env_nv(s, sf.p, n, rng, se);
+            if (ls.valid) {
+                ce = st.tp * eval_direct(ls, me, wo, n);
+                ve = !is_zero(ce);
+            }
+        }
+        nee.direct(s, st, va, sa, ca, ve, se, ce);
+        if (s.num_lights > 0 && s.lights != nullptr) nee.punctual_lights(s, st, sf.p, me, wo, n);
+    }
+    BsdfSample bs = sample_bsdf(me, wo, n, rng, entering);
+    if (!bs.valid) return false;
+    st.tp *= bs.weight;
+    bool transmitted = dot(bs.wi, gn) < 0.0f;
+    F3 off = transmitted ? -gn : gn;
+    st.o = sf.p + off * 1e-4f;
+    st.d = normalize(bs.wi);
+    if (transmitted && me.thickness > 0.0f && me.attenuation_distance < 1e20f) {  // :178-187
+        float t = me.thickness, ad = me.attenuation_distance;
+        F3 ac = me.attenuation_color;
+        st.tp *= f3(expf(-t * fmaxf(-logf(fmaxf(ac.x, 1e-6f)), 0.0f) / ad),
+                    expf(-t * fmaxf(-logf(fmaxf(ac.y, 1e-6f)), 0.0f) / ad),
+                    expf(-t * fmaxf(-logf(fmaxf(ac.z, 1e-6f)), 0.0f) / ad));
+    }
+    st.last_pdf = bs.pdf;
+    st.last_delta = bs.delta;
+    if (depth > 2) {  // :193-198
+        float surv = fminf(max3(st.tp), 0.95f);
+        if (rng.next() >= surv) return false;
+        st.tp /= surv;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void begin_path(const RenderParams& p, int x, int y, Sampler& rng, PathState& st) {
+    camera_ray(p, x, y, rng, st.o, st.d);
+    st.tp = f3(1.0f);
+    st.lo = f3(0.0f);
+    st.last_pdf = 0.0f;
+    st.last_delta = true;
+    st.depth = 0;
+}
+
+// path.cu:201-204
+__device__ __forceinline__ F3 clamp_radiance(F3 lo) {
+    return f3(fminf(lo.x, 100.0f), fminf(lo.y, 100.0f), fminf(lo.z, 100.0f));
+}
+
+template <bool COUNT>
+__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
+    PathState st;
+    begin_path(p, x, y, rng, st);
+    InlineNee<COUNT> nee(c);
+    for (; st.depth < p.max_depth; ++st.depth) {
+        c.closest++;
+        Hit h = trace_closest<COUNT>(s, st.o, st.d, c.stk, c.stride, c.cnt, c.overflow);
+        if (!bounce(s, p, h, st, rng, nee)) break;
+    }
+    return clamp_radiance(st.lo);
 }
 
 }  // namespace ptd

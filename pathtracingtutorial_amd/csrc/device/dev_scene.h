@@ -68,6 +68,7 @@ struct RenderParams {
 
 struct DevCounters {
     unsigned long long closest_rays, shadow_rays, node_visits, tri_tests, paths, stack_overflows;
+    unsigned long long shadow_node_visits, shadow_tri_tests;  // the any-hit share of node_visits / tri_tests
 };
 
 }  // namespace ptd

@@ -28,14 +28,17 @@ template <bool COUNT>
 __device__ __forceinline__ void flush_counters(DevCounters* ctr, const TraceCtx& c, uint32_t paths) {
     if (!COUNT) return;
     unsigned long long v0 = wave_sum_u64(c.closest), v1 = wave_sum_u64(c.shadow), v2 = wave_sum_u64(c.cnt.nodes),
-                       v3 = wave_sum_u64(c.cnt.tris), v4 = wave_sum_u64(paths), v5 = wave_sum_u64(c.overflow);
+                       v3 = wave_sum_u64(c.cnt.tris), v4 = wave_sum_u64(paths), v5 = wave_sum_u64(c.overflow),
+                       v6 = wave_sum_u64(c.scnt.nodes), v7 = wave_sum_u64(c.scnt.tris);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(&ctr->closest_rays, v0);
         atomicAdd(&ctr->shadow_rays, v1);
-        atomicAdd(&ctr->node_visits, v2);
-        atomicAdd(&ctr->tri_tests, v3);
+        atomicAdd(&ctr->node_visits, v2 + v6);
+        atomicAdd(&ctr->tri_tests, v3 + v7);
         atomicAdd(&ctr->paths, v4);
         atomicAdd(&ctr->stack_overflows, v5);
+        atomicAdd(&ctr->shadow_node_visits, v6);
+        atomicAdd(&ctr->shadow_tri_tests, v7);
     }
 }
 
@@ -52,6 +55,7 @@ __global__ __launch_bounds__(kThreads) void k_render_plain(DevScene s, RenderPar
     c.stk = stack + threadIdx.x;
     c.stride = kThreads;
     c.cnt.nodes = c.cnt.tris = 0;
+    c.scnt.nodes = c.scnt.tris = 0;
     c.overflow = c.closest = c.shadow = 0;
     uint32_t paths = 0;
     if (x < p.width && y < p.height) {

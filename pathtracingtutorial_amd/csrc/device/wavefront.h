@@ -1,0 +1,50 @@
+// Wavefront form of the integrator (host interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "dev_scene.h"
+
+namespace ptd {
+
+struct WavefrontConfig {
+    int chunk_spp = 8;          // samples per pixel held in flight per chunk
+    int extend_block = 128;     // threads per block of the traversal kernels
+    int shade_block = 128;      // threads per block of the shading kernel
+    int waves_per_simd = 0;     // persistent grid size: 0 = from the occupancy calculator
+    int sort_by_material = 1;   // counting-sort the shading queue by material
+    int use_graph = 1;          // capture the bounce loop of a chunk in a hipGraph
+    int time_stages = 0;        // bracket every launch with HIP events (no graph); see wavefront_stage_times
+};
+
+// Stages timed when WavefrontConfig::time_stages is set.
+enum WavefrontStage {
+    WF_GENERATE = 0,  // camera rays
+    WF_EXTEND = 1,    // closest-hit traversal of the ray queue
+    WF_SORT = 2,      // material histogram, scan and scatter
+    WF_SHADE = 3,     // surface interaction, BSDF, NEE sampling, next rays
+    WF_SHADOW = 4,    // any-hit traversal of the shadow queues
+    WF_CONTROL = 5,   // queue resets, chunk bookkeeping, finalize, film
+    WF_STAGES = 6
+};
+
+class Wavefront;
+
+// Owns path-state buffers, queues and the captured graph for one scene/film
+// pair; rebuilt when sizes or settings change.
+Wavefront* wavefront_create();
+void wavefront_destroy(Wavefront* w);
+// Renders one batch (params.spf samples per pixel) into film_new/film_acc
+// exactly as the megakernel does (batch mean + running mean).  ms = GPU time
+// of the batch from HIP events on `stream`.
+hipError_t wavefront_render(Wavefront* w, const DevScene& s, uint64_t scene_generation, const RenderParams& p,
+                            const WavefrontConfig& cfg, float4* film_new, float4* film_acc, DevCounters* ctr,
+                            bool count, hipStream_t stream, float* ms, std::string* err);
+// GPU milliseconds and launch counts per stage of the last batch rendered
+// with time_stages set (zeros otherwise).
+void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches);
+
+}  // namespace ptd

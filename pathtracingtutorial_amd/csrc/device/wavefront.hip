@@ -1,0 +1,852 @@
+// Wavefront form of the path tracer for gfx950.
+//
+// The estimator is the megakernel's (dev_integrator.h: same bounce() code,
+// same random numbers per path); only the schedule changes:
+//
+//  * A frame batch of spf samples per pixel is cut into chunks of S samples
+//    for every pixel; path p of a chunk is (pixel p / S, sample p % S).
+//  * Path state lives in HBM as SoA float4 arrays (ray origin, direction +
+//    last pdf, throughput + depth/flags, radiance, sampler state, hit).
+//  * Per bounce: a persistent closest-hit kernel over the ray queue (lean:
+//    traversal only, LDS stack), an optional counting sort of the queue by
+//    material, a persistent shading kernel that appends next rays and NEE
+//    shadow rays to queues, and persistent any-hit kernels over the area,
+//    environment and punctual shadow queues.  Area and environment shadow
+//    rays go to separate queues processed by separate launches, so each path
+//    gains at most one contribution per launch and no atomics are needed:
+//    the image is bitwise reproducible.
+//  * Work is grabbed per wave (64 items, one atomic) from 8 queue segments,
+//    one per XCD, so each XCD's L2 sees contiguous (coherent) rays; a wave
+//    steals from the other segments once its own is empty.
+//  * The bounce loop of a chunk is captured once in a hipGraph (parameters
+//    and the chunk index live in device memory) and replayed per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dev_integrator.h"
+#include "wavefront.h"
+
+namespace ptd {
+
+namespace {
+
+constexpr int kXcd = 8;
+// control words
+constexpr int C_Q0 = 0, C_Q1 = 1, C_SA = 2, C_SE = 3, C_SP = 4, C_CHUNK = 5;
+constexpr int C_CUR_EXT = 8, C_CUR_SHADE = 16, C_CUR_SA = 24, C_CUR_SE = 32, C_CUR_SP = 40, C_CUR_SORT = 48;
+constexpr int C_WORDS = 64;
+
+struct WfState {
+    float4* o;    // xyz origin
+    float4* d;    // xyz direction, w = last bsdf pdf
+    float4* tp;   // xyz throughput, w = bits: depth | last_delta << 31
+    float4* lo;   // xyz radiance
+    uint4* rng;   // x pixel, y Sobol index, z Sobol dim or PCG state, w pixel hash
+    float4* hit;  // t, u, v, triangle bits
+    uint32_t* q[2];
+    uint32_t* sorted;
+    float4 *sa_o, *sa_d, *sa_c;  // area shadow queue: origin + tmax, direction + path bits, contribution
+    float4 *se_o, *se_d, *se_c;  // environment shadow queue
+    float4 *sp_o, *sp_d, *sp_c;  // punctual shadow queue
+    uint32_t* hist;              // material histogram / offsets (num_materials + 1)
+    float4* sum;                 // per pixel sum over the batch
+    uint32_t* ctrl;
+    const RenderParams* params;
+    uint32_t capacity, sp_capacity;
+    uint32_t chunk_spp;
+};
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+// Wave-aggregated append: every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll(static_cast<long long>(mask)) - 1;
+        if (lane_id() == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+        base = __shfl(base, leader, 64);
+    }
+    unsigned long long below = (lane_id() == 0) ? 0ull : (mask & ((1ull << lane_id()) - 1ull));
+    return base + static_cast<uint32_t>(__popcll(below));
+}
+
+// Grabs the next 64 queue items for this wave: segment seg of 8 (one per
+// XCD) first, then the others.  Returns false when the queue is exhausted.
+__device__ __forceinline__ bool grab(uint32_t* cursors, uint32_t n, int& seg, int& tried, uint32_t& i, uint32_t& end) {
+    while (tried < kXcd) {
+        uint32_t s0 = static_cast<uint32_t>((static_cast<unsigned long long>(n) * seg) / kXcd);
+        uint32_t s1 = static_cast<uint32_t>((static_cast<unsigned long long>(n) * (seg + 1)) / kXcd);
+        uint32_t off = 0;
+        if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
+        off = __shfl(off, 0, 64);
+        if (s0 + off < s1) {
+            i = s0 + off + static_cast<uint32_t>(lane_id());
+            end = s1;
+            return true;
+        }
+        seg = (seg + 1) % kXcd;
+        ++tried;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void load_path(const WfState& w, uint32_t p, PathState& st, uint4& r) {
+    float4 o = w.o[p], d = w.d[p], tp = w.tp[p], lo = w.lo[p];
+    st.o = f3(o);
+    st.d = f3(d);
+    st.last_pdf = d.w;
+    st.tp = f3(tp);
+    uint32_t bits = __float_as_uint(tp.w);
+    st.depth = static_cast<int>(bits & 0xFFFFu);
+    st.last_delta = (bits >> 31) != 0;
+    st.lo = f3(lo);
+    r = w.rng[p];
+}
+
+__device__ __forceinline__ void store_path(const WfState& w, uint32_t p, const PathState& st, uint4 r) {
+    w.o[p] = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
+    w.d[p] = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
+    uint32_t bits = static_cast<uint32_t>(st.depth) | (st.last_delta ? 0x80000000u : 0u);
+    w.tp[p] = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
+    w.lo[p] = make_float4(st.lo.x, st.lo.y, st.lo.z, 0.0f);
+    w.rng[p] = r;
+}
+
+__device__ __forceinline__ Sampler sampler_of(int type, uint4 r) {
+    Sampler s;
+    s.type = static_cast<uint32_t>(type);
+    s.index = r.y;
+    s.dim = r.z;
+    s.state = r.z;
+    s.pix_hash = r.w;
+    return s;
+}
+__device__ __forceinline__ uint4 sampler_state(const Sampler& s, uint4 r) {
+    r.z = s.type == 1 ? s.dim : s.state;
+    return r;
+}
+
+template <bool COUNT, bool SHADOW>
+__device__ __forceinline__ void flush(DevCounters* ctr, unsigned long long rays, unsigned long long nodes,
+                                      unsigned long long tris, unsigned long long overflow) {
+    if (!COUNT) return;
+    rays = wave_sum_u64(rays);
+    nodes = wave_sum_u64(nodes);
+    tris = wave_sum_u64(tris);
+    overflow = wave_sum_u64(overflow);
+    if (lane_id() == 0) {
+        atomicAdd(SHADOW ? &ctr->shadow_rays : &ctr->closest_rays, rays);
+        atomicAdd(&ctr->node_visits, nodes);
+        atomicAdd(&ctr->tri_tests, tris);
+        if (SHADOW) {
+            atomicAdd(&ctr->shadow_node_visits, nodes);
+            atomicAdd(&ctr->shadow_tri_tests, tris);
+        }
+        if (overflow) atomicAdd(&ctr->stack_overflows, overflow);
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void k_wf_batch_begin(uint32_t* ctrl) {
+    if (threadIdx.x == 0) ctrl[C_CHUNK] = 0;
+}
+
+__global__ void k_wf_chunk_begin(WfState w, uint32_t npaths) {
+    if (threadIdx.x == 0) w.ctrl[C_Q0] = w.params->max_depth > 0 ? npaths : 0u;
+}
+
+__global__ void k_wf_chunk_end(uint32_t* ctrl) {
+    if (threadIdx.x == 0) ctrl[C_CHUNK] += 1;
+}
+
+// Camera rays of chunk ctrl[C_CHUNK]: path p = (pixel p / S, sample p % S).
+__global__ void k_wf_generate(WfState w, uint32_t npaths) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npaths) return;
+    const RenderParams& rp = *w.params;
+    const uint32_t S = w.chunk_spp;
+    uint32_t pixel = p / S, j = p % S;
+    int x = static_cast<int>(pixel % static_cast<uint32_t>(rp.width));
+    int y = static_cast<int>(pixel / static_cast<uint32_t>(rp.width));
+    uint32_t sample = w.ctrl[C_CHUNK] * S + j;
+    Sampler rng{};
+    if (rp.sampler == 1) rng.init_sobol(x, y, rp.frame_count * static_cast<uint32_t>(rp.spf) + sample, rp.seed);
+    else rng.init_pcg(x, y, rp.frame_count, sample, rp.seed);
+    PathState st;
+    begin_path(rp, x, y, rng, st);
+    uint4 r = make_uint4(pixel, rng.index, 0u, rng.pix_hash);
+    r = sampler_state(rng, r);
+    store_path(w, p, st, r);
+    w.q[0][p] = p;
+}
+
+__global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot, uint32_t* hist, uint32_t nbins) {
+    uint32_t t = threadIdx.x;
+    if (t == 0) {
+        ctrl[out_slot] = 0;
+        ctrl[C_SA] = 0;
+        ctrl[C_SE] = 0;
+        ctrl[C_SP] = 0;
+    }
+    for (uint32_t k = C_CUR_EXT + t; k < C_WORDS; k += blockDim.x) ctrl[k] = 0;
+    if (hist)
+        for (uint32_t k = t; k < nbins; k += blockDim.x) hist[k] = 0;
+}
+
+template <bool COUNT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int in_slot, DevCounters* ctr) {
+    __shared__ uint32_t stack[kStackSize * BLOCK];
+    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* q = w.q[in_slot];
+    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    uint32_t i, end;
+    TraceCounters cnt{0, 0};
+    uint32_t ovf = 0;
+    unsigned long long rays = 0;
+    while (grab(w.ctrl + C_CUR_EXT, n, seg, tried, i, end)) {
+        if (i < end) {
+            uint32_t p = q[i];
+            float4 o = w.o[p], d = w.d[p];
+            Hit h = trace_closest<COUNT>(s, f3(o), f3(d), stack + threadIdx.x, BLOCK, cnt, ovf);
+            w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+            rays++;
+        }
+    }
+    flush<COUNT, false>(ctr, rays, cnt.nodes, cnt.tris, ovf);
+}
+
+// Material keys of the queue (misses go to the last bin), then a counting sort.
+__global__ void k_wf_hist(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* q = w.q[in_slot];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t tri = __float_as_uint(w.hit[q[i]].w);
+        uint32_t key = tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
+        atomicAdd(&w.hist[key], 1u);
+    }
+}
+
+__global__ void k_wf_hist_scan(uint32_t* hist, uint32_t nbins) {
+    // single block: exclusive scan in place (nbins <= 4096)
+    __shared__ uint32_t part[256];
+    uint32_t t = threadIdx.x;
+    uint32_t per = (nbins + 255) / 256;
+    uint32_t b0 = t * per, b1 = min(b0 + per, nbins);
+    uint32_t sum = 0;
+    for (uint32_t k = b0; k < b1; ++k) sum += hist[k];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < 256; ++k) {
+            uint32_t v = part[k];
+            part[k] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    uint32_t acc = part[t];
+    for (uint32_t k = b0; k < b1; ++k) {
+        uint32_t v = hist[k];
+        hist[k] = acc;
+        acc += v;
+    }
+}
+
+__global__ void k_wf_scatter(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* q = w.q[in_slot];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t p = q[i];
+        uint32_t tri = __float_as_uint(w.hit[p].w);
+        uint32_t key = tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
+        w.sorted[atomicAdd(&w.hist[key], 1u)] = p;
+    }
+}
+
+// Visibility policy of the wavefront: area and environment shadow rays are
+// kept in registers and appended after the bounce (convergently); punctual
+// shadow rays are appended at once.
+struct DeferNee {
+    bool va = false, ve = false;
+    ShadowRay sa, se;
+    F3 ca, ce;
+    uint32_t path;
+    uint32_t* sp_count;
+    uint32_t sp_capacity;
+    float4 *sp_o, *sp_d, *sp_c;
+    __device__ __forceinline__ void direct(const DevScene&, PathState&, bool a, const ShadowRay& ra, F3 xa, bool e,
+                                           const ShadowRay& re, F3 xe) {
+        va = a;
+        sa = ra;
+        ca = xa;
+        ve = e;
+        se = re;
+        ce = xe;
+    }
+    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, F3 p, const MatEval<float>& m,
+                                                    F3 wo, F3 n) {
+        for (uint32_t i = 0; i < s.num_lights; ++i) {
+            F3 wi, inten;
+            float dist;
+            if (!punctual_light_sample(s, i, p, n, wi, inten, dist)) continue;
+            BsdfEvalT<float> b = eval_bsdf(m, wo, wi, n, true);
+            F3 c = st.tp * (inten * f3(b.f.x, b.f.y, b.f.z));
+            if (is_zero(c)) continue;
+            uint32_t k = atomicAdd(sp_count, 1u);
+            if (k >= sp_capacity) continue;  // cannot happen: capacity = paths x lights
+            ShadowRay r = make_shadow(p, n, wi, dist);
+            sp_o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
+            sp_d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(path));
+            sp_c[k] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+    }
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_shade(DevScene s, WfState w, int in_slot, int use_sorted) {
+    const RenderParams& rp = *w.params;
+    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
+    const int out_slot = in_slot ^ 1;
+    uint32_t* qout = w.q[out_slot];
+    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    uint32_t i, end;
+    while (grab(w.ctrl + C_CUR_SHADE, n, seg, tried, i, end)) {
+        const bool active = i < end;
+        uint32_t p = 0;
+        PathState st;
+        uint4 r = make_uint4(0, 0, 0, 0);
+        bool cont = false;
+        DeferNee nee;
+        nee.sp_count = &w.ctrl[C_SP];
+        nee.sp_capacity = w.sp_capacity;
+        nee.sp_o = w.sp_o;
+        nee.sp_d = w.sp_d;
+        nee.sp_c = w.sp_c;
+        if (active) {
+            p = q[i];
+            nee.path = p;
+            load_path(w, p, st, r);
+            float4 hv = w.hit[p];
+            Hit h;
+            h.t = hv.x;
+            h.u = hv.y;
+            h.v = hv.z;
+            h.tri = __float_as_uint(hv.w);
+            Sampler rng = sampler_of(rp.sampler, r);
+            cont = bounce(s, rp, h, st, rng, nee);
+            r = sampler_state(rng, r);
+            if (cont) st.depth += 1;
+            cont = cont && st.depth < rp.max_depth;
+            store_path(w, p, st, r);
+        }
+        const bool pa = active && nee.va, pe = active && nee.ve;
+        uint32_t ka = wave_append(&w.ctrl[C_SA], pa);
+        uint32_t ke = wave_append(&w.ctrl[C_SE], pe);
+        uint32_t ko = wave_append(&w.ctrl[out_slot], cont);
+        if (pa) {
+            w.sa_o[ka] = make_float4(nee.sa.o.x, nee.sa.o.y, nee.sa.o.z, nee.sa.tmax);
+            w.sa_d[ka] = make_float4(nee.sa.d.x, nee.sa.d.y, nee.sa.d.z, __uint_as_float(p));
+            w.sa_c[ka] = make_float4(nee.ca.x, nee.ca.y, nee.ca.z, 0.0f);
+        }
+        if (pe) {
+            w.se_o[ke] = make_float4(nee.se.o.x, nee.se.o.y, nee.se.o.z, nee.se.tmax);
+            w.se_d[ke] = make_float4(nee.se.d.x, nee.se.d.y, nee.se.d.z, __uint_as_float(p));
+            w.se_c[ke] = make_float4(nee.ce.x, nee.ce.y, nee.ce.z, 0.0f);
+        }
+        if (cont) qout[ko] = p;
+    }
+}
+
+// Any-hit over a shadow queue; unoccluded entries add their contribution.
+template <bool COUNT, int BLOCK, bool ATOMIC>
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(DevScene s, WfState w, int count_slot, int cur_slot,
+                                                     const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                     const float4* __restrict__ qc, DevCounters* ctr) {
+    __shared__ uint32_t stack[kStackSize * BLOCK];
+    uint32_t n = w.ctrl[count_slot];
+    if (ATOMIC) n = min(n, w.sp_capacity);
+    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    uint32_t i, end;
+    TraceCounters cnt{0, 0};
+    uint32_t ovf = 0;
+    unsigned long long rays = 0;
+    while (grab(w.ctrl + cur_slot, n, seg, tried, i, end)) {
+        if (i < end) {
+            float4 o = qo[i], d = qd[i];
+            rays++;
+            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf)) {
+                uint32_t p = __float_as_uint(d.w);
+                float4 c = qc[i];
+                if (ATOMIC) {
+                    atomicAdd(&w.lo[p].x, c.x);
+                    atomicAdd(&w.lo[p].y, c.y);
+                    atomicAdd(&w.lo[p].z, c.z);
+                } else {
+                    float4 l = w.lo[p];
+                    w.lo[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+                }
+            }
+        }
+    }
+    flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
+}
+
+// Per pixel: add the clamped radiance of this chunk's samples, in order.
+__global__ void k_wf_finalize(WfState w, uint32_t npixels) {
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= npixels) return;
+    const uint32_t S = w.chunk_spp;
+    // samples are added one at a time in sample order, as the megakernel does
+    F3 sum = w.ctrl[C_CHUNK] == 0 ? f3(0.0f) : f3(w.sum[px]);
+    for (uint32_t j = 0; j < S; ++j) sum += clamp_radiance(f3(w.lo[px * S + j]));
+    w.sum[px] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+}
+
+// Batch mean to the films (path.cu:228-241).
+__global__ void k_wf_film(WfState w, uint32_t npixels, float4* __restrict__ film_new, float4* __restrict__ film_acc,
+                          DevCounters* ctr, uint32_t count_paths) {
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= npixels) return;
+    const RenderParams& rp = *w.params;
+    float4 sm = w.sum[px];
+    F3 acc = f3(sm) / static_cast<float>(rp.spf);
+    float4 nw = make_float4(acc.x, acc.y, acc.z, 1.0f);
+    film_new[px] = nw;
+    float4 a = film_acc[px];
+    float inv = 1.0f / static_cast<float>(rp.frame_count + 1u);
+    a.x = a.x * (1.0f - inv) + nw.x * inv;
+    a.y = a.y * (1.0f - inv) + nw.y * inv;
+    a.z = a.z * (1.0f - inv) + nw.z * inv;
+    a.w = a.w * (1.0f - inv) + nw.w * inv;
+    film_acc[px] = a;
+    if (count_paths && px == 0) atomicAdd(&ctr->paths, static_cast<unsigned long long>(npixels) * rp.spf);
+}
+
+template <class T>
+bool dalloc(T*& p, size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+
+// Event pairs around launches, summed per stage after the batch.
+struct StageTimer {
+    hipStream_t stream = nullptr;
+    hipEvent_t open = nullptr;
+    int open_stage = 0;
+    std::vector<int> stages;
+    std::vector<hipEvent_t> begins, ends;
+    void begin(int stage) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, stream);
+        open = e;
+        open_stage = stage;
+    }
+    void end() {
+        if (!open) return;
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, stream);
+        stages.push_back(open_stage);
+        begins.push_back(open);
+        ends.push_back(e);
+        open = nullptr;
+    }
+    void collect(float* ms, uint32_t* n) {
+        for (int k = 0; k < WF_STAGES; ++k) {
+            ms[k] = 0.0f;
+            n[k] = 0;
+        }
+        for (size_t i = 0; i < stages.size(); ++i) {
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, begins[i], ends[i]);
+            ms[stages[i]] += t;
+            n[stages[i]] += 1;
+            (void)hipEventDestroy(begins[i]);
+            (void)hipEventDestroy(ends[i]);
+        }
+        stages.clear();
+        begins.clear();
+        ends.clear();
+    }
+};
+
+struct Timed {
+    StageTimer* t;
+    Timed(StageTimer* timer, int stage) : t(timer) {
+        if (t) t->begin(stage);
+    }
+    ~Timed() {
+        if (t) t->end();
+    }
+};
+
+class Wavefront {
+public:
+    WfState st{};
+    float stage_ms[WF_STAGES] = {};
+    uint32_t stage_launches[WF_STAGES] = {};
+    RenderParams* d_params = nullptr;
+    uint32_t capacity = 0, sp_capacity = 0, pixels = 0, nbins = 0;
+    int num_cus = 0;
+    // graph cache
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t key[10] = {};
+
+    ~Wavefront() { release(); }
+
+    void release_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    void release() {
+        release_graph();
+        void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
+                      st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.hist, st.sum, st.ctrl,
+                      d_params};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        st = WfState{};
+        d_params = nullptr;
+        capacity = sp_capacity = pixels = nbins = 0;
+    }
+
+    bool ensure(uint32_t npaths, uint32_t npix, uint32_t nlights, uint32_t nmat) {
+        bool ok = true;
+        if (npaths > capacity) {
+            ok = ok && dalloc(st.o, npaths) && dalloc(st.d, npaths) && dalloc(st.tp, npaths) &&
+                 dalloc(st.lo, npaths) && dalloc(st.rng, npaths) && dalloc(st.hit, npaths) &&
+                 dalloc(st.q[0], npaths) && dalloc(st.q[1], npaths) && dalloc(st.sorted, npaths) &&
+                 dalloc(st.sa_o, npaths) && dalloc(st.sa_d, npaths) && dalloc(st.sa_c, npaths) &&
+                 dalloc(st.se_o, npaths) && dalloc(st.se_d, npaths) && dalloc(st.se_c, npaths);
+            capacity = npaths;
+            release_graph();
+        }
+        const uint64_t spc64 = nlights ? static_cast<uint64_t>(npaths) * nlights : 1u;
+        if (spc64 > 0xFFFFFFFFull) {
+            release();
+            return false;
+        }
+        const uint32_t spc = static_cast<uint32_t>(spc64);
+        if (spc > sp_capacity) {
+            ok = ok && dalloc(st.sp_o, spc) && dalloc(st.sp_d, spc) && dalloc(st.sp_c, spc);
+            sp_capacity = spc;
+            release_graph();
+        }
+        if (npix > pixels) {
+            ok = ok && dalloc(st.sum, npix);
+            pixels = npix;
+            release_graph();
+        }
+        if (nmat + 2 > nbins) {
+            ok = ok && dalloc(st.hist, nmat + 2);
+            nbins = nmat + 2;
+            release_graph();
+        }
+        if (!st.ctrl) ok = ok && dalloc(st.ctrl, C_WORDS);
+        if (!d_params) ok = ok && dalloc(d_params, 1);
+        if (!ok) {
+            release();
+            (void)hipGetLastError();  // the failed allocation is reported by the caller, not by the next launch
+            return false;
+        }
+        st.params = d_params;
+        st.capacity = capacity;
+        st.sp_capacity = sp_capacity;
+        return ok;
+    }
+};
+
+Wavefront* wavefront_create() { return new Wavefront(); }
+void wavefront_destroy(Wavefront* w) { delete w; }
+
+namespace {
+
+int grid_for(const void* kernel, int block, int num_cus, int waves_per_simd) {
+    int per_cu = 0;
+    if (waves_per_simd > 0) {
+        per_cu = std::max(1, waves_per_simd * 4 * 64 / block);
+    } else if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0) {
+        per_cu = 1;
+    }
+    return num_cus * per_cu;
+}
+
+template <bool COUNT, int B>
+void launch_extend(int grid, hipStream_t s, const DevScene& sc, const WfState& w, int slot, DevCounters* c) {
+    hipLaunchKernelGGL((k_wf_extend<COUNT, B>), dim3(grid), dim3(B), 0, s, sc, w, slot, c);
+}
+template <bool COUNT, int B, bool ATOMIC>
+void launch_shadow(int grid, hipStream_t s, const DevScene& sc, const WfState& w, int cslot, int cur,
+                   const float4* o, const float4* d, const float4* c, DevCounters* ctr) {
+    hipLaunchKernelGGL((k_wf_shadow<COUNT, B, ATOMIC>), dim3(grid), dim3(B), 0, s, sc, w, cslot, cur, o, d, c, ctr);
+}
+
+// Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
+// computed beforehand (no runtime queries while a graph is being captured).
+struct Grids {
+    int extend, shade, sort;
+};
+
+template <bool COUNT, int EB, int SB>
+void enqueue_chunk(Wavefront& wf, const DevScene& s, const RenderParams& p, const WavefrontConfig& cfg,
+                   const Grids& g, uint32_t npaths, uint32_t npix, DevCounters* ctr, hipStream_t stream,
+                   StageTimer* tm) {
+    const WfState& w = wf.st;
+    const uint32_t nb = s.num_materials + 1;  // materials + miss bin
+    {
+        Timed t(tm, WF_CONTROL);
+        hipLaunchKernelGGL(k_wf_chunk_begin, dim3(1), dim3(64), 0, stream, w, npaths);
+    }
+    {
+        Timed t(tm, WF_GENERATE);
+        hipLaunchKernelGGL(k_wf_generate, dim3((npaths + 255) / 256), dim3(256), 0, stream, w, npaths);
+    }
+    for (int d = 0; d < p.max_depth; ++d) {
+        const int in = d & 1;
+        {
+            Timed t(tm, WF_CONTROL);
+            hipLaunchKernelGGL(k_wf_bounce_begin, dim3(1), dim3(256), 0, stream, w.ctrl, in ^ 1,
+                               cfg.sort_by_material ? w.hist : nullptr, nb);
+        }
+        {
+            Timed t(tm, WF_EXTEND);
+            launch_extend<COUNT, EB>(g.extend, stream, s, w, in, ctr);
+        }
+        if (cfg.sort_by_material) {
+            Timed t(tm, WF_SORT);
+            hipLaunchKernelGGL(k_wf_hist, dim3(g.sort), dim3(256), 0, stream, s, w, in, nb);
+            hipLaunchKernelGGL(k_wf_hist_scan, dim3(1), dim3(256), 0, stream, w.hist, nb);
+            hipLaunchKernelGGL(k_wf_scatter, dim3(g.sort), dim3(256), 0, stream, s, w, in, nb);
+        }
+        {
+            Timed t(tm, WF_SHADE);
+            hipLaunchKernelGGL((k_wf_shade<SB>), dim3(g.shade), dim3(SB), 0, stream, s, w, in,
+                               cfg.sort_by_material ? 1 : 0);
+        }
+        {
+            Timed t(tm, WF_SHADOW);
+            launch_shadow<COUNT, EB, false>(g.extend, stream, s, w, C_SA, C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, ctr);
+            launch_shadow<COUNT, EB, false>(g.extend, stream, s, w, C_SE, C_CUR_SE, w.se_o, w.se_d, w.se_c, ctr);
+            if (s.num_lights > 0)
+                launch_shadow<COUNT, EB, true>(g.extend, stream, s, w, C_SP, C_CUR_SP, w.sp_o, w.sp_d, w.sp_c, ctr);
+        }
+    }
+    {
+        Timed t(tm, WF_CONTROL);
+        hipLaunchKernelGGL(k_wf_finalize, dim3((npix + 255) / 256), dim3(256), 0, stream, w, npix);
+        hipLaunchKernelGGL(k_wf_chunk_end, dim3(1), dim3(64), 0, stream, w.ctrl);
+    }
+}
+
+template <bool COUNT, int EB, int SB>
+Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
+    Grids g;
+    g.extend = grid_for(reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB>), EB, wf.num_cus, cfg.waves_per_simd);
+    g.shade = grid_for(reinterpret_cast<const void*>(&k_wf_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
+    g.sort = wf.num_cus * 4;
+    return g;
+}
+
+// Calls f.template operator()<EB, SB>() for the configured block sizes.
+template <class F>
+void with_blocks(const WavefrontConfig& cfg, F&& f) {
+    const int eb = cfg.extend_block, sb = cfg.shade_block;
+    if (eb == 64 && sb == 64) f.template operator()<64, 64>();
+    else if (eb == 64 && sb == 128) f.template operator()<64, 128>();
+    else if (eb == 64 && sb == 256) f.template operator()<64, 256>();
+    else if (eb == 128 && sb == 64) f.template operator()<128, 64>();
+    else if (eb == 128 && sb == 256) f.template operator()<128, 256>();
+    else if (eb == 256 && sb == 64) f.template operator()<256, 64>();
+    else if (eb == 256 && sb == 128) f.template operator()<256, 128>();
+    else if (eb == 256 && sb == 256) f.template operator()<256, 256>();
+    else f.template operator()<128, 128>();
+}
+
+template <bool COUNT>
+struct ChunkEnqueuer {
+    Wavefront& wf;
+    const DevScene& s;
+    const RenderParams& p;
+    const WavefrontConfig& cfg;
+    uint32_t npaths, npix;
+    DevCounters* ctr;
+    hipStream_t stream;
+    StageTimer* tm;
+    int repeat;
+    template <int EB, int SB>
+    void operator()() {
+        Grids g = grids_for<COUNT, EB, SB>(wf, cfg);
+        for (int i = 0; i < repeat; ++i) enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npaths, npix, ctr, stream, tm);
+    }
+};
+
+template <bool COUNT>
+struct GridQuery {
+    const Wavefront& wf;
+    const WavefrontConfig& cfg;
+    Grids out;
+    template <int EB, int SB>
+    void operator()() {
+        out = grids_for<COUNT, EB, SB>(wf, cfg);
+    }
+};
+
+template <bool COUNT>
+struct CaptureEnqueuer {
+    Wavefront& wf;
+    const DevScene& s;
+    const RenderParams& p;
+    const WavefrontConfig& cfg;
+    const Grids& g;
+    uint32_t npaths, npix;
+    DevCounters* ctr;
+    hipStream_t stream;
+    template <int EB, int SB>
+    void operator()() {
+        enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npaths, npix, ctr, stream, nullptr);
+    }
+};
+
+int chunk_spp_for(int spf, int want) {
+    int s = std::max(1, std::min(want, spf));
+    while (spf % s) --s;
+    return s;
+}
+
+}  // namespace
+
+hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_generation, const RenderParams& p,
+                            const WavefrontConfig& cfg, float4* film_new, float4* film_acc, DevCounters* ctr,
+                            bool count, hipStream_t stream, float* ms, std::string* err) {
+    const int S = chunk_spp_for(p.spf, cfg.chunk_spp);
+    const uint64_t npix = static_cast<uint64_t>(p.width) * p.height;
+    const uint64_t npaths64 = npix * S;
+    if (npaths64 > (1ull << 31)) {
+        if (err) *err = "wavefront chunk too large (pixels x chunk spp must stay below 2^31)";
+        return hipErrorInvalidValue;
+    }
+    const uint32_t npaths = static_cast<uint32_t>(npaths64);
+    if (wf->num_cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&wf->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            wf->num_cus <= 0)
+            wf->num_cus = 256;
+    }
+    if (!wf->ensure(npaths, static_cast<uint32_t>(npix), s.num_lights, s.num_materials)) {
+        if (err) *err = "wavefront: out of device memory for path state";
+        return hipErrorOutOfMemory;
+    }
+    wf->st.chunk_spp = static_cast<uint32_t>(S);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(wf->d_params, &p, sizeof(RenderParams), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return e;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, stream);
+    hipLaunchKernelGGL(k_wf_batch_begin, dim3(1), dim3(64), 0, stream, wf->st.ctrl);
+    const int nchunks = p.spf / S;
+    const uint32_t npix32 = static_cast<uint32_t>(npix);
+    const bool use_graph = cfg.use_graph && !cfg.time_stages;
+    uint64_t key[10] = {scene_generation, static_cast<uint64_t>(p.width), static_cast<uint64_t>(p.height),
+                        static_cast<uint64_t>(S), static_cast<uint64_t>(p.max_depth),
+                        static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
+                        static_cast<uint64_t>(cfg.sort_by_material * 100 + cfg.waves_per_simd),
+                        static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
+                        static_cast<uint64_t>(s.num_lights)};
+    StageTimer timer;
+    timer.stream = stream;
+    StageTimer* tm = cfg.time_stages ? &timer : nullptr;
+    if (use_graph) {
+        if (!wf->exec || std::memcmp(key, wf->key, sizeof(key)) != 0) {
+            wf->release_graph();
+            Grids g;
+            if (count) {
+                GridQuery<true> q{*wf, cfg, {}};
+                with_blocks(cfg, q);
+                g = q.out;
+            } else {
+                GridQuery<false> q{*wf, cfg, {}};
+                with_blocks(cfg, q);
+                g = q.out;
+            }
+            hipStream_t cap;
+            if ((e = hipStreamCreate(&cap)) != hipSuccess) return e;
+            e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                if (count) {
+                    CaptureEnqueuer<true> f{*wf, s, p, cfg, g, npaths, npix32, ctr, cap};
+                    with_blocks(cfg, f);
+                } else {
+                    CaptureEnqueuer<false> f{*wf, s, p, cfg, g, npaths, npix32, ctr, cap};
+                    with_blocks(cfg, f);
+                }
+                e = hipStreamEndCapture(cap, &wf->graph);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&wf->exec, wf->graph, nullptr, nullptr, 0);
+            (void)hipStreamDestroy(cap);
+            if (e != hipSuccess) {
+                wf->release_graph();
+                if (err) *err = std::string("wavefront graph capture: ") + hipGetErrorString(e);
+                return e;
+            }
+            std::memcpy(wf->key, key, sizeof(key));
+        }
+        for (int c = 0; c < nchunks; ++c)
+            if ((e = hipGraphLaunch(wf->exec, stream)) != hipSuccess) return e;
+    } else if (count) {
+        ChunkEnqueuer<true> f{*wf, s, p, cfg, npaths, npix32, ctr, stream, tm, nchunks};
+        with_blocks(cfg, f);
+    } else {
+        ChunkEnqueuer<false> f{*wf, s, p, cfg, npaths, npix32, ctr, stream, tm, nchunks};
+        with_blocks(cfg, f);
+    }
+    {
+        Timed t(tm, WF_CONTROL);
+        hipLaunchKernelGGL(k_wf_film, dim3((npix32 + 255) / 256), dim3(256), 0, stream, wf->st, npix32, film_new,
+                           film_acc, ctr, count ? 1u : 0u);
+    }
+    (void)hipEventRecord(e1, stream);
+    e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    if (tm) {
+        timer.collect(wf->stage_ms, wf->stage_launches);
+    } else {
+        for (int k = 0; k < WF_STAGES; ++k) {
+            wf->stage_ms[k] = 0.0f;
+            wf->stage_launches[k] = 0;
+        }
+    }
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches) {
+    for (int k = 0; k < WF_STAGES; ++k) {
+        if (ms) ms[k] = w ? w->stage_ms[k] : 0.0f;
+        if (launches) launches[k] = w ? w->stage_launches[k] : 0u;
+    }
+}
+
+}  // namespace ptd
