@@ -141,6 +141,8 @@ def main(argv=None) -> int:
     ap.add_argument("--integrator", default="both", choices=["both", "path", "wavefront"])
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the wavefront autotuner (use the default config)")
+    ap.add_argument("--tune-trials", type=int, default=16)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (HBM calibration) and print its GB/s")
     a = ap.parse_args(argv)
@@ -169,6 +171,11 @@ def main(argv=None) -> int:
     sm.uploadToDevice()
     runs = {}
     order = ["wavefront", "path"] if a.integrator == "both" else [a.integrator]
+    tuned = None
+    if "wavefront" in order and not a.no_tune:
+        t0 = time.perf_counter()
+        tuned, ms_spp = sm.tuneWavefront(a.tune_trials)
+        log(f"autotuned wavefront config in {time.perf_counter() - t0:.1f} s: {tuned} ({ms_spp:.2f} ms per spp)")
     for name in order:
         integ = pt.PathIntegrator() if name == "path" else pt.WavefrontIntegrator()
         sm.uploadToDevice()  # frame_count 0: both integrators render the same frames
@@ -233,6 +240,7 @@ def main(argv=None) -> int:
     }
     if head == "wavefront":
         out["config"]["wavefront"] = sm.wavefrontConfig()
+        out["config"]["wavefront_autotuned"] = tuned is not None
     if "path" in runs and head != "path":
         P = runs["path"]
         pc = P["counters"]

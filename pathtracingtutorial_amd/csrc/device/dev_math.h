@@ -43,6 +43,21 @@ __device__ __forceinline__ F3 normalize(F3 a) {
     float l = length(a);
     return l > 0.0f ? a / l : F3{0.0f, 0.0f, 0.0f};
 }
+// Round-to-nearest forms without FMA contraction: every product and sum
+// rounds separately, as the reference's host arithmetic does.  Used where
+// results are compared bit for bit (triangle test, ray normalization).
+__device__ __forceinline__ float dot_rn(F3 a, F3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ F3 cross_rn(F3 a, F3 b) {
+#pragma clang fp contract(off)
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ F3 normalize_rn(F3 a) {
+    float l = sqrtf(dot_rn(a, a));
+    return l > 0.0f ? a / l : F3{0.0f, 0.0f, 0.0f};
+}
 __device__ __forceinline__ bool is_zero(F3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 __device__ __forceinline__ F3 reflect(F3 i, F3 n) { return i - 2.0f * dot(i, n) * n; }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }

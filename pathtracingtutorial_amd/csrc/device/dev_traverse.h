@@ -62,17 +62,17 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
     float4 a = tri[0], b = tri[1], c = tri[2];
     word = __float_as_uint(a.w);
     F3 e1 = f3(b.x, b.y, b.z), e2 = f3(c.x, c.y, c.z);
-    F3 pvec = cross(d, e2);
-    float det = dot(e1, pvec);
+    F3 pvec = cross_rn(d, e2);
+    float det = dot_rn(e1, pvec);
     if (fabsf(det) < kEps) return -1.0f;
     float inv_det = 1.0f / det;
     F3 tvec = o - f3(a.x, a.y, a.z);
-    u = dot(tvec, pvec) * inv_det;
+    u = dot_rn(tvec, pvec) * inv_det;
     if (u < 0.0f || u > 1.0f) return -1.0f;
-    F3 qvec = cross(tvec, e1);
-    v = dot(d, qvec) * inv_det;
+    F3 qvec = cross_rn(tvec, e1);
+    v = dot_rn(d, qvec) * inv_det;
     if (v < 0.0f || u + v > 1.0f) return -1.0f;
-    return dot(e2, qvec) * inv_det;
+    return dot_rn(e2, qvec) * inv_det;
 }
 
 // Closest hit.  stk points at this thread's first LDS stack slot.

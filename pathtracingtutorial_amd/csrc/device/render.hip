@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_trace_rays(DevScene s, const float* __r
     if (i >= n) return;
     const float* r = rays8 + 8 * static_cast<size_t>(i);
     F3 o = f3(r[0], r[1], r[2]);
-    F3 d = normalize(f3(r[3], r[4], r[5]));
+    F3 d = normalize_rn(f3(r[3], r[4], r[5]));  // Ray's constructor (core/ray.h:13)
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     if (any) {

@@ -33,11 +33,24 @@ namespace ptd {
 
 namespace {
 
-constexpr int kXcd = 8;
+// Queues are cut into kSeg segments, one per XCD (blocks are dispatched
+// round-robin over the XCDs, so block b runs on XCD b % 8).  Segment k of a
+// chunk's paths is the contiguous range [base(k), base(k+1)) of path ids
+// (a band of image rows), and everything a path spawns (next ray, shadow
+// rays) is appended to the same segment's region of the output queue: across
+// bounces each XCD keeps working on its own band, in roughly the original
+// order, and its L2 sees coherent rays.
+constexpr int kSeg = 8;
+constexpr int kSortParts = 32;  // blocks per segment of the material sort
+constexpr uint32_t kMaxSortBins = 8192;
 // control words
-constexpr int C_Q0 = 0, C_Q1 = 1, C_SA = 2, C_SE = 3, C_SP = 4, C_CHUNK = 5;
-constexpr int C_CUR_EXT = 8, C_CUR_SHADE = 16, C_CUR_SA = 24, C_CUR_SE = 32, C_CUR_SP = 40, C_CUR_SORT = 48;
-constexpr int C_WORDS = 64;
+constexpr int C_QCNT = 0;   // [2][8] ray queue segment counts
+constexpr int C_SACNT = 16; // [8] area shadow queue segment counts
+constexpr int C_SECNT = 24; // [8] environment shadow queue segment counts
+constexpr int C_SP = 32;    // punctual shadow queue count (one flat queue)
+constexpr int C_CHUNK = 33; // chunk index within the batch
+constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72;
+constexpr int C_WORDS = 80;
 
 struct WfState {
     float4* o;    // xyz origin
@@ -51,15 +64,20 @@ struct WfState {
     float4 *sa_o, *sa_d, *sa_c;  // area shadow queue: origin + tmax, direction + path bits, contribution
     float4 *se_o, *se_d, *se_c;  // environment shadow queue
     float4 *sp_o, *sp_d, *sp_c;  // punctual shadow queue
-    uint32_t* hist;              // material histogram / offsets (num_materials + 1)
+    uint32_t* blockhist;         // material sort: [segment][part][bin] counts, then offsets
     float4* sum;                 // per pixel sum over the batch
     uint32_t* ctrl;
     const RenderParams* params;
     uint32_t capacity, sp_capacity;
     uint32_t chunk_spp;
+    uint32_t npaths;  // paths per chunk (pixels x chunk_spp)
 };
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
+
+__device__ __forceinline__ uint32_t seg_base(uint32_t npaths, int k) {
+    return static_cast<uint32_t>((static_cast<unsigned long long>(npaths) * static_cast<unsigned>(k)) / kSeg);
+}
 
 // Wave-aggregated append: every lane of the wave must call it.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -74,12 +92,14 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + static_cast<uint32_t>(__popcll(below));
 }
 
-// Grabs the next 64 queue items for this wave: segment seg of 8 (one per
-// XCD) first, then the others.  Returns false when the queue is exhausted.
-__device__ __forceinline__ bool grab(uint32_t* cursors, uint32_t n, int& seg, int& tried, uint32_t& i, uint32_t& end) {
-    while (tried < kXcd) {
-        uint32_t s0 = static_cast<uint32_t>((static_cast<unsigned long long>(n) * seg) / kXcd);
-        uint32_t s1 = static_cast<uint32_t>((static_cast<unsigned long long>(n) * (seg + 1)) / kXcd);
+// Grabs the next 64 items of a segmented queue for this wave: its own
+// segment first, then the others.  Segment k holds [base(k), base(k) +
+// counts[k]).  Returns false when every segment is exhausted.
+__device__ __forceinline__ bool grab_seg(uint32_t* cursors, const uint32_t* counts, uint32_t npaths, int& seg,
+                                         int& tried, uint32_t& i, uint32_t& end) {
+    while (tried < kSeg) {
+        uint32_t s0 = seg_base(npaths, seg);
+        uint32_t s1 = s0 + counts[seg];
         uint32_t off = 0;
         if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
         off = __shfl(off, 0, 64);
@@ -88,7 +108,26 @@ __device__ __forceinline__ bool grab(uint32_t* cursors, uint32_t n, int& seg, in
             end = s1;
             return true;
         }
-        seg = (seg + 1) % kXcd;
+        seg = (seg + 1) % kSeg;
+        ++tried;
+    }
+    return false;
+}
+
+// The same over a flat queue of n items split in kSeg equal ranges.
+__device__ __forceinline__ bool grab_flat(uint32_t* cursors, uint32_t n, int& seg, int& tried, uint32_t& i,
+                                          uint32_t& end) {
+    while (tried < kSeg) {
+        uint32_t s0 = seg_base(n, seg), s1 = seg_base(n, seg + 1);
+        uint32_t off = 0;
+        if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
+        off = __shfl(off, 0, 64);
+        if (s0 + off < s1) {
+            i = s0 + off + static_cast<uint32_t>(lane_id());
+            end = s1;
+            return true;
+        }
+        seg = (seg + 1) % kSeg;
         ++tried;
     }
     return false;
@@ -156,8 +195,12 @@ __global__ void k_wf_batch_begin(uint32_t* ctrl) {
     if (threadIdx.x == 0) ctrl[C_CHUNK] = 0;
 }
 
-__global__ void k_wf_chunk_begin(WfState w, uint32_t npaths) {
-    if (threadIdx.x == 0) w.ctrl[C_Q0] = w.params->max_depth > 0 ? npaths : 0u;
+// Ray queue 0 of a chunk: every path, segment k = [base(k), base(k+1)).
+__global__ void k_wf_chunk_begin(WfState w) {
+    const int k = static_cast<int>(threadIdx.x);
+    if (k < kSeg)
+        w.ctrl[C_QCNT + k] =
+            w.params->max_depth > 0 ? seg_base(w.npaths, k + 1) - seg_base(w.npaths, k) : 0u;
 }
 
 __global__ void k_wf_chunk_end(uint32_t* ctrl) {
@@ -165,9 +208,9 @@ __global__ void k_wf_chunk_end(uint32_t* ctrl) {
 }
 
 // Camera rays of chunk ctrl[C_CHUNK]: path p = (pixel p / S, sample p % S).
-__global__ void k_wf_generate(WfState w, uint32_t npaths) {
+__global__ void k_wf_generate(WfState w) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npaths) return;
+    if (p >= w.npaths) return;
     const RenderParams& rp = *w.params;
     const uint32_t S = w.chunk_spp;
     uint32_t pixel = p / S, j = p % S;
@@ -185,30 +228,29 @@ __global__ void k_wf_generate(WfState w, uint32_t npaths) {
     w.q[0][p] = p;
 }
 
-__global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot, uint32_t* hist, uint32_t nbins) {
+// Clears the output queue's and the shadow queues' counts and all cursors.
+__global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
     uint32_t t = threadIdx.x;
-    if (t == 0) {
-        ctrl[out_slot] = 0;
-        ctrl[C_SA] = 0;
-        ctrl[C_SE] = 0;
-        ctrl[C_SP] = 0;
+    if (t < kSeg) {
+        ctrl[C_QCNT + out_slot * kSeg + t] = 0;
+        ctrl[C_SACNT + t] = 0;
+        ctrl[C_SECNT + t] = 0;
     }
+    if (t == 0) ctrl[C_SP] = 0;
     for (uint32_t k = C_CUR_EXT + t; k < C_WORDS; k += blockDim.x) ctrl[k] = 0;
-    if (hist)
-        for (uint32_t k = t; k < nbins; k += blockDim.x) hist[k] = 0;
 }
 
 template <bool COUNT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int in_slot, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
-    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = w.q[in_slot];
-    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab(w.ctrl + C_CUR_EXT, n, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + C_CUR_EXT, counts, w.npaths, seg, tried, i, end)) {
         if (i < end) {
             uint32_t p = q[i];
             float4 o = w.o[p], d = w.d[p];
@@ -220,52 +262,81 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int 
     flush<COUNT, false>(ctr, rays, cnt.nodes, cnt.tris, ovf);
 }
 
-// Material keys of the queue (misses go to the last bin), then a counting sort.
-__global__ void k_wf_hist(DevScene s, WfState w, int in_slot, uint32_t nbins) {
-    const uint32_t n = w.ctrl[in_slot];
-    const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t tri = __float_as_uint(w.hit[q[i]].w);
-        uint32_t key = tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
-        atomicAdd(&w.hist[key], 1u);
-    }
+// ---- material sort: a counting sort within each segment -------------------
+// Block j sorts part j / kSeg of segment j % kSeg: per-block histograms in
+// LDS, one scan per segment over (bin, part), then a scatter that ranks the
+// block's items with LDS atomics.  Misses go to the last bin.
+
+__device__ __forceinline__ uint32_t material_key(const DevScene& s, const WfState& w, uint32_t p, uint32_t nbins) {
+    uint32_t tri = __float_as_uint(w.hit[p].w);
+    return tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
 }
 
-__global__ void k_wf_hist_scan(uint32_t* hist, uint32_t nbins) {
-    // single block: exclusive scan in place (nbins <= 4096)
-    __shared__ uint32_t part[256];
-    uint32_t t = threadIdx.x;
-    uint32_t per = (nbins + 255) / 256;
-    uint32_t b0 = t * per, b1 = min(b0 + per, nbins);
+__device__ __forceinline__ void sort_part_range(const WfState& w, const uint32_t* counts, int seg, int part,
+                                                uint32_t& b0, uint32_t& b1) {
+    const uint32_t base = seg_base(w.npaths, seg), n = counts[seg];
+    b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(n) * part) / kSortParts);
+    b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(n) * (part + 1)) / kSortParts);
+}
+
+__global__ __launch_bounds__(256) void k_wf_sort_count(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+    extern __shared__ uint32_t hist[];
+    const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    uint32_t b0, b1;
+    sort_part_range(w, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
+    const uint32_t* q = w.q[in_slot];
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[material_key(s, w, q[i], nbins)], 1u);
+    __syncthreads();
+    uint32_t* out = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
+}
+
+// One block per segment: offsets in (bin, part) order, starting at the
+// segment's base, written over the counts.
+__global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins) {
+    __shared__ uint32_t part_sum[256];
+    const int seg = static_cast<int>(blockIdx.x);
+    uint32_t* h = w.blockhist + static_cast<size_t>(seg) * kSortParts * nbins;
+    const uint32_t E = nbins * kSortParts;  // entry e = bin * parts + part
+    const uint32_t t = threadIdx.x;
+    const uint32_t e0 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * t) / 256);
+    const uint32_t e1 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * (t + 1)) / 256);
+    auto at = [&](uint32_t e) -> uint32_t& { return h[(e % kSortParts) * nbins + e / kSortParts]; };
     uint32_t sum = 0;
-    for (uint32_t k = b0; k < b1; ++k) sum += hist[k];
-    part[t] = sum;
+    for (uint32_t e = e0; e < e1; ++e) sum += at(e);
+    part_sum[t] = sum;
     __syncthreads();
     if (t == 0) {
-        uint32_t acc = 0;
+        uint32_t acc = seg_base(w.npaths, seg);
         for (int k = 0; k < 256; ++k) {
-            uint32_t v = part[k];
-            part[k] = acc;
+            uint32_t v = part_sum[k];
+            part_sum[k] = acc;
             acc += v;
         }
     }
     __syncthreads();
-    uint32_t acc = part[t];
-    for (uint32_t k = b0; k < b1; ++k) {
-        uint32_t v = hist[k];
-        hist[k] = acc;
+    uint32_t acc = part_sum[t];
+    for (uint32_t e = e0; e < e1; ++e) {
+        uint32_t v = at(e);
+        at(e) = acc;
         acc += v;
     }
 }
 
-__global__ void k_wf_scatter(DevScene s, WfState w, int in_slot, uint32_t nbins) {
-    const uint32_t n = w.ctrl[in_slot];
+__global__ __launch_bounds__(256) void k_wf_sort_scatter(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+    extern __shared__ uint32_t off[];
+    const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
+    const uint32_t* src = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
+    __syncthreads();
+    uint32_t b0, b1;
+    sort_part_range(w, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
     const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
         uint32_t p = q[i];
-        uint32_t tri = __float_as_uint(w.hit[p].w);
-        uint32_t key = tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
-        w.sorted[atomicAdd(&w.hist[key], 1u)] = p;
+        w.sorted[atomicAdd(&off[material_key(s, w, p, nbins)], 1u)] = p;
     }
 }
 
@@ -311,13 +382,14 @@ struct DeferNee {
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_wf_shade(DevScene s, WfState w, int in_slot, int use_sorted) {
     const RenderParams& rp = *w.params;
-    const uint32_t n = w.ctrl[in_slot];
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
-    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
+    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
-    while (grab(w.ctrl + C_CUR_SHADE, n, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + C_CUR_SHADE, counts, w.npaths, seg, tried, i, end)) {
         const bool active = i < end;
         uint32_t p = 0;
         PathState st;
@@ -346,10 +418,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DevScene s, WfState w, int i
             cont = cont && st.depth < rp.max_depth;
             store_path(w, p, st, r);
         }
+        // everything this wave spawns goes to the segment its items came from
+        const uint32_t base = seg_base(w.npaths, seg);
         const bool pa = active && nee.va, pe = active && nee.ve;
-        uint32_t ka = wave_append(&w.ctrl[C_SA], pa);
-        uint32_t ke = wave_append(&w.ctrl[C_SE], pe);
-        uint32_t ko = wave_append(&w.ctrl[out_slot], cont);
+        uint32_t ka = base + wave_append(&w.ctrl[C_SACNT + seg], pa);
+        uint32_t ke = base + wave_append(&w.ctrl[C_SECNT + seg], pe);
+        uint32_t ko = base + wave_append(&out_counts[seg], cont);
         if (pa) {
             w.sa_o[ka] = make_float4(nee.sa.o.x, nee.sa.o.y, nee.sa.o.z, nee.sa.tmax);
             w.sa_d[ka] = make_float4(nee.sa.d.x, nee.sa.d.y, nee.sa.d.z, __uint_as_float(p));
@@ -364,35 +438,58 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(DevScene s, WfState w, int i
     }
 }
 
-// Any-hit over a shadow queue; unoccluded entries add their contribution.
-template <bool COUNT, int BLOCK, bool ATOMIC>
+__device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, float4 c, bool atomic) {
+    if (atomic) {
+        atomicAdd(&w.lo[p].x, c.x);
+        atomicAdd(&w.lo[p].y, c.y);
+        atomicAdd(&w.lo[p].z, c.z);
+    } else {
+        float4 l = w.lo[p];
+        w.lo[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+    }
+}
+
+// Any-hit over a segmented shadow queue (area or environment): each path has
+// at most one entry, so the contribution is added without atomics.
+template <bool COUNT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(DevScene s, WfState w, int count_slot, int cur_slot,
                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                      const float4* __restrict__ qc, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
-    uint32_t n = w.ctrl[count_slot];
-    if (ATOMIC) n = min(n, w.sp_capacity);
-    int seg = static_cast<int>(blockIdx.x % kXcd), tried = 0;
+    const uint32_t* counts = w.ctrl + count_slot;
+    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab(w.ctrl + cur_slot, n, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + cur_slot, counts, w.npaths, seg, tried, i, end)) {
         if (i < end) {
             float4 o = qo[i], d = qd[i];
             rays++;
-            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf)) {
-                uint32_t p = __float_as_uint(d.w);
-                float4 c = qc[i];
-                if (ATOMIC) {
-                    atomicAdd(&w.lo[p].x, c.x);
-                    atomicAdd(&w.lo[p].y, c.y);
-                    atomicAdd(&w.lo[p].z, c.z);
-                } else {
-                    float4 l = w.lo[p];
-                    w.lo[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
-                }
-            }
+            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
+                add_unoccluded(w, __float_as_uint(d.w), qc[i], false);
+        }
+    }
+    flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
+}
+
+// Any-hit over the flat punctual-light queue (several entries per path:
+// contributions are added with float atomics).
+template <bool COUNT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(DevScene s, WfState w, DevCounters* ctr) {
+    __shared__ uint32_t stack[kStackSize * BLOCK];
+    const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
+    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
+    uint32_t i, end;
+    TraceCounters cnt{0, 0};
+    uint32_t ovf = 0;
+    unsigned long long rays = 0;
+    while (grab_flat(w.ctrl + C_CUR_SP, n, seg, tried, i, end)) {
+        if (i < end) {
+            float4 o = w.sp_o[i], d = w.sp_d[i];
+            rays++;
+            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
+                add_unoccluded(w, __float_as_uint(d.w), w.sp_c[i], true);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -517,7 +614,7 @@ public:
     void release() {
         release_graph();
         void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
-                      st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.hist, st.sum, st.ctrl,
+                      st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
                       d_params};
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -553,9 +650,10 @@ public:
             pixels = npix;
             release_graph();
         }
-        if (nmat + 2 > nbins) {
-            ok = ok && dalloc(st.hist, nmat + 2);
-            nbins = nmat + 2;
+        const uint32_t nb = std::min<uint32_t>(nmat + 1, kMaxSortBins);  // materials + the miss bin
+        if (nb > nbins) {
+            ok = ok && dalloc(st.blockhist, static_cast<size_t>(kSeg) * kSortParts * nb);
+            nbins = nb;
             release_graph();
         }
         if (!st.ctrl) ok = ok && dalloc(st.ctrl, C_WORDS);
@@ -587,64 +685,57 @@ int grid_for(const void* kernel, int block, int num_cus, int waves_per_simd) {
     return num_cus * per_cu;
 }
 
-template <bool COUNT, int B>
-void launch_extend(int grid, hipStream_t s, const DevScene& sc, const WfState& w, int slot, DevCounters* c) {
-    hipLaunchKernelGGL((k_wf_extend<COUNT, B>), dim3(grid), dim3(B), 0, s, sc, w, slot, c);
-}
-template <bool COUNT, int B, bool ATOMIC>
-void launch_shadow(int grid, hipStream_t s, const DevScene& sc, const WfState& w, int cslot, int cur,
-                   const float4* o, const float4* d, const float4* c, DevCounters* ctr) {
-    hipLaunchKernelGGL((k_wf_shadow<COUNT, B, ATOMIC>), dim3(grid), dim3(B), 0, s, sc, w, cslot, cur, o, d, c, ctr);
-}
-
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
-    int extend, shade, sort;
+    int extend, shade;
 };
 
 template <bool COUNT, int EB, int SB>
 void enqueue_chunk(Wavefront& wf, const DevScene& s, const RenderParams& p, const WavefrontConfig& cfg,
-                   const Grids& g, uint32_t npaths, uint32_t npix, DevCounters* ctr, hipStream_t stream,
-                   StageTimer* tm) {
+                   const Grids& g, uint32_t npix, DevCounters* ctr, hipStream_t stream, StageTimer* tm) {
     const WfState& w = wf.st;
     const uint32_t nb = s.num_materials + 1;  // materials + miss bin
+    const bool sort = cfg.sort_by_material && nb <= kMaxSortBins;
+    const size_t sort_lds = sizeof(uint32_t) * nb;
     {
         Timed t(tm, WF_CONTROL);
-        hipLaunchKernelGGL(k_wf_chunk_begin, dim3(1), dim3(64), 0, stream, w, npaths);
+        hipLaunchKernelGGL(k_wf_chunk_begin, dim3(1), dim3(64), 0, stream, w);
     }
     {
         Timed t(tm, WF_GENERATE);
-        hipLaunchKernelGGL(k_wf_generate, dim3((npaths + 255) / 256), dim3(256), 0, stream, w, npaths);
+        hipLaunchKernelGGL(k_wf_generate, dim3((w.npaths + 255) / 256), dim3(256), 0, stream, w);
     }
     for (int d = 0; d < p.max_depth; ++d) {
         const int in = d & 1;
         {
             Timed t(tm, WF_CONTROL);
-            hipLaunchKernelGGL(k_wf_bounce_begin, dim3(1), dim3(256), 0, stream, w.ctrl, in ^ 1,
-                               cfg.sort_by_material ? w.hist : nullptr, nb);
+            hipLaunchKernelGGL(k_wf_bounce_begin, dim3(1), dim3(128), 0, stream, w.ctrl, in ^ 1);
         }
         {
             Timed t(tm, WF_EXTEND);
-            launch_extend<COUNT, EB>(g.extend, stream, s, w, in, ctr);
+            hipLaunchKernelGGL((k_wf_extend<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, in, ctr);
         }
-        if (cfg.sort_by_material) {
+        if (sort) {
             Timed t(tm, WF_SORT);
-            hipLaunchKernelGGL(k_wf_hist, dim3(g.sort), dim3(256), 0, stream, s, w, in, nb);
-            hipLaunchKernelGGL(k_wf_hist_scan, dim3(1), dim3(256), 0, stream, w.hist, nb);
-            hipLaunchKernelGGL(k_wf_scatter, dim3(g.sort), dim3(256), 0, stream, s, w, in, nb);
+            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, s, w, in, nb);
+            hipLaunchKernelGGL(k_wf_sort_scan, dim3(kSeg), dim3(256), 0, stream, w, nb);
+            hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, s, w, in,
+                               nb);
         }
         {
             Timed t(tm, WF_SHADE);
-            hipLaunchKernelGGL((k_wf_shade<SB>), dim3(g.shade), dim3(SB), 0, stream, s, w, in,
-                               cfg.sort_by_material ? 1 : 0);
+            hipLaunchKernelGGL((k_wf_shade<SB>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, sort ? 1 : 0);
         }
         {
             Timed t(tm, WF_SHADOW);
-            launch_shadow<COUNT, EB, false>(g.extend, stream, s, w, C_SA, C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, ctr);
-            launch_shadow<COUNT, EB, false>(g.extend, stream, s, w, C_SE, C_CUR_SE, w.se_o, w.se_d, w.se_c, ctr);
+            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, C_SACNT,
+                               C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, ctr);
+            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, C_SECNT,
+                               C_CUR_SE, w.se_o, w.se_d, w.se_c, ctr);
             if (s.num_lights > 0)
-                launch_shadow<COUNT, EB, true>(g.extend, stream, s, w, C_SP, C_CUR_SP, w.sp_o, w.sp_d, w.sp_c, ctr);
+                hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w,
+                                   ctr);
         }
     }
     {
@@ -659,7 +750,6 @@ Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
     Grids g;
     g.extend = grid_for(reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB>), EB, wf.num_cus, cfg.waves_per_simd);
     g.shade = grid_for(reinterpret_cast<const void*>(&k_wf_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
-    g.sort = wf.num_cus * 4;
     return g;
 }
 
@@ -684,7 +774,7 @@ struct ChunkEnqueuer {
     const DevScene& s;
     const RenderParams& p;
     const WavefrontConfig& cfg;
-    uint32_t npaths, npix;
+    uint32_t npix;
     DevCounters* ctr;
     hipStream_t stream;
     StageTimer* tm;
@@ -692,7 +782,7 @@ struct ChunkEnqueuer {
     template <int EB, int SB>
     void operator()() {
         Grids g = grids_for<COUNT, EB, SB>(wf, cfg);
-        for (int i = 0; i < repeat; ++i) enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npaths, npix, ctr, stream, tm);
+        for (int i = 0; i < repeat; ++i) enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npix, ctr, stream, tm);
     }
 };
 
@@ -714,12 +804,12 @@ struct CaptureEnqueuer {
     const RenderParams& p;
     const WavefrontConfig& cfg;
     const Grids& g;
-    uint32_t npaths, npix;
+    uint32_t npix;
     DevCounters* ctr;
     hipStream_t stream;
     template <int EB, int SB>
     void operator()() {
-        enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npaths, npix, ctr, stream, nullptr);
+        enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npix, ctr, stream, nullptr);
     }
 };
 
@@ -754,6 +844,7 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
         return hipErrorOutOfMemory;
     }
     wf->st.chunk_spp = static_cast<uint32_t>(S);
+    wf->st.npaths = npaths;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_params, &p, sizeof(RenderParams), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return e;
@@ -792,10 +883,10 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
             e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
                 if (count) {
-                    CaptureEnqueuer<true> f{*wf, s, p, cfg, g, npaths, npix32, ctr, cap};
+                    CaptureEnqueuer<true> f{*wf, s, p, cfg, g, npix32, ctr, cap};
                     with_blocks(cfg, f);
                 } else {
-                    CaptureEnqueuer<false> f{*wf, s, p, cfg, g, npaths, npix32, ctr, cap};
+                    CaptureEnqueuer<false> f{*wf, s, p, cfg, g, npix32, ctr, cap};
                     with_blocks(cfg, f);
                 }
                 e = hipStreamEndCapture(cap, &wf->graph);
@@ -812,10 +903,10 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
         for (int c = 0; c < nchunks; ++c)
             if ((e = hipGraphLaunch(wf->exec, stream)) != hipSuccess) return e;
     } else if (count) {
-        ChunkEnqueuer<true> f{*wf, s, p, cfg, npaths, npix32, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<true> f{*wf, s, p, cfg, npix32, ctr, stream, tm, nchunks};
         with_blocks(cfg, f);
     } else {
-        ChunkEnqueuer<false> f{*wf, s, p, cfg, npaths, npix32, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<false> f{*wf, s, p, cfg, npix32, ctr, stream, tm, nchunks};
         with_blocks(cfg, f);
     }
     {
