@@ -143,6 +143,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the wavefront autotuner (use the default config)")
     ap.add_argument("--tune-trials", type=int, default=16)
+    ap.add_argument("--wavefront-config", default="",
+                    help='JSON object of wavefront config fields to use instead of autotuning, '
+                         'e.g. \'{"chunk_spp": 32, "extend_block": 64}\'')
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (HBM calibration) and print its GB/s")
     a = ap.parse_args(argv)
@@ -172,7 +175,9 @@ def main(argv=None) -> int:
     runs = {}
     order = ["wavefront", "path"] if a.integrator == "both" else [a.integrator]
     tuned = None
-    if "wavefront" in order and not a.no_tune:
+    if a.wavefront_config:
+        sm.setWavefrontConfig(**json.loads(a.wavefront_config))
+    elif "wavefront" in order and not a.no_tune:
         t0 = time.perf_counter()
         tuned, ms_spp = sm.tuneWavefront(a.tune_trials)
         log(f"autotuned wavefront config in {time.perf_counter() - t0:.1f} s: {tuned} ({ms_spp:.2f} ms per spp)")

@@ -356,8 +356,19 @@ struct PathState {
     bool last_delta;
 };
 
+// What a bounce knows about its shading point, handed to the policy hooks.
+struct ShadeCtx {
+    const pt_material* mat;
+    const MatEval<float>* me;
+    F3 p, wo, n;   // point, outgoing direction, shading normal
+    float tu, tv;  // texture coordinates after the uv transform
+    bool entering;
+};
+
 // Visibility policy of the plain megakernel: shadow rays are traced inline
 // and the contributions added at once (direct_lighting.cuh:220-249).
+// A policy also receives hooks at the points where radiance or throughput
+// changes (the adjoint uses them; here they are empty).
 template <bool COUNT>
 struct InlineNee {
     TraceCtx& c;
@@ -365,14 +376,17 @@ struct InlineNee {
     // The area term is added before the environment term, one at a time,
     // so the wavefront schedule (one shadow queue per term) sums alike.
     __device__ __forceinline__ void direct(const DevScene& s, PathState& st, bool va, const ShadowRay& sa, F3 ca,
-                                           bool ve, const ShadowRay& se, F3 ce) {
+                                           bool ve, const ShadowRay& se, F3 ce, const LightSample&,
+                                           const LightSample&, const ShadeCtx&) {
         if (va && !occluded<COUNT>(s, sa, c)) st.lo += ca;
         if (ve && !occluded<COUNT>(s, se, c)) st.lo += ce;
     }
-    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, F3 p, const MatEval<float>& m,
-                                                    F3 wo, F3 n) {
-        st.lo += st.tp * punctual<COUNT>(s, p, m, wo, n, c);
+    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, const ShadeCtx& x) {
+        st.lo += st.tp * punctual<COUNT>(s, x.p, *x.me, x.wo, x.n, c);
     }
+    __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState&, float) {}
+    __device__ __forceinline__ void on_bsdf_sample(const DevScene&, const PathState&, const BsdfSample&,
+                                                   const ShadeCtx&) {}
 };
 
 // One bounce of pathTrace (path.cu:69-199) after the closest-hit query of the
@@ -388,6 +402,7 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
         if (!is_zero(env)) {
             float mis = 1.0f;
             if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf(s, st.d));
+            nee.on_env_miss(s, st, mis);
             st.lo += st.tp * env * mis;
         }
         return false;
@@ -432,6 +447,15 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
     }
     bool delta_surface =
         (me.metallic >= 1.0f && me.roughness < 0.01f) || (me.spec_trans >= 1.0f && me.roughness < 0.01f);
+    ShadeCtx x;
+    x.mat = &mat;
+    x.me = &me;
+    x.p = sf.p;
+    x.wo = wo;
+    x.n = n;
+    x.tu = tu;
+    x.tv = tv;
+    x.entering = entering;
     if (!delta_surface) {  // :161-166
         const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
         const bool has_env =
@@ -439,25 +463,28 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
         ShadowRay sa, se;
         bool va = false, ve = false;
         F3 ca = f3(0.0f), ce = f3(0.0f);
+        LightSample la, le;
+        la.valid = le.valid = false;
         if (has_area) {
-            LightSample ls = sample_area_nv(s, sf.p, n, rng, sa);
-            if (ls.valid) {
-                ca = st.tp * eval_direct(ls, me, wo, n);
+            la = sample_area_nv(s, sf.p, n, rng, sa);
+            if (la.valid) {
+                ca = st.tp * eval_direct(la, me, wo, n);
                 va = !is_zero(ca);
             }
         }
         if (has_env) {
-            LightSample ls = sample_env_nv(s, sf.p, n, rng, se);
-            if (ls.valid) {
-                ce = st.tp * eval_direct(ls, me, wo, n);
+            le = sample_env_nv(s, sf.p, n, rng, se);
+            if (le.valid) {
+                ce = st.tp * eval_direct(le, me, wo, n);
                 ve = !is_zero(ce);
             }
         }
-        nee.direct(s, st, va, sa, ca, ve, se, ce);
-        if (s.num_lights > 0 && s.lights != nullptr) nee.punctual_lights(s, st, sf.p, me, wo, n);
+        nee.direct(s, st, va, sa, ca, ve, se, ce, la, le, x);
+        if (s.num_lights > 0 && s.lights != nullptr) nee.punctual_lights(s, st, x);
     }
     BsdfSample bs = sample_bsdf(me, wo, n, rng, entering);
     if (!bs.valid) return false;
+    nee.on_bsdf_sample(s, st, bs, x);
     st.tp *= bs.weight;
     bool transmitted = dot(bs.wi, gn) < 0.0f;
     F3 off = transmitted ? -gn : gn;

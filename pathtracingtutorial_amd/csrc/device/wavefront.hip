@@ -352,7 +352,8 @@ struct DeferNee {
     uint32_t sp_capacity;
     float4 *sp_o, *sp_d, *sp_c;
     __device__ __forceinline__ void direct(const DevScene&, PathState&, bool a, const ShadowRay& ra, F3 xa, bool e,
-                                           const ShadowRay& re, F3 xe) {
+                                           const ShadowRay& re, F3 xe, const LightSample&, const LightSample&,
+                                           const ShadeCtx&) {
         va = a;
         sa = ra;
         ca = xa;
@@ -360,8 +361,12 @@ struct DeferNee {
         se = re;
         ce = xe;
     }
-    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, F3 p, const MatEval<float>& m,
-                                                    F3 wo, F3 n) {
+    __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState&, float) {}
+    __device__ __forceinline__ void on_bsdf_sample(const DevScene&, const PathState&, const BsdfSample&,
+                                                   const ShadeCtx&) {}
+    __device__ __forceinline__ void punctual_lights(const DevScene& s, PathState& st, const ShadeCtx& x) {
+        const F3 p = x.p, wo = x.wo, n = x.n;
+        const MatEval<float>& m = *x.me;
         for (uint32_t i = 0; i < s.num_lights; ++i) {
             F3 wi, inten;
             float dist;
