@@ -323,6 +323,46 @@ pt_status pt_scene_get_wavefront_config(const pt_scene* scene, pt_wavefront_conf
  * frame_count are untouched. */
 pt_status pt_scene_tune_wavefront(pt_scene* scene, int max_trials, pt_wavefront_config* best, float* ms_per_spp);
 pt_status pt_scene_wavefront_stats(const pt_scene* scene, pt_wavefront_stats* out);
+
+/* ------------------------------------------------------------------------ */
+/* Differentiation (no reference counterpart: the reference renders only)    */
+/* ------------------------------------------------------------------------ */
+/* Gradient buffers, device memory, accumulated with += (zero them first);
+ * any may be NULL.  base_color: num_materials x 3 (the baseColor factor);
+ * roughness, metallic: num_materials; textures: 4 floats per texel of the
+ * texel pool (pt_scene_texture_offset), gradients of base-colour textures;
+ * env: env_width x env_height x 4. */
+typedef struct pt_adjoint_grads {
+    float* base_color;
+    float* roughness;
+    float* metallic;
+    float* textures;
+    float* env;
+} pt_adjoint_grads;
+/* Path-replay backpropagation of batch `frame` (sample_per_frame samples per
+ * pixel with the random numbers pt_render_batch used at that frame count):
+ * image_grad (device, W*H*4 floats) is d loss / d (batch mean image, RGB;
+ * the 4th channel is ignored).  replay_image (device W*H*4, may be NULL)
+ * gets the batch image recomputed by the replay.  Sampling pdfs, MIS weights
+ * and Russian roulette are detached; channels clamped at 100 carry no
+ * gradient; FP32 atomics accumulate (last-bit nondeterminism). */
+pt_status pt_render_adjoint(pt_scene* scene, uint32_t frame, const float* image_grad, const pt_adjoint_grads* grads,
+                            float* replay_image, float* ms_out);
+/* Sets frame_count (the sample-index offset of the next batch). */
+pt_status pt_scene_set_frame(pt_scene* scene, uint32_t frame_count);
+/* First texel and texel count of texture tex in the device texel pool. */
+pt_status pt_scene_texture_offset(const pt_scene* scene, uint32_t tex, uint64_t* texel_offset,
+                                  uint64_t* texel_count);
+/* Parameter updates of an uploaded scene without a re-upload: material
+ * baseColor (num_materials x 3), roughness, metallic from host arrays (NULL
+ * keeps the field); texels of texture tex (0xFFFFFFFF = the environment)
+ * from host or device memory (on_device), RGBA floats of the same size.
+ * The environment's sampling tables keep their old distribution until
+ * pt_scene_rebuild_env_tables. */
+pt_status pt_scene_update_materials(pt_scene* scene, const float* base_color, const float* roughness,
+                                    const float* metallic);
+pt_status pt_scene_update_texture(pt_scene* scene, uint32_t tex, const float* rgba, size_t count, int on_device);
+pt_status pt_scene_rebuild_env_tables(pt_scene* scene);
 /* Counter control: when enabled, render kernels count rays, node visits and
  * triangle tests (slower; used to price algorithmic bytes). */
 pt_status pt_counters_enable(pt_scene* scene, int enable);

@@ -164,6 +164,11 @@ class WavefrontStats(C.Structure):
 WAVEFRONT_STAGES = ("generate", "extend", "sort", "shade", "shadow", "control")
 
 
+class AdjointGrads(C.Structure):
+    _fields_ = [("base_color", C.c_void_p), ("roughness", C.c_void_p), ("metallic", C.c_void_p),
+                ("textures", C.c_void_p), ("env", C.c_void_p)]
+
+
 # Every exported symbol with its signature (kept in sync with include/pt_amd.h).
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
@@ -206,6 +211,12 @@ SIGNATURES = {
     "pt_scene_get_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),
     "pt_scene_tune_wavefront": (C.c_int, [_P, C.c_int, C.POINTER(WavefrontConfig), _FP]),
     "pt_scene_wavefront_stats": (C.c_int, [_P, C.POINTER(WavefrontStats)]),
+    "pt_render_adjoint": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.POINTER(AdjointGrads), C.c_void_p, _FP]),
+    "pt_scene_set_frame": (C.c_int, [_P, C.c_uint32]),
+    "pt_scene_texture_offset": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pt_scene_update_materials": (C.c_int, [_P, _FP, _FP, _FP]),
+    "pt_scene_update_texture": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_size_t, C.c_int]),
+    "pt_scene_rebuild_env_tables": (C.c_int, [_P]),
     "pt_counters_enable": (C.c_int, [_P, C.c_int]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
     "pt_counters_reset": (C.c_int, [_P]),

@@ -26,7 +26,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 ARCH = "gfx950"
 HOST_SOURCES = ["host/json.cpp", "host/scene_io.cpp", "host/generators.cpp", "host/light_tables.cpp"]
-HIP_SOURCES = ["capi.hip", "device/bvh_build.hip", "device/render.hip", "device/wavefront.hip"]
+HIP_SOURCES = ["capi.hip", "device/bvh_build.hip", "device/render.hip", "device/wavefront.hip", "device/adjoint.hip"]
 CLI_SOURCES = ["cli/pt_main.cpp"]
 
 CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

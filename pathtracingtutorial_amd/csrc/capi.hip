@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "device/adjoint.h"
 #include "device/bvh_build.h"
 #include "device/dev_scene.h"
 #include "device/render.h"
@@ -874,6 +875,128 @@ pt_status pt_scene_get_wavefront_config(const pt_scene* s, pt_wavefront_config* 
     return guarded([&] {
         if (!s || !cfg) throw Error(PT_ERR_INVALID_ARG, "null argument");
         *cfg = to_abi_config(s->wf_cfg);
+    });
+}
+
+pt_status pt_render_adjoint(pt_scene* s, uint32_t frame, const float* image_grad, const pt_adjoint_grads* g,
+                            float* replay_image, float* ms_out) {
+    return guarded([&] {
+        if (!s || !image_grad || !g) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        ptd::DevScene v = make_view(s);
+        ptd::RenderParams p = make_params(s);
+        p.frame_count = frame;
+        ptd::AdjointGrads G;
+        G.base = g->base_color;
+        G.rough = g->roughness;
+        G.metal = g->metallic;
+        G.tex = g->textures;
+        G.env = g->env;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
+        hipError_t le = ptd::launch_adjoint(v, p, reinterpret_cast<const float4*>(image_grad), G,
+                                            reinterpret_cast<float4*>(replay_image), s->counters.p, s->counting,
+                                            s->stream);
+        hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
+        hipError_t se = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hip_check(le, "adjoint launch");
+        hip_check(se, "adjoint");
+        if (ms_out) *ms_out = ms;
+    });
+}
+
+pt_status pt_scene_set_frame(pt_scene* s, uint32_t frame_count) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->frame_count = frame_count;
+    });
+}
+
+pt_status pt_scene_texture_offset(const pt_scene* s, uint32_t tex, uint64_t* texel_offset, uint64_t* texel_count) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (tex >= s->host.textures.size()) throw Error(PT_ERR_INVALID_ARG, "texture index out of range");
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < tex; ++i)
+            off += static_cast<uint64_t>(s->host.textures[i].width) * s->host.textures[i].height;
+        if (texel_offset) *texel_offset = off;
+        if (texel_count)
+            *texel_count = static_cast<uint64_t>(s->host.textures[tex].width) * s->host.textures[tex].height;
+    });
+}
+
+pt_status pt_scene_update_materials(pt_scene* s, const float* base_color, const float* roughness,
+                                    const float* metallic) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        auto& M = s->host.materials;
+        for (size_t i = 0; i < M.size(); ++i) {
+            if (base_color) {
+                M[i].base_color.x = base_color[3 * i + 0];
+                M[i].base_color.y = base_color[3 * i + 1];
+                M[i].base_color.z = base_color[3 * i + 2];
+            }
+            if (roughness) M[i].roughness = roughness[i];
+            if (metallic) M[i].metallic = metallic[i];
+        }
+        if (s->uploaded && s->materials.p && s->materials.n == M.size())
+            hip_check(hipMemcpy(s->materials.p, M.data(), sizeof(pt_material) * M.size(), hipMemcpyHostToDevice),
+                      "material update");
+    });
+}
+
+pt_status pt_scene_update_texture(pt_scene* s, uint32_t tex, const float* rgba, size_t count, int on_device) {
+    return guarded([&] {
+        if (!s || !rgba) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        ptamd::Image* img = nullptr;
+        float4* dst = nullptr;
+        if (tex == 0xFFFFFFFFu) {
+            img = &s->host.env;
+            dst = s->env.p;
+        } else {
+            if (tex >= s->host.textures.size()) throw Error(PT_ERR_INVALID_ARG, "texture index out of range");
+            img = &s->host.textures[tex];
+            uint64_t off = 0;
+            for (uint32_t i = 0; i < tex; ++i)
+                off += static_cast<uint64_t>(s->host.textures[i].width) * s->host.textures[i].height;
+            dst = s->texels.p + off;
+        }
+        if (img->empty() || dst == nullptr || count != img->rgba.size())
+            throw Error(PT_ERR_INVALID_ARG, "texel count does not match the texture");
+        const size_t bytes = count * sizeof(float);
+        if (on_device) {
+            hip_check(hipMemcpy(dst, rgba, bytes, hipMemcpyDeviceToDevice), "texture update");
+            hip_check(hipMemcpy(img->rgba.data(), rgba, bytes, hipMemcpyDeviceToHost), "texture update");
+        } else {
+            std::memcpy(img->rgba.data(), rgba, bytes);
+            hip_check(hipMemcpy(dst, rgba, bytes, hipMemcpyHostToDevice), "texture update");
+        }
+    });
+}
+
+pt_status pt_scene_rebuild_env_tables(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        ptamd::LightTables t;
+        ptamd::build_env_tables(s->host.env, t);
+        if (t.env_pmf.size() != s->env_pmf.n || t.env_alias.size() != s->env_alias.n)
+            throw Error(PT_ERR_STATE, "environment sampling tables changed size (an all-zero environment?)");
+        s->lights.env_pmf = t.env_pmf;
+        s->lights.env_alias = t.env_alias;
+        if (!t.env_pmf.empty()) {
+            hip_check(hipMemcpy(s->env_pmf.p, t.env_pmf.data(), t.env_pmf.size() * 4, hipMemcpyHostToDevice),
+                      "env table update");
+            hip_check(hipMemcpy(s->env_alias.p, t.env_alias.data(), t.env_alias.size() * 8, hipMemcpyHostToDevice),
+                      "env table update");
+        }
     });
 }
 

@@ -82,8 +82,12 @@ __device__ __forceinline__ int fold_index(int i, int n, int mode) {
 
 __device__ __forceinline__ float filt_weight(float f) { return rintf(f * 256.0f) * (1.0f / 256.0f); }
 
-__device__ __forceinline__ float4 tex_bilinear(const float4* texels, int w, int h, int wrap_s, int wrap_t, float u,
-                                               float v) {
+// The four texels and weights of a bilinear lookup (order 00, 10, 01, 11).
+struct TexTaps {
+    uint32_t idx[4];
+    float w[4];
+};
+__device__ __forceinline__ TexTaps tex_taps(int w, int h, int wrap_s, int wrap_t, float u, float v) {
     if (wrap_s == 0) u = u - floorf(u);
     if (wrap_t == 0) v = v - floorf(v);
     float x = u * static_cast<float>(w) - 0.5f, y = v * static_cast<float>(h) - 0.5f;
@@ -92,9 +96,24 @@ __device__ __forceinline__ float4 tex_bilinear(const float4* texels, int w, int 
     int x0 = static_cast<int>(fx), y0 = static_cast<int>(fy);
     int xa = fold_index(x0, w, wrap_s), xb = fold_index(x0 + 1, w, wrap_s);
     int ya = fold_index(y0, h, wrap_t), yb = fold_index(y0 + 1, h, wrap_t);
-    float4 t00 = texels[static_cast<size_t>(ya) * w + xa], t10 = texels[static_cast<size_t>(ya) * w + xb];
-    float4 t01 = texels[static_cast<size_t>(yb) * w + xa], t11 = texels[static_cast<size_t>(yb) * w + xb];
-    float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    TexTaps t;
+    t.idx[0] = static_cast<uint32_t>(ya * w + xa);
+    t.idx[1] = static_cast<uint32_t>(ya * w + xb);
+    t.idx[2] = static_cast<uint32_t>(yb * w + xa);
+    t.idx[3] = static_cast<uint32_t>(yb * w + xb);
+    t.w[0] = (1.0f - a) * (1.0f - b);
+    t.w[1] = a * (1.0f - b);
+    t.w[2] = (1.0f - a) * b;
+    t.w[3] = a * b;
+    return t;
+}
+
+__device__ __forceinline__ float4 tex_bilinear(const float4* texels, int w, int h, int wrap_s, int wrap_t, float u,
+                                               float v) {
+    TexTaps tp = tex_taps(w, h, wrap_s, wrap_t, u, v);
+    float4 t00 = texels[tp.idx[0]], t10 = texels[tp.idx[1]];
+    float4 t01 = texels[tp.idx[2]], t11 = texels[tp.idx[3]];
+    float w00 = tp.w[0], w10 = tp.w[1], w01 = tp.w[2], w11 = tp.w[3];
     return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
                        w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
                        w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z,

@@ -521,17 +521,24 @@ __device__ __forceinline__ F3 clamp_radiance(F3 lo) {
     return f3(fminf(lo.x, 100.0f), fminf(lo.y, 100.0f), fminf(lo.z, 100.0f));
 }
 
-template <bool COUNT>
-__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
+// One path with a given visibility policy; returns its radiance before the clamp.
+template <bool COUNT, class Nee>
+__device__ __forceinline__ F3 path_radiance(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng,
+                                            TraceCtx& c, Nee& nee) {
     PathState st;
     begin_path(p, x, y, rng, st);
-    InlineNee<COUNT> nee(c);
     for (; st.depth < p.max_depth; ++st.depth) {
         c.closest++;
         Hit h = trace_closest<COUNT>(s, st.o, st.d, c.stk, c.stride, c.cnt, c.overflow);
         if (!bounce(s, p, h, st, rng, nee)) break;
     }
-    return clamp_radiance(st.lo);
+    return st.lo;
+}
+
+template <bool COUNT>
+__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
+    InlineNee<COUNT> nee(c);
+    return clamp_radiance(path_radiance<COUNT>(s, p, x, y, rng, c, nee));
 }
 
 }  // namespace ptd
