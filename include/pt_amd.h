@@ -231,7 +231,10 @@ pt_status pt_scene_load(const char* path, pt_scene** out);
 /* Builds a scene from host arrays (HostScene, scene/scene.h:24-35). */
 pt_status pt_scene_create(const pt_host_scene_desc* desc, pt_scene** out);
 /* Procedural workloads: 0 = Cornell box (36 tris, 512x512), 1 = 64 displaced
- * icospheres (1,310,720 tris + ground, 2048x1024 sky+sun env, 1920x1080).
+ * icospheres (1,310,720 tris + ground, 2048x1024 sky+sun env, 1920x1080),
+ * 2 = 8 small icospheres (tests), 3 = differentiable Cornell box (8
+ * materials, 256x256 albedo texture on the back wall, 512x256 environment,
+ * 256x256, 64 spp, depth 6; the seeded values are a ground truth).
  * No reference counterpart; seeds fix every value. */
 pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out);
 void pt_scene_destroy(pt_scene* scene);
@@ -302,6 +305,8 @@ pt_status pt_film_reset(pt_film* film, pt_scene* scene);
 pt_status pt_film_download(const pt_film* film, int which, float* rgba_host);
 /* Device pointer of the acc (which 0) or new (which 1) surface, W*H*4 floats. */
 pt_status pt_film_device_ptr(const pt_film* film, int which, void** dev_ptr);
+/* Copies the acc (0) or new (1) surface into device memory dst (W*H*4 floats). */
+pt_status pt_film_copy_device(const pt_film* film, int which, float* dst_device);
 
 /* ------------------------------------------------------------------------ */
 /* Rendering                                                                  */

@@ -53,6 +53,8 @@ _SIGS = {
     "oracle_tex2d": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _FP]),
     "oracle_render": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                              C.c_int, C.c_int, C.c_int, _FP, _U64P]),
+    "oracle_adjoint_fd": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _FP, C.c_int, _I32P,
+                                 _I32P, _I32P, _I32P, C.c_double, C.c_int, _DP, _DP]),
 }
 
 _lib = None
@@ -184,6 +186,22 @@ class OracleScene:
         lib().oracle_render(self.h, C.byref(camera), spf, max_depth, sampler, seed & 0xFFFFFFFF, frame, x0, x1, y0,
                             y1, threads, 1 if fp64 else 0, _f(out), cnt.ctypes.data_as(_U64P))
         return out, {"closest_rays": int(cnt[0]), "shadow_rays": int(cnt[1]), "paths": int(cnt[2])}
+
+    def adjoint_fd(self, camera, spf: int, max_depth: int, sampler: int, seed: int, frame: int,
+                   weights: np.ndarray, params, h: float = 1e-3, threads: int = 0):
+        """Central differences of sum(weights * batch image) for params = [(kind, a, b, c), ...]
+        (kind 0 baseColor, 1 roughness, 2 metallic, 3 texture texel, 4 env texel) of the
+        detached estimator in double precision; returns (gradients, value at the current parameters)."""
+        weights = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        assert weights.size == camera.width * camera.height * 3
+        p = np.ascontiguousarray(np.array(params, np.int32).reshape(-1, 4))
+        k, a, b, c = (np.ascontiguousarray(p[:, i]) for i in range(4))
+        grad = np.zeros(len(p), np.float64)
+        value = C.c_double(0.0)
+        lib().oracle_adjoint_fd(self.h, C.byref(camera), spf, max_depth, sampler, seed & 0xFFFFFFFF, frame,
+                                _f(weights), len(p), _i32(k), _i32(a), _i32(b), _i32(c), h, threads,
+                                grad.ctypes.data_as(_DP), C.byref(value))
+        return grad, float(value.value)
 
     def __del__(self):
         try:

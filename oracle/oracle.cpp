@@ -493,7 +493,11 @@ inline int foldIndex(long long i, int n, int mode) {
 }
 inline int wrapMode(int gl) { return gl == 33071 ? 1 : (gl == 33648 ? 2 : 0); }
 
-void tex2D(const float* rgba, int w, int h, int ws, int wt, float u, float v, float out[4]) {
+// The filter weights are those of the texture unit (float coordinates, 8-bit
+// fractions); the weighted sum is formed in T (float: the renderer's
+// arithmetic; double: the FP64 oracle's).
+template <class T>
+void tex2DT(const float* rgba, int w, int h, int ws, int wt, float u, float v, T out[4]) {
     if (ws == 0) u = u - std::floor(u);
     if (wt == 0) v = v - std::floor(v);
     float x = u * static_cast<float>(w) - 0.5f, y = v * static_cast<float>(h) - 0.5f;
@@ -509,8 +513,12 @@ void tex2D(const float* rgba, int w, int h, int ws, int wt, float u, float v, fl
     const float* t10 = rgba + 4 * (static_cast<size_t>(ya) * w + xb);
     const float* t01 = rgba + 4 * (static_cast<size_t>(yb) * w + xa);
     const float* t11 = rgba + 4 * (static_cast<size_t>(yb) * w + xb);
+    const float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
     for (int c = 0; c < 4; ++c)
-        out[c] = (1 - a) * (1 - b) * t00[c] + a * (1 - b) * t10[c] + (1 - a) * b * t01[c] + a * b * t11[c];
+        out[c] = T(w00) * T(t00[c]) + T(w10) * T(t10[c]) + T(w01) * T(t01[c]) + T(w11) * T(t11[c]);
+}
+void tex2D(const float* rgba, int w, int h, int ws, int wt, float u, float v, float out[4]) {
+    tex2DT<float>(rgba, w, h, ws, wt, u, v, out);
 }
 
 // -------------------------------------------------------- materials/microfacet.cuh
@@ -592,9 +600,9 @@ struct MatEval {  // :20-36
     R emissiveStrength, anisotropyRotation, thickness, attDistance;
 };
 
-template <class R> V3<R> powColor(const float c[4], bool linear) {
-    if (linear) return V3<R>(R(c[0]), R(c[1]), R(c[2]));
-    return V3<R>(R(std::pow(c[0], 2.2f)), R(std::pow(c[1], 2.2f)), R(std::pow(c[2], 2.2f)));
+template <class R> V3<R> powColor(const R c[4], bool linear) {
+    if (linear) return V3<R>(c[0], c[1], c[2]);
+    return V3<R>(std::pow(c[0], R(2.2)), std::pow(c[1], R(2.2)), std::pow(c[2], R(2.2)));
 }
 
 template <class R>
@@ -615,13 +623,13 @@ MatEval<R> fetchMaterial(const oracle_scene* s, const pt_material& m, float u, f
     e.attColor = V3<R>(R(m.att_color_r), R(m.att_color_g), R(m.att_color_b));
     if (s && !s->tex.empty()) {
         auto valid = [&](int id) { return id >= 0 && static_cast<size_t>(id) < s->tex.size(); };
-        auto fetch = [&](int id, float o[4]) {
+        auto fetch = [&](int id, R o[4]) {
             const Texture& t = s->tex[id];
-            tex2D(t.rgba.data(), t.w, t.h, t.wrapS, t.wrapT, u, v, o);
+            tex2DT<R>(t.rgba.data(), t.w, t.h, t.wrapS, t.wrapT, u, v, o);
         };
-        float c[4];
+        R c[4];
         if (valid(m.base_color_tex_id)) { fetch(m.base_color_tex_id, c); e.base = e.base * powColor<R>(c, s->tex[m.base_color_tex_id].linear); }
-        if (valid(m.metallic_roughness_tex_id)) { fetch(m.metallic_roughness_tex_id, c); e.roughness *= R(c[1]); e.metallic *= R(c[2]); }
+        if (valid(m.metallic_roughness_tex_id)) { fetch(m.metallic_roughness_tex_id, c); e.roughness *= c[1]; e.metallic *= c[2]; }
         if (valid(m.emissive_tex_id)) { fetch(m.emissive_tex_id, c); e.emissive = e.emissive * powColor<R>(c, s->tex[m.emissive_tex_id].linear); }
         if (valid(m.clearcoat_tex_id)) { fetch(m.clearcoat_tex_id, c); e.clearcoat *= R(c[0]); }
         if (valid(m.clearcoat_roughness_tex_id)) { fetch(m.clearcoat_roughness_tex_id, c); e.clearcoatRoughness *= R(c[1]); }
@@ -950,10 +958,10 @@ template <class R> V3<R> envRadiance(const oracle_scene* s, const V3<R>& d) {  /
     if (s->env.w <= 0) return V3<R>(0);
     R u, v;
     dirToUv(d, u, v);
-    float c[4];
-    tex2D(s->env.rgba.data(), s->env.w, s->env.h, s->env.wrapS, s->env.wrapT, static_cast<float>(u),
-          static_cast<float>(v), c);
-    return V3<R>(R(c[0]), R(c[1]), R(c[2])) * K<R>::TwoPi;
+    R c[4];
+    tex2DT<R>(s->env.rgba.data(), s->env.w, s->env.h, s->env.wrapS, s->env.wrapT, static_cast<float>(u),
+              static_cast<float>(v), c);
+    return V3<R>(c[0], c[1], c[2]) * K<R>::TwoPi;
 }
 template <class R> R envPdf(const oracle_scene* s, const V3<R>& d) {  // light_pdf.cuh:12-27
     if (s->env.w <= 0 || s->envPmf.empty()) return R(0);
@@ -1239,6 +1247,267 @@ V3<R> pathTrace(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, 
     return Lo;
 }
 
+// ----------------------------------------------- detached estimator (adjoint)
+// A path traced at the current parameters is recorded as a tape of its
+// decisions (hit points, sampled and light directions, pdfs, MIS weights,
+// Russian-roulette factors).  Evaluating the tape with other material /
+// texture / environment values gives the "detached" estimator L(theta):
+// directions and pdfs stay those of the recorded path, while BSDF values,
+// base colours and environment radiance follow theta.  Its derivative is
+// what a path-replay adjoint estimates, so central differences of it check
+// an adjoint path by path.
+struct TapeVertex {
+    int kind = 2;  // 0 environment on a miss, 1 emitter hit, 2 surface, 3 alpha pass-through
+    V3<float> d;   // kind 0: direction
+    double mis = 1.0;
+    V3<double> Le;  // kind 1: emitted radiance
+    int mat = 0;
+    float mu = 0, mv = 0;
+    V3<float> wo, N;
+    bool entering = true;
+    bool nee = false, hasA = false, hasE = false;
+    V3<float> wiA, wiE;
+    V3<double> LA;
+    double kA = 0, kE = 0;  // mis / (pdf_light + 1e-7) at the recorded parameters
+    std::vector<std::pair<V3<float>, V3<double>>> punct;  // visible punctual lights: direction, intensity
+    bool valid = false, delta = false;  // BSDF continuation
+    V3<float> wi;
+    double pdf0 = 0;
+    V3<double> beer = V3<double>(1.0);
+    double surv = 1.0;
+    bool rrDead = false;
+};
+
+inline V3<double> d3(const V3<float>& v) { return V3<double>(v.x, v.y, v.z); }
+
+// pathTrace<float> with the tape recorded (same draws, same decisions).
+std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, int maxDepth, int x,
+                                 int y, Sampler& rng, Ctx& c) {
+    using R = float;
+    std::vector<TapeVertex> tape;
+    V3<R> T(1);
+    R jx = R(rng.get1D());
+    R jy = R(rng.get1D());
+    R u = (R(2) * (R(x) + jx) / R(W) - R(1)) * R(cf.tan_half_fov) * R(cf.aspect);
+    R v = (R(2) * (R(y) + jy) / R(H) - R(1)) * R(cf.tan_half_fov);
+    V3<R> pos(cf.position[0], cf.position[1], cf.position[2]);
+    V3<R> fwd(cf.forward[0], cf.forward[1], cf.forward[2]);
+    V3<R> rgt(cf.right[0], cf.right[1], cf.right[2]);
+    V3<R> cup(cf.up[0], cf.up[1], cf.up[2]);
+    V3<R> dir = normalize(u * rgt + v * cup + fwd);
+    if (cf.aperture > 0.0f) {
+        V3<R> focal = pos + dir * (R(cf.focal_distance) / dot(dir, fwd));
+        R l1 = R(rng.get1D());
+        R l2 = R(rng.get1D());
+        R r = std::sqrt(l1) * R(cf.aperture);
+        R th = K<R>::TwoPi * l2;
+        pos = pos + (rgt * (r * std::cos(th)) + cup * (r * std::sin(th)));
+        dir = normalize(focal - pos);
+    }
+    V3<R> o = pos, d = normalize(dir);
+    R lastPdf = 0;
+    bool lastDelta = true;
+    const bool hasArea = !s->emissive.empty();
+    const bool hasEnv = s->env.w > 0 && !s->envPmf.empty();
+    for (int depth = 0; depth < maxDepth; ++depth) {
+        c.closest++;
+        Hit<R> h = intersectScene(s, o, d);
+        TapeVertex tv;
+        if (!h.hit) {
+            V3<R> env = envRadiance(s, d);
+            if (env != V3<R>(0)) {
+                R mis = R(1);
+                if (depth > 0 && !lastDelta) mis = powerHeuristic(lastPdf, envPdf(s, d));
+                tv.kind = 0;
+                tv.d = d;
+                tv.mis = mis;
+                tape.push_back(tv);
+            }
+            break;
+        }
+        const pt_face& f = s->f[h.prim];
+        const pt_vertex &va = s->v[f.v0], &vb = s->v[f.v1], &vc = s->v[f.v2];
+        R w = R(1) - h.u - h.v;
+        V3<R> P = o + h.t * d;
+        V3<R> sn = normalize(v3of<R>(va.normal) * w + v3of<R>(vb.normal) * h.u + v3of<R>(vc.normal) * h.v);
+        R tu = R(va.u) * w + R(vb.u) * h.u + R(vc.u) * h.v;
+        R tvv = R(va.v) * w + R(vb.v) * h.u + R(vc.v) * h.v;
+        const pt_material& mat = s->m[f.material_id];
+        bool entering = dot(sn, d) < R(0);
+        V3<R> gn;
+        if (mat.double_sided && !entering) { gn = -sn; entering = true; }
+        else gn = entering ? sn : -sn;
+        V3<R> wo = -d;
+        float mu = tu, mv = tvv;
+        if (!(mat.uv_rotation == 0.0f && mat.uv_scale_x == 1.0f && mat.uv_scale_y == 1.0f && mat.uv_offset_x == 0.0f &&
+              mat.uv_offset_y == 0.0f)) {
+            float cr = std::cos(mat.uv_rotation), sr = std::sin(mat.uv_rotation);
+            float ru = cr * mu + sr * mv, rv = -sr * mu + cr * mv;
+            mu = ru * mat.uv_scale_x + mat.uv_offset_x;
+            mv = rv * mat.uv_scale_y + mat.uv_offset_y;
+        }
+        V3<R> N = gn;
+        MatEval<R> me = fetchMaterial<R>(s, mat, mu, mv);
+        if (isEmissive(mat)) {
+            R mis = R(1);
+            if (depth > 0 && !lastDelta) mis = powerHeuristic(lastPdf, emissivePdf(s, h.prim, h.t, d));
+            tv.kind = 1;
+            tv.Le = d3(v3of<R>(mat.emissive) * R(mat.emissive_strength));
+            tv.mis = mis;
+            tape.push_back(tv);
+            break;
+        }
+        if (mat.alpha_mode == 1 || mat.alpha_mode == 2) {
+            float alpha = mat.alpha_mode == 1 ? 1.0f : mat.base_alpha;
+            int tid = mat.base_color_tex_id;
+            if (tid >= 0 && static_cast<size_t>(tid) < s->tex.size()) {
+                float cc[4];
+                const Texture& t = s->tex[tid];
+                tex2D(t.rgba.data(), t.w, t.h, t.wrapS, t.wrapT, tu, tvv, cc);
+                alpha = mat.alpha_mode == 1 ? cc[3] : alpha * cc[3];
+            }
+            bool skip = mat.alpha_mode == 1 ? (alpha < mat.alpha_cutoff) : (rng.get1D() >= alpha);
+            if (skip) {
+                tv.kind = 3;
+                tape.push_back(tv);
+                o = P + d * R(1e-4);
+                d = normalize(d);
+                continue;
+            }
+        }
+        tv.kind = 2;
+        tv.mat = static_cast<int>(f.material_id);
+        tv.mu = mu;
+        tv.mv = mv;
+        tv.wo = wo;
+        tv.N = N;
+        tv.entering = entering;
+        bool deltaSurf = (me.metallic >= R(1) && me.roughness < R(0.01)) || (me.specTrans >= R(1) && me.roughness < R(0.01));
+        if (!deltaSurf) {
+            tv.nee = true;
+            auto record = [&](const LightSample<R>& ls, bool env) {
+                if (!ls.valid) return;
+                BsdfEval<R> b = evalDisneyBsdf(me, wo, ls.wi, N, true);
+                if (b.pdf <= R(0)) return;
+                double k = double(powerHeuristic(ls.pdf, b.pdf)) / (double(ls.pdf) + 1e-7);
+                if (env) { tv.hasE = true; tv.wiE = ls.wi; tv.kE = k; }
+                else { tv.hasA = true; tv.wiA = ls.wi; tv.LA = d3(ls.L); tv.kA = k; }
+            };
+            if (hasArea) record(sampleAreaLight<R>(s, P, N, rng, c), false);
+            if (hasEnv) record(sampleEnvLight<R>(s, P, N, rng, c), true);
+            for (const auto& l : s->lights) {  // evaluatePunctualLights' visibility, recorded
+                V3<R> wi;
+                R dist = R(FLT_MAX);
+                V3<R> I;
+                if (l.type == 0 || l.type == 2) {
+                    V3<R> tl = v3of<R>(l.position) - P;
+                    R d2 = dot(tl, tl);
+                    if (d2 < R(1e-8)) continue;
+                    dist = std::sqrt(d2);
+                    wi = tl / dist;
+                    R att = dist >= R(l.range) ? R(0) : R(1) / (dist * dist + R(1e-7));
+                    if (l.type == 2) {
+                        R ca = dot(normalize(-tl), v3of<R>(l.direction));
+                        R sp;
+                        if (ca < R(l.outer_cone)) sp = R(0);
+                        else if (ca > R(l.inner_cone)) sp = R(1);
+                        else {
+                            R t = (ca - R(l.outer_cone)) / (R(l.inner_cone) - R(l.outer_cone) + R(1e-7));
+                            sp = t * t;
+                        }
+                        att *= sp;
+                    }
+                    I = v3of<R>(l.color) * (R(l.intensity) * att);
+                } else {
+                    wi = -v3of<R>(l.direction);
+                    I = v3of<R>(l.color) * R(l.intensity);
+                }
+                if (dot(N, wi) <= R(0)) continue;
+                if (isOccluded(s, P, N, wi, dist, c)) continue;
+                tv.punct.push_back({wi, d3(I)});
+            }
+        }
+        BsdfSample<R> bs = sampleDisneyBsdf(me, wo, N, rng, entering);
+        if (!bs.valid) {
+            tape.push_back(tv);
+            break;
+        }
+        tv.valid = true;
+        tv.delta = bs.delta;
+        tv.wi = bs.wi;
+        tv.pdf0 = bs.pdf;
+        T *= bs.weight;
+        bool trans = dot(bs.wi, gn) < R(0);
+        V3<R> off = trans ? -gn : gn;
+        o = P + off * R(1e-4);
+        d = normalize(bs.wi);
+        if (trans && me.thickness > R(0) && me.attDistance < R(1e20)) {
+            R t = me.thickness, ad = me.attDistance;
+            V3<R> ac = me.attColor;
+            V3<R> bf(std::exp(-t * fmx(-std::log(fmx(ac.x, R(1e-6))), R(0)) / ad),
+                     std::exp(-t * fmx(-std::log(fmx(ac.y, R(1e-6))), R(0)) / ad),
+                     std::exp(-t * fmx(-std::log(fmx(ac.z, R(1e-6))), R(0)) / ad));
+            T *= bf;
+            tv.beer = d3(bf);
+        }
+        lastPdf = bs.pdf;
+        lastDelta = bs.delta;
+        if (depth > 2) {
+            R surv = fmn(fmx(fmx(T.x, T.y), T.z), R(0.95));
+            tv.surv = surv;
+            if (R(rng.get1D()) >= surv) {
+                tv.rrDead = true;
+                tape.push_back(tv);
+                break;
+            }
+            T /= surv;
+        }
+        tape.push_back(tv);
+    }
+    return tape;
+}
+
+// The detached estimator of a tape under the parameters of scene sp (no clamp).
+V3<double> tapeEval(const oracle_scene* sp, const std::vector<TapeVertex>& tape) {
+    V3<double> L(0), T(1);
+    for (const TapeVertex& v : tape) {
+        if (v.kind == 0) {
+            L += T * envRadiance(sp, d3(v.d)) * v.mis;
+            break;
+        }
+        if (v.kind == 1) {
+            L += T * v.Le * v.mis;
+            break;
+        }
+        if (v.kind == 3) continue;
+        MatEval<double> me = fetchMaterial<double>(sp, sp->m[v.mat], v.mu, v.mv);
+        const V3<double> wo = d3(v.wo), N = d3(v.N);
+        if (v.nee) {
+            V3<double> ld(0);
+            if (v.hasA) ld += v.LA * evalDisneyBsdf(me, wo, d3(v.wiA), N, true).f * v.kA;
+            if (v.hasE) ld += envRadiance(sp, d3(v.wiE)) * evalDisneyBsdf(me, wo, d3(v.wiE), N, true).f * v.kE;
+            L += T * ld;
+            V3<double> pl(0);
+            for (const auto& p : v.punct) pl += p.second * evalDisneyBsdf(me, wo, d3(p.first), N, true).f;
+            L += T * pl;
+        }
+        if (!v.valid) break;
+        const V3<double> wi = d3(v.wi);
+        V3<double> weight;
+        if (v.delta) {
+            if (dot(N, wi) < 0.0) weight = me.base;
+            else weight = F_SchlickV(fmx(dot(normalize(wo + wi), wi), 0.0), computeF0(me));
+        } else {
+            weight = evalDisneyBsdf(me, wo, wi, N, v.entering).f / (v.pdf0 + 1e-7);
+        }
+        T *= weight;
+        T *= v.beer;
+        if (v.rrDead) break;
+        T /= v.surv;
+    }
+    return L;
+}
+
 }  // namespace orc
 
 // =============================================================== C interface
@@ -1448,6 +1717,79 @@ void oracle_render(const oracle_scene* s, const pt_camera* cam, int spf, int max
         counters[0] = cl.load();
         counters[1] = sh.load();
         counters[2] = paths.load();
+    }
+}
+
+void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
+                       uint32_t frame, const float* weights, int nparams, const int* kind, const int* pa, const int* pb,
+                       const int* pc, double h, int threads, double* grad, double* value) {
+    pt_camera_frame cf;
+    oracle_camera_frame(cam, &cf);
+    const int W = cam->width, H = cam->height;
+    const size_t npaths = static_cast<size_t>(W) * H * spf;
+    std::vector<std::vector<TapeVertex>> tapes(npaths);
+    std::vector<float> f32(static_cast<size_t>(W) * H * 3, 0.0f);
+    int nt = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    auto parallel = [&](size_t n, auto&& fn) {
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            while (true) {
+                size_t i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int i = 1; i < nt; ++i) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    };
+    parallel(static_cast<size_t>(W) * H, [&](size_t px) {
+        const int x = static_cast<int>(px % W), y = static_cast<int>(px / W);
+        Ctx c;
+        for (int i = 0; i < spf; ++i) {
+            Sampler rng;
+            if (sampler == 1) rng.initSobol(x, y, frame * static_cast<uint32_t>(spf) + static_cast<uint32_t>(i), seed);
+            else rng.initPCG(x, y, frame, static_cast<uint32_t>(i), seed);
+            tapes[px * spf + i] = pathTape(s, cf, W, H, maxDepth, x, y, rng, c);
+        }
+    });
+    auto image_dot = [&](const oracle_scene* sp) {
+        std::vector<double> part(static_cast<size_t>(W) * H, 0.0);
+        parallel(static_cast<size_t>(W) * H, [&](size_t px) {
+            double acc = 0.0;
+            for (int i = 0; i < spf; ++i) {
+                V3<double> L = tapeEval(sp, tapes[px * spf + i]);
+                acc += weights[3 * px + 0] * std::fmin(L.x, 100.0) + weights[3 * px + 1] * std::fmin(L.y, 100.0) +
+                       weights[3 * px + 2] * std::fmin(L.z, 100.0);
+            }
+            part[px] = acc / spf;
+        });
+        double sum = 0.0;
+        for (double v : part) sum += v;
+        return sum;
+    };
+    if (value) *value = image_dot(s);
+    for (int k = 0; k < nparams; ++k) {
+        oracle_scene sp = *s, sm = *s;
+        // the parameter is a float: the step actually taken is the difference
+        // of the two rounded values
+        auto field = [&](oracle_scene& t) -> float& {
+            const int a = pa[k], b = pb[k], ch = pc[k];
+            switch (kind[k]) {
+                case 0: return (&t.m[a].base_color.x)[ch];
+                case 1: return t.m[a].roughness;
+                case 2: return t.m[a].metallic;
+                case 3: return t.tex[a].rgba[4 * static_cast<size_t>(b) + ch];
+                default: return t.env.rgba[4 * static_cast<size_t>(b) + ch];
+            }
+        };
+        float& fp = field(sp);
+        float& fm = field(sm);
+        const double x0 = fp;
+        fp = static_cast<float>(x0 + h);
+        fm = static_cast<float>(x0 - h);
+        grad[k] = (image_dot(&sp) - image_dot(&sm)) / (static_cast<double>(fp) - static_cast<double>(fm));
     }
 }
 

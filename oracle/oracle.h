@@ -85,6 +85,21 @@ void oracle_render(const oracle_scene* s, const pt_camera* cam, int spf, int max
                    uint32_t frame, int x0, int x1, int y0, int y1, int threads, int fp64, float* out,
                    uint64_t* counters);
 
+/* Gradient of  sum_pixels weights . I_batch  (I_batch = the batch mean image
+ * of `frame`, weights W*H*3) with respect to nparams scalars, by central
+ * differences with step h of the detached estimator in double precision:
+ * every path is traced once at the current parameters (float, the
+ * integrator's draws and decisions) and recorded; the recorded paths are then
+ * evaluated with each parameter at +h and -h (directions, pdfs, MIS weights
+ * and Russian-roulette factors kept).  kind: 0 material a baseColor channel c,
+ * 1 material a roughness, 2 material a metallic, 3 texture a texel b channel
+ * c, 4 environment texel b channel c.  value (may be null) gets the weighted
+ * sum at the current parameters.  Parameters are floats: the difference is
+ * divided by the step between the two rounded values actually used. */
+void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int max_depth, int sampler, uint32_t seed,
+                       uint32_t frame, const float* weights, int nparams, const int* kind, const int* a, const int* b,
+                       const int* c, double h, int threads, double* grad, double* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,6 +205,7 @@ SIGNATURES = {
     "pt_film_reset": (C.c_int, [_P, _P]),
     "pt_film_download": (C.c_int, [_P, C.c_int, _FP]),
     "pt_film_device_ptr": (C.c_int, [_P, C.c_int, _PP]),
+    "pt_film_copy_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
     "pt_render_batch": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_all": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_scene_set_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),

@@ -451,6 +451,7 @@ pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out) {
         if (which == 0) ptamd::generate_cornell(sc->host);
         else if (which == 1) ptamd::generate_spheres(sc->host, seed, 64, 5, 2048, 1024);
         else if (which == 2) ptamd::generate_spheres(sc->host, seed, 8, 3, 256, 128);  // small variant for tests
+        else if (which == 3) ptamd::generate_diff_cornell(sc->host, seed);
         else throw Error(PT_ERR_INVALID_ARG, "unknown workload id " + std::to_string(which));
         validate_scene(sc->host);
         *out = sc.release();
@@ -832,6 +833,15 @@ pt_status pt_film_device_ptr(const pt_film* f, int which, void** p) {
     });
 }
 
+
+pt_status pt_film_copy_device(const pt_film* f, int which, float* dst) {
+    return guarded([&] {
+        if (!f || !dst) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        const float4* src = which == 1 ? f->film_new.p : f->film_acc.p;
+        hip_check(hipMemcpy(dst, src, sizeof(float4) * static_cast<size_t>(f->w) * f->h, hipMemcpyDeviceToDevice),
+                  "film copy");
+    });
+}
 
 pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out) {
     return guarded([&] {

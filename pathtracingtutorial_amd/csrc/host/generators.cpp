@@ -268,6 +268,92 @@ bool generate_cornell(HostScene& s) {
     return true;
 }
 
+// Differentiable Cornell box: the Cornell geometry with eight materials
+// (floor, ceiling, textured back wall, red, green, light, tall box, short
+// box), a linear 256x256 RGB albedo texture on the back wall, a 512x256
+// environment seen around the open +z side, 256x256, 64 spp, depth 6, PCG.
+// The seeded values are the ground truth of an inverse-rendering problem:
+// value-noise albedo in [0.1, 0.9], roughness U[0.2, 0.9] and metallic
+// U[0, 0.6] for the six reflective materials, and a smooth two-colour sky
+// gradient of magnitude about 1.
+bool generate_diff_cornell(HostScene& s, uint64_t seed) {
+    generate_cornell(s);
+    s.path = "<diff-cornell>";
+    Rng rng(seed ^ 0xD1FFull);
+    std::vector<pt_material> old = s.materials;
+    s.materials.clear();
+    auto mk = [](float r, float g, float b) {
+        pt_material m = default_material();
+        m.base_color = v3p(r, g, b);
+        m.roughness = 1.0f;
+        return m;
+    };
+    s.materials.push_back(mk(0.73f, 0.73f, 0.73f));  // 0 floor
+    s.materials.push_back(mk(0.73f, 0.73f, 0.73f));  // 1 ceiling
+    s.materials.push_back(mk(1.0f, 1.0f, 1.0f));     // 2 back wall (albedo texture)
+    s.materials.push_back(mk(0.65f, 0.05f, 0.05f));  // 3 left
+    s.materials.push_back(mk(0.12f, 0.45f, 0.15f));  // 4 right
+    s.materials.push_back(old[3]);                   // 5 light
+    s.materials.push_back(mk(0.95f, 0.95f, 0.95f));  // 6 tall box
+    s.materials.push_back(mk(0.73f, 0.73f, 0.73f));  // 7 short box
+    for (int m : {0, 1, 2, 3, 4, 6, 7}) {
+        s.materials[m].roughness = rng.uniform(0.2f, 0.9f);
+        s.materials[m].metallic = rng.uniform(0.0f, 0.6f);
+    }
+    // faces in generate_cornell order: floor 2, ceiling 2, back 2, left 2, right 2, light 2, tall box 12, short box 12
+    const int remap[] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5};
+    for (size_t i = 0; i < s.faces.size(); ++i) {
+        if (i < 12) s.faces[i].material_id = remap[i];
+        else if (i < 24) s.faces[i].material_id = 6;
+        else s.faces[i].material_id = 7;
+    }
+    Image tex;
+    tex.width = tex.height = 256;
+    tex.wrap_s = tex.wrap_t = 10497;
+    tex.linear = 1;
+    tex.rgba.resize(256 * 256 * 4);
+    const uint32_t ts = static_cast<uint32_t>(rng.next());
+    for (int y = 0; y < 256; ++y)
+        for (int x = 0; x < 256; ++x) {
+            float* o = &tex.rgba[(static_cast<size_t>(y) * 256 + x) * 4];
+            for (int c = 0; c < 3; ++c) {
+                float n = 0.6f * value_noise(x / 32.0f, y / 32.0f, c * 7.0f, ts) +
+                          0.4f * value_noise(x / 9.0f, y / 9.0f, c * 7.0f + 3.0f, ts);
+                o[c] = std::fmin(0.9f, std::fmax(0.1f, 0.5f + 0.45f * n));
+            }
+            o[3] = 1.0f;
+        }
+    s.textures.clear();
+    s.textures.push_back(std::move(tex));
+    s.materials[2].base_color_tex_id = 0;
+    Image& env = s.env;
+    env.width = 512;
+    env.height = 256;
+    env.wrap_s = env.wrap_t = 10497;
+    env.linear = 1;
+    env.rgba.assign(512 * 256 * 4, 0.0f);
+    float top[3], bot[3];
+    for (int c = 0; c < 3; ++c) {
+        top[c] = rng.uniform(0.4f, 1.6f);
+        bot[c] = rng.uniform(0.2f, 1.0f);
+    }
+    for (int y = 0; y < 256; ++y)
+        for (int x = 0; x < 512; ++x) {
+            const float t = (y + 0.5f) / 256.0f;
+            const float wob = 1.0f + 0.15f * std::sin(6.2831853f * (x + 0.5f) / 512.0f);
+            float* o = &env.rgba[(static_cast<size_t>(y) * 512 + x) * 4];
+            for (int c = 0; c < 3; ++c) o[c] = (top[c] + (bot[c] - top[c]) * t) * wob;
+            o[3] = 1.0f;
+        }
+    s.camera.width = 256;
+    s.camera.height = 256;
+    s.settings.sample_per_frame = 64;
+    s.settings.target_samples = 64;
+    s.settings.max_depth = 6;
+    s.output_file = "diff_cornell.hdr";
+    return true;
+}
+
 bool generate_spheres(HostScene& s, uint64_t seed, int instances, int level, int env_w, int env_h) {
     init_scene_defaults(s);
     s.path = "<spheres>";
