@@ -133,6 +133,42 @@ def stage_times(sm, steps: int) -> dict:
     return st
 
 
+def backward_leg(steps: int = 3) -> dict:
+    """Forward and path-replay backward of the differentiable Cornell box
+    (256x256, 64 spp, depth 6): GPU ms of each pass and the rays the backward
+    pass traces (its primal replay plus the adjoint replay) per second."""
+    import torch
+
+    import pathtracingtutorial_amd as pt
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    sm = pt.SceneManager()
+    sm.generate(3, 3)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    dr = DiffRenderer(sm)
+    info = sm.info()
+    g = torch.ones((info.height, info.width, 4), dtype=torch.float32, device="cuda")
+    dr.forward_image(0)
+    dr.adjoint(0, g)
+    f_ms, b_ms = [], []
+    for i in range(steps):
+        dr.forward_image(i)
+        f_ms.append(dr.last_forward_ms)
+        dr.adjoint(i, g)
+        b_ms.append(dr.last_backward_ms)
+    sm.enableCounters(True)
+    sm.resetCounters()
+    dr.adjoint(0, g)
+    c = sm.counters()
+    sm.enableCounters(False)
+    rays = c["closest_rays"] + c["shadow_rays"]
+    fm, bm = float(np.mean(f_ms)), float(np.mean(b_ms))
+    return {"workload": "diff-cornell-36tri-256x256-64spp-depth6-pcg", "value": round(rays / (bm / 1e3) / 1e6, 2),
+            "unit": "Mrays/s", "backward_ms": round(bm, 3), "forward_ms": round(fm, 3),
+            "backward_over_forward": round(bm / fm, 3), "rays_per_backward": rays,
+            "params": "256x256 albedo texture, roughness + metallic of 8 materials, 512x256 env"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +178,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the wavefront autotuner (use the default config)")
+    ap.add_argument("--no-backward", action="store_true", help="skip the differentiable (backward) measurement")
     ap.add_argument("--tune-trials", type=int, default=16)
     ap.add_argument("--wavefront-config", default="",
                     help='JSON object of wavefront config fields to use instead of autotuning, '
@@ -263,6 +300,12 @@ def main(argv=None) -> int:
                         # same random numbers per path: the images are the same estimate
                         "image_channels_differing_1e-4": round(differ, 6),
                         "rays_per_step": prays // a.steps}
+    if not a.no_backward:
+        try:
+            out["backward"] = backward_leg()
+            log(f"backward: {out['backward']}")
+        except Exception as e:  # reported beside the forward number, never required for it
+            out["backward"] = {"value": None, "error": str(e)}
     if not a.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(sm)

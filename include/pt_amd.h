@@ -257,6 +257,9 @@ pt_status pt_scene_set_sampler(pt_scene* scene, int sampler);
 pt_status pt_scene_set_seed(pt_scene* scene, uint32_t seed);
 pt_status pt_scene_set_camera(pt_scene* scene, const pt_camera* cam);
 pt_status pt_scene_get_camera(const pt_scene* scene, pt_camera* cam);
+/* SceneManager::outputPath: the scene's "render.output" file (default
+ * "output.hdr"), NUL-terminated into buf of cap bytes. */
+pt_status pt_scene_output_path(const pt_scene* scene, char* buf, size_t cap);
 /* buildCameraFrame (camera/camera.h:36-51) of the scene's current camera. */
 pt_status pt_scene_camera_frame(const pt_scene* scene, pt_camera_frame* frame);
 
@@ -305,6 +308,14 @@ pt_status pt_film_reset(pt_film* film, pt_scene* scene);
 pt_status pt_film_download(const pt_film* film, int which, float* rgba_host);
 /* Device pointer of the acc (which 0) or new (which 1) surface, W*H*4 floats. */
 pt_status pt_film_device_ptr(const pt_film* film, int which, void** dev_ptr);
+/* ImageIO::write (io/image_io.cpp:79-144) for .hdr (Radiance RGBE, rows
+ * flipped so the file's first row is the image top) and .pfm: host RGBA
+ * floats, row 0 = bottom.  Other extensions return PT_ERR_UNSUPPORTED. */
+pt_status pt_image_write(const char* path, const float* rgba, int width, int height);
+/* ImageIO::load of a Radiance .hdr (stb_image's RGBE decode, io/image_io.cpp:
+ * 25-45): call with rgba NULL for the size, then with W*H*4 floats; row 0 =
+ * bottom (the same convention as films, so a saved film reads back equal). */
+pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* height);
 /* Copies the acc (0) or new (1) surface into device memory dst (W*H*4 floats). */
 pt_status pt_film_copy_device(const pt_film* film, int which, float* dst_device);
 

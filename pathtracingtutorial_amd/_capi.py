@@ -206,6 +206,9 @@ SIGNATURES = {
     "pt_film_download": (C.c_int, [_P, C.c_int, _FP]),
     "pt_film_device_ptr": (C.c_int, [_P, C.c_int, _PP]),
     "pt_film_copy_device": (C.c_int, [_P, C.c_int, C.c_void_p]),
+    "pt_image_write": (C.c_int, [C.c_char_p, _FP, C.c_int, C.c_int]),
+    "pt_scene_output_path": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "pt_image_read_hdr": (C.c_int, [C.c_char_p, _FP, _I32P, _I32P]),
     "pt_render_batch": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_all": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_scene_set_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),

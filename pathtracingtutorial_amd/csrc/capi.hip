@@ -834,6 +834,50 @@ pt_status pt_film_device_ptr(const pt_film* f, int which, void** p) {
 }
 
 
+pt_status pt_scene_output_path(const pt_scene* s, char* buf, size_t cap) {
+    return guarded([&] {
+        if (!s || !buf || cap == 0) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        const std::string& o = s->host.output_file.empty() ? std::string("output.hdr") : s->host.output_file;
+        if (o.size() + 1 > cap) throw Error(PT_ERR_INVALID_ARG, "buffer too small for the output path");
+        std::memcpy(buf, o.c_str(), o.size() + 1);
+    });
+}
+
+pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* height) {
+    return guarded([&] {
+        if (!path || !width || !height) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        ptamd::Image img;
+        std::string err;
+        if (!ptamd::load_hdr(path, img, &err)) throw Error(PT_ERR_IO, err.empty() ? std::string("cannot read ") + path : err);
+        if (rgba) {
+            if (*width != img.width || *height != img.height) throw Error(PT_ERR_INVALID_ARG, "buffer size mismatch");
+            // the loader returns the file's top row first; films keep row 0 at the bottom
+            const size_t row = static_cast<size_t>(img.width) * 4;
+            for (int y = 0; y < img.height; ++y)
+                std::memcpy(rgba + static_cast<size_t>(img.height - 1 - y) * row, img.rgba.data() + y * row,
+                            row * sizeof(float));
+        }
+        *width = img.width;
+        *height = img.height;
+    });
+}
+
+pt_status pt_image_write(const char* path, const float* rgba, int w, int h) {
+    return guarded([&] {
+        if (!path || !rgba || w <= 0 || h <= 0) throw Error(PT_ERR_INVALID_ARG, "bad image arguments");
+        std::string p(path), err;
+        auto ends = [&](const char* e) {
+            size_t n = std::strlen(e);
+            return p.size() >= n && p.compare(p.size() - n, n, e) == 0;
+        };
+        bool ok;
+        if (ends(".hdr")) ok = ptamd::write_hdr(p, w, h, rgba, true, &err);
+        else if (ends(".pfm")) ok = ptamd::write_pfm(p, w, h, rgba, &err);
+        else throw Error(PT_ERR_UNSUPPORTED, "image format not supported (use .hdr or .pfm): " + p);
+        if (!ok) throw Error(PT_ERR_IO, err.empty() ? "cannot write " + p : err);
+    });
+}
+
 pt_status pt_film_copy_device(const pt_film* f, int which, float* dst) {
     return guarded([&] {
         if (!f || !dst) throw Error(PT_ERR_INVALID_ARG, "null argument");

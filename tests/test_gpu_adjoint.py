@@ -132,27 +132,31 @@ def test_adjoint_matches_fp64_detached_fd():
 
 
 def test_autograd_function_and_descent():
+    # recover the base colours of the diffuse-looking materials from a
+    # 64-spp target: the parameter error must shrink under Adam
     from pathtracingtutorial_amd.diff import DiffRenderer
-    sm = _diff_cornell(w=32, h=32, spf=8)
+    sm = _diff_cornell(w=32, h=32, spf=16)
     dr = DiffRenderer(sm)
+    v = host_view(sm)
+    mats = (_capi.Material * v.num_materials).from_buffer_copy(C.string_at(v.materials, 256 * v.num_materials))
+    gt = torch.tensor([[m.base_color.x, m.base_color.y, m.base_color.z] for m in mats], device="cuda")
     target = torch.stack([dr.forward_image(f) for f in range(4)]).mean(0)
-    M = sm.info().num_materials
-    rough = torch.full((M,), 0.5, device="cuda", requires_grad=True)
-    tex = torch.full((256, 256, 4), 0.5, device="cuda")
-    tex[..., 3] = 1.0
-    tex.requires_grad_(True)
-    opt = torch.optim.Adam([rough, tex], lr=2e-2)
-    losses = []
-    for it in range(12):
+    bc = gt.clone()
+    learn = [0, 1, 3, 4, 7]
+    bc[learn] = 0.5
+    bc.requires_grad_(True)
+    opt = torch.optim.Adam([bc], lr=2e-2)
+    err0 = float((bc.detach()[learn] - gt[learn]).abs().mean())
+    for it in range(40):
         opt.zero_grad()
-        img = dr.render(100 + it, roughness=rough, texture=tex)
+        img = dr.render(100 + it, base_color=bc)
         loss = ((img[..., :3] - target[..., :3]) ** 2).mean()
         loss.backward()
-        assert rough.grad is not None and torch.isfinite(rough.grad).all()
-        assert tex.grad is not None and torch.isfinite(tex.grad).all() and tex.grad.abs().sum() > 0
+        assert bc.grad is not None and torch.isfinite(bc.grad).all()
+        with torch.no_grad():
+            bc.grad[[2, 5, 6]] = 0.0  # textured wall, light, tall box stay
         opt.step()
         with torch.no_grad():
-            rough.clamp_(0.05, 1.0)
-            tex.clamp_(0.0, 1.0)
-        losses.append(float(loss))
-    assert np.mean(losses[-4:]) < np.mean(losses[:4])
+            bc.clamp_(0.0, 1.0)
+    err1 = float((bc.detach()[learn] - gt[learn]).abs().mean())
+    assert err1 < 0.6 * err0, (err0, err1)
