@@ -198,6 +198,15 @@ def main(argv=None) -> int:
               "(one process renders on one GPU; frames would shard by sample index across GPUs)", file=sys.stderr)
         return 2
 
+    if not a.no_backward:
+        # torch ships its own HIP runtime: it must initialise the GPU before
+        # the library's runtime does, or it finds no device
+        try:
+            import torch
+            torch.cuda.init()
+        except Exception as e:
+            log(f"torch GPU runtime unavailable ({e}); skipping the backward leg")
+            a.no_backward = True
     import pathtracingtutorial_amd as pt
     sm = pt.SceneManager()
     t0 = time.perf_counter()

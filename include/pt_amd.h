@@ -197,6 +197,7 @@ typedef struct pt_wavefront_config {
     int32_t sort_by_material; /* counting-sort each bounce's shading queue by material */
     int32_t use_graph;        /* replay one captured hipGraph per chunk */
     int32_t time_stages;      /* bracket every launch with HIP events (no graph): pt_scene_wavefront_stats */
+    int32_t shade_waves;      /* shading kernel variant built for >= this many waves per SIMD: 0, 3 or 4 */
 } pt_wavefront_config;
 
 /* GPU time (ms) and launch count per stage of the last wavefront batch
@@ -329,7 +330,7 @@ pt_status pt_film_copy_device(const pt_film* film, int which, float* dst_device)
 pt_status pt_render_batch(pt_scene* scene, pt_film* film, int integrator, float* ms_out);
 /* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
 pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
-/* Wavefront configuration of a scene (defaults: 8, 128, 128, 0, 1, 1, 0). */
+/* Wavefront configuration of a scene (defaults: 8, 128, 128, 0, 1, 1, 0, 0). */
 pt_status pt_scene_set_wavefront_config(pt_scene* scene, const pt_wavefront_config* cfg);
 pt_status pt_scene_get_wavefront_config(const pt_scene* scene, pt_wavefront_config* cfg);
 /* Autotuner: times up to max_trials configurations (coordinate descent over

@@ -154,7 +154,7 @@ class Counters(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("chunk_spp", C.c_int32), ("extend_block", C.c_int32), ("shade_block", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
-                ("time_stages", C.c_int32)]
+                ("time_stages", C.c_int32), ("shade_waves", C.c_int32)]
 
 
 class WavefrontStats(C.Structure):

@@ -321,6 +321,8 @@ void check_wavefront_config(const pt_wavefront_config& c) {
     if (!block_ok(c.extend_block) || !block_ok(c.shade_block))
         throw Error(PT_ERR_INVALID_ARG, "block sizes must be 64, 128 or 256");
     if (c.waves_per_simd < 0 || c.waves_per_simd > 8) throw Error(PT_ERR_INVALID_ARG, "waves_per_simd must be in [0, 8]");
+    if (c.shade_waves != 0 && c.shade_waves != 3 && c.shade_waves != 4)
+        throw Error(PT_ERR_INVALID_ARG, "shade_waves must be 0, 3 or 4");
 }
 
 ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
@@ -332,6 +334,7 @@ ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
     d.sort_by_material = c.sort_by_material ? 1 : 0;
     d.use_graph = c.use_graph ? 1 : 0;
     d.time_stages = c.time_stages ? 1 : 0;
+    d.shade_waves = c.shade_waves;
     return d;
 }
 
@@ -344,6 +347,7 @@ pt_wavefront_config to_abi_config(const ptd::WavefrontConfig& d) {
     c.sort_by_material = d.sort_by_material;
     c.use_graph = d.use_graph;
     c.time_stages = d.time_stages;
+    c.shade_waves = d.shade_waves;
     return c;
 }
 
@@ -1125,6 +1129,7 @@ pt_status pt_scene_tune_wavefront(pt_scene* s, int max_trials, pt_wavefront_conf
             try_axis([](ptd::WavefrontConfig& c, int v) { c.shade_block = v; }, {64, 128, 256});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.sort_by_material = v; }, {0, 1});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.waves_per_simd = v; }, {0, 1, 2, 4});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.shade_waves = v; }, {0, 3, 4});
         } catch (...) {
             s->counting = counting;
             throw;

@@ -384,8 +384,11 @@ struct DeferNee {
     }
 };
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shade(DevScene s, WfState w, int in_slot, int use_sorted) {
+// WAVES > 0 asks the compiler for at least that many waves per SIMD (fewer
+// registers, possibly spills): the shading kernel's occupancy knob.
+template <int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8))) void k_wf_shade(
+    DevScene s, WfState w, int in_slot, int use_sorted) {
     const RenderParams& rp = *w.params;
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
@@ -730,7 +733,13 @@ void enqueue_chunk(Wavefront& wf, const DevScene& s, const RenderParams& p, cons
         }
         {
             Timed t(tm, WF_SHADE);
-            hipLaunchKernelGGL((k_wf_shade<SB>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, sort ? 1 : 0);
+            const int us = sort ? 1 : 0;
+            if (cfg.shade_waves == 4)
+                hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
+            else if (cfg.shade_waves == 3)
+                hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
+            else
+                hipLaunchKernelGGL((k_wf_shade<SB, 0>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
         }
         {
             Timed t(tm, WF_SHADOW);
@@ -754,7 +763,10 @@ template <bool COUNT, int EB, int SB>
 Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
     Grids g;
     g.extend = grid_for(reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB>), EB, wf.num_cus, cfg.waves_per_simd);
-    g.shade = grid_for(reinterpret_cast<const void*>(&k_wf_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
+    const void* shade = cfg.shade_waves == 4   ? reinterpret_cast<const void*>(&k_wf_shade<SB, 4>)
+                        : cfg.shade_waves == 3 ? reinterpret_cast<const void*>(&k_wf_shade<SB, 3>)
+                                               : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
+    g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
     return g;
 }
 
@@ -864,7 +876,7 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
     uint64_t key[10] = {scene_generation, static_cast<uint64_t>(p.width), static_cast<uint64_t>(p.height),
                         static_cast<uint64_t>(S), static_cast<uint64_t>(p.max_depth),
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
-                        static_cast<uint64_t>(cfg.sort_by_material * 100 + cfg.waves_per_simd),
+                        static_cast<uint64_t>(cfg.sort_by_material * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
                         static_cast<uint64_t>(s.num_lights)};
     StageTimer timer;

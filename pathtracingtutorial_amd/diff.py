@@ -9,6 +9,11 @@ d loss / d image and returns d loss / d parameter.  Tensors live on the GPU
 (torch's ROCm device 0, the device the library renders on) and are handed
 to the library as device pointers; no CPU copy of images or textures.
 
+torch carries its own copy of the HIP runtime, and it only finds the GPU if
+it initialises before the library's runtime: importing this module calls
+torch.cuda.init(), so import it before the first scene is built or
+uploaded.
+
 Parameters (any may be None, then it is left as it is and gets no gradient):
   base_color (M, 3), roughness (M,), metallic (M,): material factors
   texture    (H, W, 4): texels of one texture (`DiffRenderer.texture_id`)
@@ -29,6 +34,11 @@ from . import _capi
 from ._capi import check, lib
 
 ENV = 0xFFFFFFFF
+
+try:
+    torch.cuda.init()
+except RuntimeError:  # no GPU here: the module still imports (for inspection)
+    pass
 
 
 class DiffRenderer:

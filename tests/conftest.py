@@ -12,6 +12,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP kernels)")
 
 
+def pytest_sessionstart(session):
+    # torch ships its own HIP runtime, which finds the GPU only if it
+    # initialises before libptamd's: GPU sessions bring it up first
+    expr = session.config.getoption("markexpr") or ""
+    if "gpu" in expr and "not gpu" not in expr:
+        try:
+            import torch
+            torch.cuda.init()
+        except Exception:
+            pass
+
+
 def pytest_collection_modifyitems(config, items):
     # the session-scoped GPU fixtures are shared; keep file order stable
     pass
