@@ -235,7 +235,9 @@ pt_status pt_scene_create(const pt_host_scene_desc* desc, pt_scene** out);
  * icospheres (1,310,720 tris + ground, 2048x1024 sky+sun env, 1920x1080),
  * 2 = 8 small icospheres (tests), 3 = differentiable Cornell box (8
  * materials, 256x256 albedo texture on the back wall, 512x256 environment,
- * 256x256, 64 spp, depth 6; the seeded values are a ground truth).
+ * 256x256, 64 spp, depth 6; the seeded values are a ground truth), 4 = one
+ * job of the batched workload (icospheres of 1e3..5e5 triangles, 128^2..1024^2
+ * pixels, 4..64 spp, depth 12, 512x256 sky; all drawn from the seed).
  * No reference counterpart; seeds fix every value. */
 pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out);
 void pt_scene_destroy(pt_scene* scene);
@@ -328,6 +330,18 @@ pt_status pt_film_copy_device(const pt_film* film, int which, float* dst_device)
  * sample_per_frame samples per pixel into film, running-mean accumulates,
  * increments frame_count.  ms_out (may be NULL) gets the HIP-event time. */
 pt_status pt_render_batch(pt_scene* scene, pt_film* film, int integrator, float* ms_out);
+/* One batch of several scenes in one persistent wavefront schedule (no
+ * reference counterpart).  Each job renders its scene's sample_per_frame
+ * samples per pixel at the scene's frame_count into its own film (sizes may
+ * differ) and advances that frame_count, with exactly the image
+ * pt_render_batch(..., PT_INTEGRATOR_WAVEFRONT) would give.  The first job's
+ * scene lends its wavefront buffers, configuration, stream and counters.
+ * Films must be distinct; a scene may appear in several jobs. */
+typedef struct pt_job {
+    pt_scene* scene;
+    pt_film* film;
+} pt_job;
+pt_status pt_render_jobs(const pt_job* jobs, int n, float* ms_out);
 /* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
 pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
 /* Wavefront configuration of a scene (defaults: 8, 128, 128, 0, 1, 1, 0, 0). */

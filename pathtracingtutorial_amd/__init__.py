@@ -24,6 +24,7 @@ from ._capi import (BvhOptions, Camera, CameraFrame, Counters, PtError, SceneInf
 
 __all__ = [
     "SceneManager", "Renderer", "Integrator", "PathIntegrator", "WavefrontIntegrator", "PtError", "QualityMetrics",
+    "render_jobs",
     "compute_rmse", "compute_relmse", "probe_sampler", "trace_rays", "bvh_options", "REFERENCE_BVH", "lib",
 ]
 
@@ -224,6 +225,22 @@ class SceneManager:
             self._free()
         except Exception:
             pass
+
+
+def render_jobs(renderers) -> float:
+    """One batch of every renderer's scene in one persistent wavefront schedule
+    (pt_render_jobs); returns the GPU ms.  Each renderer's film and scene
+    frame count advance as with renderBatch()."""
+    arr = (_capi.Job * len(renderers))()
+    for i, r in enumerate(renderers):
+        arr[i].scene = r._scene.handle.value
+        arr[i].film = r._film.value
+    ms = C.c_float(0.0)
+    check(lib().pt_render_jobs(arr, len(renderers), C.byref(ms)))
+    for r in renderers:
+        r._last_ms = ms.value / len(renderers)
+        r._batches += 1
+    return float(ms.value)
 
 
 @dataclass

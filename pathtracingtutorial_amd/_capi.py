@@ -164,6 +164,10 @@ class WavefrontStats(C.Structure):
 WAVEFRONT_STAGES = ("generate", "extend", "sort", "shade", "shadow", "control")
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("film", C.c_void_p)]
+
+
 class AdjointGrads(C.Structure):
     _fields_ = [("base_color", C.c_void_p), ("roughness", C.c_void_p), ("metallic", C.c_void_p),
                 ("textures", C.c_void_p), ("env", C.c_void_p)]
@@ -211,6 +215,7 @@ SIGNATURES = {
     "pt_image_read_hdr": (C.c_int, [C.c_char_p, _FP, _I32P, _I32P]),
     "pt_render_batch": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_all": (C.c_int, [_P, _P, C.c_int, _FP]),
+    "pt_render_jobs": (C.c_int, [C.POINTER(Job), C.c_int, _FP]),
     "pt_scene_set_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),
     "pt_scene_get_wavefront_config": (C.c_int, [_P, C.POINTER(WavefrontConfig)]),
     "pt_scene_tune_wavefront": (C.c_int, [_P, C.c_int, C.POINTER(WavefrontConfig), _FP]),

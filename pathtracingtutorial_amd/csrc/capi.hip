@@ -300,19 +300,33 @@ void check_renderable(const pt_scene* s, const pt_film* f) {
         throw Error(PT_ERR_INVALID_ARG, "film size does not match the camera resolution");
 }
 
-float render_wavefront(pt_scene* s, pt_film* f, const ptd::WavefrontConfig& cfg, uint32_t frame_count) {
-    if (!s->wf) s->wf = ptd::wavefront_create();
-    ptd::DevScene v = make_view(s);
-    ptd::RenderParams p = make_params(s);
-    p.frame_count = frame_count;
+ptd::WavefrontJob make_job(const pt_scene* s, const pt_film* f, uint32_t frame_count) {
+    ptd::WavefrontJob j;
+    j.scene = make_view(s);
+    j.params = make_params(s);
+    j.params.frame_count = frame_count;
+    j.film_new = f->film_new.p;
+    j.film_acc = f->film_acc.p;
+    return j;
+}
+
+// Renders the jobs with the wavefront engine of `owner` (its buffers, stream,
+// counters and graph cache).
+float render_wavefront_jobs(pt_scene* owner, const std::vector<ptd::WavefrontJob>& jobs,
+                            const ptd::WavefrontConfig& cfg) {
+    if (!owner->wf) owner->wf = ptd::wavefront_create();
     float ms = 0.0f;
     std::string err;
-    hipError_t e = ptd::wavefront_render(s->wf, v, s->generation, p, cfg, f->film_new.p, f->film_acc.p,
-                                         s->counters.p, s->counting, s->stream, &ms, &err);
+    hipError_t e =
+        ptd::wavefront_render(owner->wf, jobs, cfg, owner->counters.p, owner->counting, owner->stream, &ms, &err);
     if (e == hipErrorOutOfMemory) throw Error(PT_ERR_OUT_OF_MEMORY, err.empty() ? "wavefront: out of memory" : err);
     if (e != hipSuccess && !err.empty()) throw Error(e == hipErrorInvalidValue ? PT_ERR_INVALID_ARG : PT_ERR_DEVICE, err);
     hip_check(e, "wavefront render");
     return ms;
+}
+
+float render_wavefront(pt_scene* s, pt_film* f, const ptd::WavefrontConfig& cfg, uint32_t frame_count) {
+    return render_wavefront_jobs(s, std::vector<ptd::WavefrontJob>{make_job(s, f, frame_count)}, cfg);
 }
 
 void check_wavefront_config(const pt_wavefront_config& c) {
@@ -456,6 +470,7 @@ pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out) {
         else if (which == 1) ptamd::generate_spheres(sc->host, seed, 64, 5, 2048, 1024);
         else if (which == 2) ptamd::generate_spheres(sc->host, seed, 8, 3, 256, 128);  // small variant for tests
         else if (which == 3) ptamd::generate_diff_cornell(sc->host, seed);
+        else if (which == 4) ptamd::generate_batch_job(sc->host, seed);
         else throw Error(PT_ERR_INVALID_ARG, "unknown workload id " + std::to_string(which));
         validate_scene(sc->host);
         *out = sc.release();
@@ -918,6 +933,25 @@ pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out
         }
         if (ms_out) *ms_out = ms;
         s->frame_count++;
+    });
+}
+
+pt_status pt_render_jobs(const pt_job* jobs, int n, float* ms_out) {
+    return guarded([&] {
+        if (!jobs || n <= 0) throw Error(PT_ERR_INVALID_ARG, "no jobs");
+        std::vector<ptd::WavefrontJob> js;
+        js.reserve(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            check_renderable(jobs[i].scene, jobs[i].film);
+            for (int k = 0; k < i; ++k)
+                if (jobs[k].film == jobs[i].film) throw Error(PT_ERR_INVALID_ARG, "two jobs share a film");
+            js.push_back(make_job(jobs[i].scene, jobs[i].film, jobs[i].scene->frame_count));
+        }
+        pt_scene* owner = jobs[0].scene;
+        float ms = render_wavefront_jobs(owner, js, owner->wf_cfg);
+        // a scene that appears in several jobs advances once per job
+        for (int i = 0; i < n; ++i) jobs[i].scene->frame_count++;
+        if (ms_out) *ms_out = ms;
     });
 }
 

@@ -5,13 +5,14 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "dev_scene.h"
 
 namespace ptd {
 
 struct WavefrontConfig {
-    int chunk_spp = 8;          // samples per pixel held in flight per chunk
+    int chunk_spp = 8;          // paths in flight = chunk_spp x (pixels of all jobs)
     int extend_block = 128;     // threads per block of the traversal kernels
     int shade_block = 128;      // threads per block of the shading kernel
     int waves_per_simd = 0;     // persistent grid size: 0 = from the occupancy calculator
@@ -32,18 +33,25 @@ enum WavefrontStage {
     WF_STAGES = 6
 };
 
+// One render job: a scene, the parameters of its batch and its films.
+struct WavefrontJob {
+    DevScene scene;
+    RenderParams params;  // params.spf samples per pixel of frame params.frame_count
+    float4* film_new;
+    float4* film_acc;
+};
+
 class Wavefront;
 
-// Owns path-state buffers, queues and the captured graph for one scene/film
-// pair; rebuilt when sizes or settings change.
+// Owns path-state buffers, queues and the captured graph; rebuilt when sizes
+// or settings change.
 Wavefront* wavefront_create();
 void wavefront_destroy(Wavefront* w);
-// Renders one batch (params.spf samples per pixel) into film_new/film_acc
-// exactly as the megakernel does (batch mean + running mean).  ms = GPU time
-// of the batch from HIP events on `stream`.
-hipError_t wavefront_render(Wavefront* w, const DevScene& s, uint64_t scene_generation, const RenderParams& p,
-                            const WavefrontConfig& cfg, float4* film_new, float4* film_acc, DevCounters* ctr,
-                            bool count, hipStream_t stream, float* ms, std::string* err);
+// Renders one batch of every job (all jobs' paths share the queues of one
+// persistent schedule) into their films exactly as the megakernel does
+// (batch mean + running mean).  ms = GPU time from HIP events on `stream`.
+hipError_t wavefront_render(Wavefront* w, const std::vector<WavefrontJob>& jobs, const WavefrontConfig& cfg,
+                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err);
 // GPU milliseconds and launch counts per stage of the last batch rendered
 // with time_stages set (zeros otherwise).
 void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches);

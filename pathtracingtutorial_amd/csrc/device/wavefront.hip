@@ -3,23 +3,27 @@
 // The estimator is the megakernel's (dev_integrator.h: same bounce() code,
 // same random numbers per path); only the schedule changes:
 //
-//  * A frame batch of spf samples per pixel is cut into chunks of S samples
-//    for every pixel; path p of a chunk is (pixel p / S, sample p % S).
+//  * A batch renders one or many jobs (scene + camera + resolution + spp +
+//    film).  Job j owns paths [O_j, O_j + W_j*H_j*spf_j) of a global path
+//    index, path O_j + pixel*spf_j + sample; the index space is processed in
+//    chunks of C paths (C = chunk_spp x all jobs' pixels), so scenes of any
+//    size and resolution share the same persistent kernels and queues.
 //  * Path state lives in HBM as SoA float4 arrays (ray origin, direction +
-//    last pdf, throughput + depth/flags, radiance, sampler state, hit).
+//    last pdf, throughput + depth/flags, radiance, sampler state + job, hit).
 //  * Per bounce: a persistent closest-hit kernel over the ray queue (lean:
 //    traversal only, LDS stack), an optional counting sort of the queue by
 //    material, a persistent shading kernel that appends next rays and NEE
 //    shadow rays to queues, and persistent any-hit kernels over the area,
 //    environment and punctual shadow queues.  Area and environment shadow
 //    rays go to separate queues processed by separate launches, so each path
-//    gains at most one contribution per launch and no atomics are needed:
-//    the image is bitwise reproducible.
+//    gains at most one contribution per launch and no atomics are needed.
 //  * Work is grabbed per wave (64 items, one atomic) from 8 queue segments,
 //    one per XCD, so each XCD's L2 sees contiguous (coherent) rays; a wave
 //    steals from the other segments once its own is empty.
-//  * The bounce loop of a chunk is captured once in a hipGraph (parameters
-//    and the chunk index live in device memory) and replayed per chunk.
+//  * A pixel's samples are summed in sample order across chunks, so the image
+//    is bitwise the megakernel's sum (up to FMA contraction differences).
+//  * The bounce loop of a chunk is captured once in a hipGraph (job table and
+//    chunk index live in device memory) and replayed per chunk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,31 +37,35 @@ namespace ptd {
 
 namespace {
 
-// Queues are cut into kSeg segments, one per XCD (blocks are dispatched
-// round-robin over the XCDs, so block b runs on XCD b % 8).  Segment k of a
-// chunk's paths is the contiguous range [base(k), base(k+1)) of path ids
-// (a band of image rows), and everything a path spawns (next ray, shadow
-// rays) is appended to the same segment's region of the output queue: across
-// bounces each XCD keeps working on its own band, in roughly the original
-// order, and its L2 sees coherent rays.
 constexpr int kSeg = 8;
 constexpr int kSortParts = 32;  // blocks per segment of the material sort
 constexpr uint32_t kMaxSortBins = 8192;
 // control words
-constexpr int C_QCNT = 0;   // [2][8] ray queue segment counts
-constexpr int C_SACNT = 16; // [8] area shadow queue segment counts
-constexpr int C_SECNT = 24; // [8] environment shadow queue segment counts
-constexpr int C_SP = 32;    // punctual shadow queue count (one flat queue)
-constexpr int C_CHUNK = 33; // chunk index within the batch
+constexpr int C_QCNT = 0;    // [2][8] ray queue segment counts
+constexpr int C_SACNT = 16;  // [8] area shadow queue segment counts
+constexpr int C_SECNT = 24;  // [8] environment shadow queue segment counts
+constexpr int C_SP = 32;     // punctual shadow queue count (one flat queue)
+constexpr int C_CHUNK = 33;  // chunk index within the batch
 constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72;
 constexpr int C_WORDS = 80;
+
+// A job as the kernels see it (device memory, one entry per job).
+struct JobDesc {
+    DevScene scene;
+    RenderParams params;
+    float4* film_new;
+    float4* film_acc;
+    unsigned long long path_offset;  // first global path
+    uint32_t pixel_offset;           // first global pixel (per-pixel sums)
+    uint32_t npix;
+};
 
 struct WfState {
     float4* o;    // xyz origin
     float4* d;    // xyz direction, w = last bsdf pdf
     float4* tp;   // xyz throughput, w = bits: depth | last_delta << 31
     float4* lo;   // xyz radiance
-    uint4* rng;   // x pixel, y Sobol index, z Sobol dim or PCG state, w pixel hash
+    uint4* rng;   // x job, y Sobol index, z Sobol dim or PCG state, w pixel hash
     float4* hit;  // t, u, v, triangle bits
     uint32_t* q[2];
     uint32_t* sorted;
@@ -65,18 +73,51 @@ struct WfState {
     float4 *se_o, *se_d, *se_c;  // environment shadow queue
     float4 *sp_o, *sp_d, *sp_c;  // punctual shadow queue
     uint32_t* blockhist;         // material sort: [segment][part][bin] counts, then offsets
-    float4* sum;                 // per pixel sum over the batch
+    float4* sum;                 // per global pixel sum over the batch
     uint32_t* ctrl;
-    const RenderParams* params;
+    const JobDesc* jobs;
+    uint32_t njobs;
     uint32_t capacity, sp_capacity;
-    uint32_t chunk_spp;
-    uint32_t npaths;  // paths per chunk (pixels x chunk_spp)
+    uint32_t chunk_paths;              // C
+    unsigned long long total_paths;    // all jobs' paths of the batch
+    uint32_t total_pixels;
 };
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63); }
 
-__device__ __forceinline__ uint32_t seg_base(uint32_t npaths, int k) {
-    return static_cast<uint32_t>((static_cast<unsigned long long>(npaths) * static_cast<unsigned>(k)) / kSeg);
+__device__ __forceinline__ uint32_t seg_base(uint32_t n, int k) {
+    return static_cast<uint32_t>((static_cast<unsigned long long>(n) * static_cast<unsigned>(k)) / kSeg);
+}
+
+// Paths of the current chunk and its first global path.
+__device__ __forceinline__ uint32_t chunk_n(const WfState& w, unsigned long long& c0) {
+    c0 = static_cast<unsigned long long>(w.ctrl[C_CHUNK]) * w.chunk_paths;
+    const unsigned long long left = w.total_paths > c0 ? w.total_paths - c0 : 0ull;
+    return static_cast<uint32_t>(left < w.chunk_paths ? left : w.chunk_paths);
+}
+
+// Job of a global path / global pixel (the job table is sorted by both).
+__device__ __forceinline__ uint32_t job_of_path(const WfState& w, unsigned long long g) {
+    uint32_t lo = 0, hi = w.njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (w.jobs[mid].path_offset <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t job_of_pixel(const WfState& w, uint32_t gp) {
+    uint32_t lo = 0, hi = w.njobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (w.jobs[mid].pixel_offset <= gp) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t path_job(const WfState& w, uint32_t p) {
+    return reinterpret_cast<const uint32_t*>(w.rng)[4 * static_cast<size_t>(p)];
 }
 
 // Wave-aggregated append: every lane of the wave must call it.
@@ -95,10 +136,10 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 // Grabs the next 64 items of a segmented queue for this wave: its own
 // segment first, then the others.  Segment k holds [base(k), base(k) +
 // counts[k]).  Returns false when every segment is exhausted.
-__device__ __forceinline__ bool grab_seg(uint32_t* cursors, const uint32_t* counts, uint32_t npaths, int& seg,
+__device__ __forceinline__ bool grab_seg(uint32_t* cursors, const uint32_t* counts, uint32_t n, int& seg,
                                          int& tried, uint32_t& i, uint32_t& end) {
     while (tried < kSeg) {
-        uint32_t s0 = seg_base(npaths, seg);
+        uint32_t s0 = seg_base(n, seg);
         uint32_t s1 = s0 + counts[seg];
         uint32_t off = 0;
         if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
@@ -198,31 +239,36 @@ __global__ void k_wf_batch_begin(uint32_t* ctrl) {
 // Ray queue 0 of a chunk: every path, segment k = [base(k), base(k+1)).
 __global__ void k_wf_chunk_begin(WfState w) {
     const int k = static_cast<int>(threadIdx.x);
-    if (k < kSeg)
-        w.ctrl[C_QCNT + k] =
-            w.params->max_depth > 0 ? seg_base(w.npaths, k + 1) - seg_base(w.npaths, k) : 0u;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    if (k < kSeg) w.ctrl[C_QCNT + k] = seg_base(n, k + 1) - seg_base(n, k);
 }
 
 __global__ void k_wf_chunk_end(uint32_t* ctrl) {
     if (threadIdx.x == 0) ctrl[C_CHUNK] += 1;
 }
 
-// Camera rays of chunk ctrl[C_CHUNK]: path p = (pixel p / S, sample p % S).
+// Camera rays of the chunk: global path g = O_j + pixel * spf_j + sample.
 __global__ void k_wf_generate(WfState w) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= w.npaths) return;
-    const RenderParams& rp = *w.params;
-    const uint32_t S = w.chunk_spp;
-    uint32_t pixel = p / S, j = p % S;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long g = c0 + p;
+    const uint32_t j = job_of_path(w, g);
+    const JobDesc& job = w.jobs[j];
+    const RenderParams& rp = job.params;
+    const unsigned long long local = g - job.path_offset;
+    const uint32_t spf = static_cast<uint32_t>(rp.spf);
+    const uint32_t pixel = static_cast<uint32_t>(local / spf), sample = static_cast<uint32_t>(local % spf);
     int x = static_cast<int>(pixel % static_cast<uint32_t>(rp.width));
     int y = static_cast<int>(pixel / static_cast<uint32_t>(rp.width));
-    uint32_t sample = w.ctrl[C_CHUNK] * S + j;
     Sampler rng{};
-    if (rp.sampler == 1) rng.init_sobol(x, y, rp.frame_count * static_cast<uint32_t>(rp.spf) + sample, rp.seed);
+    if (rp.sampler == 1) rng.init_sobol(x, y, rp.frame_count * spf + sample, rp.seed);
     else rng.init_pcg(x, y, rp.frame_count, sample, rp.seed);
     PathState st;
     begin_path(rp, x, y, rng, st);
-    uint4 r = make_uint4(pixel, rng.index, 0u, rng.pix_hash);
+    uint4 r = make_uint4(j, rng.index, 0u, rng.pix_hash);
     r = sampler_state(rng, r);
     store_path(w, p, st, r);
     w.q[0][p] = p;
@@ -241,8 +287,10 @@ __global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
 }
 
 template <bool COUNT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int in_slot, DevCounters* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = w.q[in_slot];
     int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
@@ -250,9 +298,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int 
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab_seg(w.ctrl + C_CUR_EXT, counts, w.npaths, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + C_CUR_EXT, counts, n, seg, tried, i, end)) {
         if (i < end) {
             uint32_t p = q[i];
+            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             float4 o = w.o[p], d = w.d[p];
             Hit h = trace_closest<COUNT>(s, f3(o), f3(d), stack + threadIdx.x, BLOCK, cnt, ovf);
             w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
@@ -265,29 +314,34 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(DevScene s, WfState w, int 
 // ---- material sort: a counting sort within each segment -------------------
 // Block j sorts part j / kSeg of segment j % kSeg: per-block histograms in
 // LDS, one scan per segment over (bin, part), then a scatter that ranks the
-// block's items with LDS atomics.  Misses go to the last bin.
+// block's items with LDS atomics.  Misses go to the last bin; the key is the
+// material index (of whichever job the path belongs to).
 
-__device__ __forceinline__ uint32_t material_key(const DevScene& s, const WfState& w, uint32_t p, uint32_t nbins) {
+__device__ __forceinline__ uint32_t material_key(const WfState& w, uint32_t p, uint32_t nbins) {
     uint32_t tri = __float_as_uint(w.hit[p].w);
-    return tri == 0xFFFFFFFFu ? nbins - 1 : min(s.faces[tri].w, nbins - 2);
+    if (tri == 0xFFFFFFFFu) return nbins - 1;
+    const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
+    return min(s.faces[tri].w, nbins - 2);
 }
 
-__device__ __forceinline__ void sort_part_range(const WfState& w, const uint32_t* counts, int seg, int part,
-                                                uint32_t& b0, uint32_t& b1) {
-    const uint32_t base = seg_base(w.npaths, seg), n = counts[seg];
-    b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(n) * part) / kSortParts);
-    b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(n) * (part + 1)) / kSortParts);
+__device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* counts, int seg, int part, uint32_t& b0,
+                                                uint32_t& b1) {
+    const uint32_t base = seg_base(n, seg), cnt = counts[seg];
+    b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * part) / kSortParts);
+    b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * (part + 1)) / kSortParts);
 }
 
-__global__ __launch_bounds__(256) void k_wf_sort_count(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+__global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, int in_slot, uint32_t nbins) {
     extern __shared__ uint32_t hist[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
     __syncthreads();
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(w, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
+    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
     const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[material_key(s, w, q[i], nbins)], 1u);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[material_key(w, q[i], nbins)], 1u);
     __syncthreads();
     uint32_t* out = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
@@ -298,6 +352,8 @@ __global__ __launch_bounds__(256) void k_wf_sort_count(DevScene s, WfState w, in
 __global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins) {
     __shared__ uint32_t part_sum[256];
     const int seg = static_cast<int>(blockIdx.x);
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     uint32_t* h = w.blockhist + static_cast<size_t>(seg) * kSortParts * nbins;
     const uint32_t E = nbins * kSortParts;  // entry e = bin * parts + part
     const uint32_t t = threadIdx.x;
@@ -309,7 +365,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins)
     part_sum[t] = sum;
     __syncthreads();
     if (t == 0) {
-        uint32_t acc = seg_base(w.npaths, seg);
+        uint32_t acc = seg_base(n, seg);
         for (int k = 0; k < 256; ++k) {
             uint32_t v = part_sum[k];
             part_sum[k] = acc;
@@ -325,18 +381,20 @@ __global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins)
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_sort_scatter(DevScene s, WfState w, int in_slot, uint32_t nbins) {
+__global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot, uint32_t nbins) {
     extern __shared__ uint32_t off[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
     const uint32_t* src = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
     __syncthreads();
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(w, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
+    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
     const uint32_t* q = w.q[in_slot];
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
         uint32_t p = q[i];
-        w.sorted[atomicAdd(&off[material_key(s, w, p, nbins)], 1u)] = p;
+        w.sorted[atomicAdd(&off[material_key(w, p, nbins)], 1u)] = p;
     }
 }
 
@@ -388,8 +446,9 @@ struct DeferNee {
 // registers, possibly spills): the shading kernel's occupancy knob.
 template <int BLOCK, int WAVES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8))) void k_wf_shade(
-    DevScene s, WfState w, int in_slot, int use_sorted) {
-    const RenderParams& rp = *w.params;
+    WfState w, int in_slot, int use_sorted) {
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
     const int out_slot = in_slot ^ 1;
@@ -397,7 +456,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
     int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
-    while (grab_seg(w.ctrl + C_CUR_SHADE, counts, w.npaths, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + C_CUR_SHADE, counts, n, seg, tried, i, end)) {
         const bool active = i < end;
         uint32_t p = 0;
         PathState st;
@@ -413,21 +472,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
             p = q[i];
             nee.path = p;
             load_path(w, p, st, r);
+            const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
+            const RenderParams& rp = job.params;
             float4 hv = w.hit[p];
             Hit h;
             h.t = hv.x;
             h.u = hv.y;
             h.v = hv.z;
             h.tri = __float_as_uint(hv.w);
-            Sampler rng = sampler_of(rp.sampler, r);
-            cont = bounce(s, rp, h, st, rng, nee);
-            r = sampler_state(rng, r);
-            if (cont) st.depth += 1;
-            cont = cont && st.depth < rp.max_depth;
-            store_path(w, p, st, r);
+            if (st.depth < rp.max_depth) {  // a job of smaller depth than the batch's stops here
+                Sampler rng = sampler_of(rp.sampler, r);
+                cont = bounce(job.scene, rp, h, st, rng, nee);
+                r = sampler_state(rng, r);
+                if (cont) st.depth += 1;
+                cont = cont && st.depth < rp.max_depth;
+                store_path(w, p, st, r);
+            }
         }
         // everything this wave spawns goes to the segment its items came from
-        const uint32_t base = seg_base(w.npaths, seg);
+        const uint32_t base = seg_base(n, seg);
         const bool pa = active && nee.va, pe = active && nee.ve;
         uint32_t ka = base + wave_append(&w.ctrl[C_SACNT + seg], pa);
         uint32_t ke = base + wave_append(&w.ctrl[C_SECNT + seg], pe);
@@ -460,22 +523,26 @@ __device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, flo
 // Any-hit over a segmented shadow queue (area or environment): each path has
 // at most one entry, so the contribution is added without atomics.
 template <bool COUNT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(DevScene s, WfState w, int count_slot, int cur_slot,
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, int cur_slot,
                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                      const float4* __restrict__ qc, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
     int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab_seg(w.ctrl + cur_slot, counts, w.npaths, seg, tried, i, end)) {
+    while (grab_seg(w.ctrl + cur_slot, counts, n, seg, tried, i, end)) {
         if (i < end) {
             float4 o = qo[i], d = qd[i];
+            const uint32_t p = __float_as_uint(d.w);
+            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
             if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
-                add_unoccluded(w, __float_as_uint(d.w), qc[i], false);
+                add_unoccluded(w, p, qc[i], false);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -484,7 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(DevScene s, WfState w, int 
 // Any-hit over the flat punctual-light queue (several entries per path:
 // contributions are added with float atomics).
 template <bool COUNT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(DevScene s, WfState w, DevCounters* ctr) {
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
     int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
@@ -495,43 +562,53 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(DevScene s, WfStat
     while (grab_flat(w.ctrl + C_CUR_SP, n, seg, tried, i, end)) {
         if (i < end) {
             float4 o = w.sp_o[i], d = w.sp_d[i];
+            const uint32_t p = __float_as_uint(d.w);
+            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
             if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
-                add_unoccluded(w, __float_as_uint(d.w), w.sp_c[i], true);
+                add_unoccluded(w, p, w.sp_c[i], true);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
 }
 
-// Per pixel: add the clamped radiance of this chunk's samples, in order.
-__global__ void k_wf_finalize(WfState w, uint32_t npixels) {
-    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= npixels) return;
-    const uint32_t S = w.chunk_spp;
-    // samples are added one at a time in sample order, as the megakernel does
-    F3 sum = w.ctrl[C_CHUNK] == 0 ? f3(0.0f) : f3(w.sum[px]);
-    for (uint32_t j = 0; j < S; ++j) sum += clamp_radiance(f3(w.lo[px * S + j]));
-    w.sum[px] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+// Per global pixel touched by the chunk: add its samples of this chunk to the
+// pixel's sum, one at a time in sample order (as the megakernel does).
+__global__ void k_wf_finalize(WfState w, uint32_t gp0, uint32_t gp1) {
+    const uint32_t gp = gp0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= gp1 || gp >= w.total_pixels) return;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const JobDesc& job = w.jobs[job_of_pixel(w, gp)];
+    const uint32_t spf = static_cast<uint32_t>(job.params.spf);
+    const unsigned long long first = job.path_offset + static_cast<unsigned long long>(gp - job.pixel_offset) * spf;
+    const unsigned long long a = first > c0 ? first : c0;
+    const unsigned long long b = (first + spf) < (c0 + n) ? (first + spf) : (c0 + n);
+    if (a >= b) return;
+    F3 sum = first >= c0 ? f3(0.0f) : f3(w.sum[gp]);
+    for (unsigned long long g = a; g < b; ++g) sum += clamp_radiance(f3(w.lo[g - c0]));
+    w.sum[gp] = make_float4(sum.x, sum.y, sum.z, 0.0f);
 }
 
-// Batch mean to the films (path.cu:228-241).
-__global__ void k_wf_film(WfState w, uint32_t npixels, float4* __restrict__ film_new, float4* __restrict__ film_acc,
-                          DevCounters* ctr, uint32_t count_paths) {
-    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= npixels) return;
-    const RenderParams& rp = *w.params;
-    float4 sm = w.sum[px];
+// Batch mean to each job's films (path.cu:228-241).
+__global__ void k_wf_film(WfState w, DevCounters* ctr, uint32_t count_paths) {
+    const uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= w.total_pixels) return;
+    const JobDesc& job = w.jobs[job_of_pixel(w, gp)];
+    const RenderParams& rp = job.params;
+    const uint32_t px = gp - job.pixel_offset;
+    float4 sm = w.sum[gp];
     F3 acc = f3(sm) / static_cast<float>(rp.spf);
     float4 nw = make_float4(acc.x, acc.y, acc.z, 1.0f);
-    film_new[px] = nw;
-    float4 a = film_acc[px];
+    job.film_new[px] = nw;
+    float4 a = job.film_acc[px];
     float inv = 1.0f / static_cast<float>(rp.frame_count + 1u);
     a.x = a.x * (1.0f - inv) + nw.x * inv;
     a.y = a.y * (1.0f - inv) + nw.y * inv;
     a.z = a.z * (1.0f - inv) + nw.z * inv;
     a.w = a.w * (1.0f - inv) + nw.w * inv;
-    film_acc[px] = a;
-    if (count_paths && px == 0) atomicAdd(&ctr->paths, static_cast<unsigned long long>(npixels) * rp.spf);
+    job.film_acc[px] = a;
+    if (count_paths && gp == 0) atomicAdd(&ctr->paths, static_cast<unsigned long long>(w.total_paths));
 }
 
 template <class T>
@@ -603,13 +680,14 @@ public:
     WfState st{};
     float stage_ms[WF_STAGES] = {};
     uint32_t stage_launches[WF_STAGES] = {};
-    RenderParams* d_params = nullptr;
+    JobDesc* d_jobs = nullptr;
+    uint32_t job_capacity = 0;
     uint32_t capacity = 0, sp_capacity = 0, pixels = 0, nbins = 0;
     int num_cus = 0;
     // graph cache
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    uint64_t key[10] = {};
+    uint64_t key[12] = {};
 
     ~Wavefront() { release(); }
 
@@ -623,15 +701,15 @@ public:
         release_graph();
         void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
-                      d_params};
+                      d_jobs};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         st = WfState{};
-        d_params = nullptr;
-        capacity = sp_capacity = pixels = nbins = 0;
+        d_jobs = nullptr;
+        job_capacity = capacity = sp_capacity = pixels = nbins = 0;
     }
 
-    bool ensure(uint32_t npaths, uint32_t npix, uint32_t nlights, uint32_t nmat) {
+    bool ensure(uint32_t npaths, uint32_t npix, uint32_t nlights, uint32_t nmat, uint32_t njobs) {
         bool ok = true;
         if (npaths > capacity) {
             ok = ok && dalloc(st.o, npaths) && dalloc(st.d, npaths) && dalloc(st.tp, npaths) &&
@@ -664,14 +742,18 @@ public:
             nbins = nb;
             release_graph();
         }
+        if (njobs > job_capacity) {
+            ok = ok && dalloc(d_jobs, njobs);
+            job_capacity = njobs;
+            release_graph();
+        }
         if (!st.ctrl) ok = ok && dalloc(st.ctrl, C_WORDS);
-        if (!d_params) ok = ok && dalloc(d_params, 1);
         if (!ok) {
             release();
             (void)hipGetLastError();  // the failed allocation is reported by the caller, not by the next launch
             return false;
         }
-        st.params = d_params;
+        st.jobs = d_jobs;
         st.capacity = capacity;
         st.sp_capacity = sp_capacity;
         return ok;
@@ -693,6 +775,13 @@ int grid_for(const void* kernel, int block, int num_cus, int waves_per_simd) {
     return num_cus * per_cu;
 }
 
+// What a chunk's launches depend on besides device memory.
+struct ChunkShape {
+    int max_depth;
+    uint32_t nbins;
+    bool sort, lights;
+};
+
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
@@ -700,21 +789,19 @@ struct Grids {
 };
 
 template <bool COUNT, int EB, int SB>
-void enqueue_chunk(Wavefront& wf, const DevScene& s, const RenderParams& p, const WavefrontConfig& cfg,
-                   const Grids& g, uint32_t npix, DevCounters* ctr, hipStream_t stream, StageTimer* tm) {
+void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
+                   DevCounters* ctr, hipStream_t stream, StageTimer* tm, uint32_t gp0, uint32_t gp1) {
     const WfState& w = wf.st;
-    const uint32_t nb = s.num_materials + 1;  // materials + miss bin
-    const bool sort = cfg.sort_by_material && nb <= kMaxSortBins;
-    const size_t sort_lds = sizeof(uint32_t) * nb;
+    const size_t sort_lds = sizeof(uint32_t) * cs.nbins;
     {
         Timed t(tm, WF_CONTROL);
         hipLaunchKernelGGL(k_wf_chunk_begin, dim3(1), dim3(64), 0, stream, w);
     }
     {
         Timed t(tm, WF_GENERATE);
-        hipLaunchKernelGGL(k_wf_generate, dim3((w.npaths + 255) / 256), dim3(256), 0, stream, w);
+        hipLaunchKernelGGL(k_wf_generate, dim3((w.chunk_paths + 255) / 256), dim3(256), 0, stream, w);
     }
-    for (int d = 0; d < p.max_depth; ++d) {
+    for (int d = 0; d < cs.max_depth; ++d) {
         const int in = d & 1;
         {
             Timed t(tm, WF_CONTROL);
@@ -722,39 +809,41 @@ void enqueue_chunk(Wavefront& wf, const DevScene& s, const RenderParams& p, cons
         }
         {
             Timed t(tm, WF_EXTEND);
-            hipLaunchKernelGGL((k_wf_extend<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, in, ctr);
+            hipLaunchKernelGGL((k_wf_extend<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
         }
-        if (sort) {
+        if (cs.sort) {
             Timed t(tm, WF_SORT);
-            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, s, w, in, nb);
-            hipLaunchKernelGGL(k_wf_sort_scan, dim3(kSeg), dim3(256), 0, stream, w, nb);
-            hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, s, w, in,
-                               nb);
+            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, in,
+                               cs.nbins);
+            hipLaunchKernelGGL(k_wf_sort_scan, dim3(kSeg), dim3(256), 0, stream, w, cs.nbins);
+            hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, in,
+                               cs.nbins);
         }
         {
             Timed t(tm, WF_SHADE);
-            const int us = sort ? 1 : 0;
+            const int us = cs.sort ? 1 : 0;
             if (cfg.shade_waves == 4)
-                hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
             else if (cfg.shade_waves == 3)
-                hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
             else
-                hipLaunchKernelGGL((k_wf_shade<SB, 0>), dim3(g.shade), dim3(SB), 0, stream, s, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 0>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
         }
         {
             Timed t(tm, WF_SHADOW);
-            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, C_SACNT,
-                               C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, ctr);
-            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w, C_SECNT,
-                               C_CUR_SE, w.se_o, w.se_d, w.se_c, ctr);
-            if (s.num_lights > 0)
-                hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, s, w,
-                                   ctr);
+            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
+                               w.sa_o, w.sa_d, w.sa_c, ctr);
+            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
+                               w.se_o, w.se_d, w.se_c, ctr);
+            if (cs.lights)
+                hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
         }
     }
     {
         Timed t(tm, WF_CONTROL);
-        hipLaunchKernelGGL(k_wf_finalize, dim3((npix + 255) / 256), dim3(256), 0, stream, w, npix);
+        // pixels touched by the chunk: [gp0, gp1) when known (eager), else all
+        const uint32_t np = gp1 - gp0;
+        hipLaunchKernelGGL(k_wf_finalize, dim3((np + 255) / 256), dim3(256), 0, stream, w, gp0, gp1);
         hipLaunchKernelGGL(k_wf_chunk_end, dim3(1), dim3(64), 0, stream, w.ctrl);
     }
 }
@@ -785,21 +874,40 @@ void with_blocks(const WavefrontConfig& cfg, F&& f) {
     else f.template operator()<128, 128>();
 }
 
+// Global pixels touched by global paths [c0, c1) (host copy of the job table).
+void chunk_pixels(const std::vector<JobDesc>& jobs, unsigned long long c0, unsigned long long c1, uint32_t& gp0,
+                  uint32_t& gp1) {
+    auto pixel_of = [&](unsigned long long g) -> uint32_t {
+        size_t j = 0;
+        while (j + 1 < jobs.size() && jobs[j + 1].path_offset <= g) ++j;
+        const JobDesc& d = jobs[j];
+        const unsigned long long px = (g - d.path_offset) / static_cast<unsigned long long>(d.params.spf);
+        return d.pixel_offset + static_cast<uint32_t>(std::min<unsigned long long>(px, d.npix));
+    };
+    gp0 = pixel_of(c0);
+    gp1 = c1 > c0 ? pixel_of(c1 - 1) + 1 : gp0;
+}
+
 template <bool COUNT>
 struct ChunkEnqueuer {
     Wavefront& wf;
-    const DevScene& s;
-    const RenderParams& p;
     const WavefrontConfig& cfg;
-    uint32_t npix;
+    const ChunkShape& cs;
+    const std::vector<JobDesc>& host_jobs;
     DevCounters* ctr;
     hipStream_t stream;
     StageTimer* tm;
-    int repeat;
+    int nchunks;
     template <int EB, int SB>
     void operator()() {
         Grids g = grids_for<COUNT, EB, SB>(wf, cfg);
-        for (int i = 0; i < repeat; ++i) enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npix, ctr, stream, tm);
+        for (int c = 0; c < nchunks; ++c) {
+            const unsigned long long c0 = static_cast<unsigned long long>(c) * wf.st.chunk_paths;
+            const unsigned long long c1 = std::min(c0 + wf.st.chunk_paths, wf.st.total_paths);
+            uint32_t gp0, gp1;
+            chunk_pixels(host_jobs, c0, c1, gp0, gp1);
+            enqueue_chunk<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, tm, gp0, gp1);
+        }
     }
 };
 
@@ -814,41 +922,62 @@ struct GridQuery {
     }
 };
 
+// The captured chunk covers every pixel (the finalize kernel skips pixels the
+// chunk does not touch), so one graph serves every chunk.
 template <bool COUNT>
 struct CaptureEnqueuer {
     Wavefront& wf;
-    const DevScene& s;
-    const RenderParams& p;
     const WavefrontConfig& cfg;
+    const ChunkShape& cs;
     const Grids& g;
-    uint32_t npix;
     DevCounters* ctr;
     hipStream_t stream;
     template <int EB, int SB>
     void operator()() {
-        enqueue_chunk<COUNT, EB, SB>(wf, s, p, cfg, g, npix, ctr, stream, nullptr);
+        enqueue_chunk<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, nullptr, 0u, wf.st.total_pixels);
     }
 };
 
-int chunk_spp_for(int spf, int want) {
-    int s = std::max(1, std::min(want, spf));
-    while (spf % s) --s;
-    return s;
-}
-
 }  // namespace
 
-hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_generation, const RenderParams& p,
-                            const WavefrontConfig& cfg, float4* film_new, float4* film_acc, DevCounters* ctr,
-                            bool count, hipStream_t stream, float* ms, std::string* err) {
-    const int S = chunk_spp_for(p.spf, cfg.chunk_spp);
-    const uint64_t npix = static_cast<uint64_t>(p.width) * p.height;
-    const uint64_t npaths64 = npix * S;
-    if (npaths64 > (1ull << 31)) {
-        if (err) *err = "wavefront chunk too large (pixels x chunk spp must stay below 2^31)";
+hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, const WavefrontConfig& cfg,
+                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err) {
+    if (jobs_in.empty()) return hipSuccess;
+    // job table: global path and pixel offsets
+    std::vector<JobDesc> jobs(jobs_in.size());
+    unsigned long long paths = 0, pixels = 0;
+    uint32_t max_lights = 0, max_mat = 0;
+    int max_depth = 0;
+    for (size_t j = 0; j < jobs_in.size(); ++j) {
+        const WavefrontJob& in = jobs_in[j];
+        JobDesc& d = jobs[j];
+        std::memset(&d, 0, sizeof(d));
+        d.scene = in.scene;
+        d.params = in.params;
+        d.film_new = in.film_new;
+        d.film_acc = in.film_acc;
+        const unsigned long long npix = static_cast<unsigned long long>(in.params.width) * in.params.height;
+        if (in.params.spf <= 0 || npix == 0) {
+            if (err) *err = "wavefront job with no pixels or samples";
+            return hipErrorInvalidValue;
+        }
+        d.path_offset = paths;
+        d.pixel_offset = static_cast<uint32_t>(pixels);
+        d.npix = static_cast<uint32_t>(npix);
+        paths += npix * static_cast<unsigned long long>(in.params.spf);
+        pixels += npix;
+        max_lights = std::max(max_lights, in.scene.num_lights);
+        max_mat = std::max(max_mat, in.scene.num_materials);
+        max_depth = std::max(max_depth, in.params.max_depth);
+    }
+    if (pixels > 0xFFFFFFFFull) {
+        if (err) *err = "wavefront batch has more than 2^32 pixels";
         return hipErrorInvalidValue;
     }
-    const uint32_t npaths = static_cast<uint32_t>(npaths64);
+    const unsigned long long want = std::max(1ull, static_cast<unsigned long long>(std::max(1, cfg.chunk_spp)) * pixels);
+    const unsigned long long cap = std::min<unsigned long long>(1ull << 31, want);
+    const uint32_t C = static_cast<uint32_t>(std::min(cap, paths));
+    const int nchunks = static_cast<int>((paths + C - 1) / C);
     if (wf->num_cus == 0) {
         int dev = 0;
         (void)hipGetDevice(&dev);
@@ -856,29 +985,36 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
             wf->num_cus <= 0)
             wf->num_cus = 256;
     }
-    if (!wf->ensure(npaths, static_cast<uint32_t>(npix), s.num_lights, s.num_materials)) {
+    const uint32_t njobs = static_cast<uint32_t>(jobs.size());
+    if (!wf->ensure(C, static_cast<uint32_t>(pixels), max_lights, max_mat, njobs)) {
         if (err) *err = "wavefront: out of device memory for path state";
         return hipErrorOutOfMemory;
     }
-    wf->st.chunk_spp = static_cast<uint32_t>(S);
-    wf->st.npaths = npaths;
+    wf->st.njobs = njobs;
+    wf->st.chunk_paths = C;
+    wf->st.total_paths = paths;
+    wf->st.total_pixels = static_cast<uint32_t>(pixels);
+    ChunkShape cs;
+    cs.max_depth = max_depth;
+    cs.nbins = max_mat + 1;
+    cs.sort = cfg.sort_by_material && cs.nbins <= kMaxSortBins;
+    cs.lights = max_lights > 0;
     hipError_t e;
-    if ((e = hipMemcpyAsync(wf->d_params, &p, sizeof(RenderParams), hipMemcpyHostToDevice, stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
+        hipSuccess)
         return e;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, stream);
     hipLaunchKernelGGL(k_wf_batch_begin, dim3(1), dim3(64), 0, stream, wf->st.ctrl);
-    const int nchunks = p.spf / S;
-    const uint32_t npix32 = static_cast<uint32_t>(npix);
     const bool use_graph = cfg.use_graph && !cfg.time_stages;
-    uint64_t key[10] = {scene_generation, static_cast<uint64_t>(p.width), static_cast<uint64_t>(p.height),
-                        static_cast<uint64_t>(S), static_cast<uint64_t>(p.max_depth),
+    uint64_t key[12] = {static_cast<uint64_t>(C), static_cast<uint64_t>(paths), static_cast<uint64_t>(pixels),
+                        static_cast<uint64_t>(njobs), static_cast<uint64_t>(max_depth),
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
-                        static_cast<uint64_t>(cfg.sort_by_material * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
+                        static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
-                        static_cast<uint64_t>(s.num_lights)};
+                        static_cast<uint64_t>(cs.lights), static_cast<uint64_t>(cs.nbins), 0};
     StageTimer timer;
     timer.stream = stream;
     StageTimer* tm = cfg.time_stages ? &timer : nullptr;
@@ -895,21 +1031,21 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
                 with_blocks(cfg, q);
                 g = q.out;
             }
-            hipStream_t cap;
-            if ((e = hipStreamCreate(&cap)) != hipSuccess) return e;
-            e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+            hipStream_t cap_stream;
+            if ((e = hipStreamCreate(&cap_stream)) != hipSuccess) return e;
+            e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
                 if (count) {
-                    CaptureEnqueuer<true> f{*wf, s, p, cfg, g, npix32, ctr, cap};
+                    CaptureEnqueuer<true> f{*wf, cfg, cs, g, ctr, cap_stream};
                     with_blocks(cfg, f);
                 } else {
-                    CaptureEnqueuer<false> f{*wf, s, p, cfg, g, npix32, ctr, cap};
+                    CaptureEnqueuer<false> f{*wf, cfg, cs, g, ctr, cap_stream};
                     with_blocks(cfg, f);
                 }
-                e = hipStreamEndCapture(cap, &wf->graph);
+                e = hipStreamEndCapture(cap_stream, &wf->graph);
             }
             if (e == hipSuccess) e = hipGraphInstantiate(&wf->exec, wf->graph, nullptr, nullptr, 0);
-            (void)hipStreamDestroy(cap);
+            (void)hipStreamDestroy(cap_stream);
             if (e != hipSuccess) {
                 wf->release_graph();
                 if (err) *err = std::string("wavefront graph capture: ") + hipGetErrorString(e);
@@ -920,16 +1056,16 @@ hipError_t wavefront_render(Wavefront* wf, const DevScene& s, uint64_t scene_gen
         for (int c = 0; c < nchunks; ++c)
             if ((e = hipGraphLaunch(wf->exec, stream)) != hipSuccess) return e;
     } else if (count) {
-        ChunkEnqueuer<true> f{*wf, s, p, cfg, npix32, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<true> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks};
         with_blocks(cfg, f);
     } else {
-        ChunkEnqueuer<false> f{*wf, s, p, cfg, npix32, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<false> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks};
         with_blocks(cfg, f);
     }
     {
         Timed t(tm, WF_CONTROL);
-        hipLaunchKernelGGL(k_wf_film, dim3((npix32 + 255) / 256), dim3(256), 0, stream, wf->st, npix32, film_new,
-                           film_acc, ctr, count ? 1u : 0u);
+        hipLaunchKernelGGL(k_wf_film, dim3((static_cast<uint32_t>(pixels) + 255) / 256), dim3(256), 0, stream,
+                           wf->st, ctr, count ? 1u : 0u);
     }
     (void)hipEventRecord(e1, stream);
     e = hipEventSynchronize(e1);

@@ -450,4 +450,34 @@ bool generate_spheres(HostScene& s, uint64_t seed, int instances, int level, int
     return true;
 }
 
+// One job of the batched workload: the icosphere scene with a triangle count
+// log-uniform in [1e3, 5e5] (sphere level 2..5, up to 64 instances), a pixel
+// count log-uniform in [128^2, 1024^2] at an aspect ratio in [1, 16/9], spp
+// from {4, 8, 16, 32, 64}, depth 12 and its own 512x256 sky.
+bool generate_batch_job(HostScene& s, uint64_t seed) {
+    Rng rng(seed ^ 0xBA7Cull);
+    const double tris = std::exp(std::log(1e3) + rng.uniform() * (std::log(5e5) - std::log(1e3)));
+    int level = 2;
+    for (int l = 5; l >= 2; --l)
+        if (tris >= 2.0 * 20.0 * std::pow(4.0, l)) {
+            level = l;
+            break;
+        }
+    const double per = 20.0 * std::pow(4.0, level);
+    const int instances = static_cast<int>(std::max(1.0, std::min(64.0, std::round(tris / per))));
+    generate_spheres(s, seed, instances, level, 512, 256);
+    const double pix = std::exp(std::log(128.0 * 128.0) + rng.uniform() * (std::log(1024.0 * 1024.0) - std::log(128.0 * 128.0)));
+    const double aspect = 1.0 + rng.uniform() * (16.0 / 9.0 - 1.0);
+    s.camera.width = std::max(16, static_cast<int>(std::lround(std::sqrt(pix * aspect))));
+    s.camera.height = std::max(16, static_cast<int>(std::lround(pix / s.camera.width)));
+    const int spps[5] = {4, 8, 16, 32, 64};
+    const int spp = spps[std::min<int>(4, static_cast<int>(rng.uniform() * 5.0f))];
+    s.settings.sample_per_frame = spp;
+    s.settings.target_samples = spp;
+    s.settings.max_depth = 12;
+    s.path = "<batch-job>";
+    s.output_file = "batch_job.hdr";
+    return true;
+}
+
 }  // namespace ptamd
