@@ -53,6 +53,7 @@ _SIGS = {
     "oracle_tex2d": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _FP]),
     "oracle_render": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                              C.c_int, C.c_int, C.c_int, _FP, _U64P]),
+    "oracle_microfacet": (None, [_FP, _FP, _FP, _FP, _FP]),
     "oracle_adjoint_fd": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _FP, C.c_int, _I32P,
                                  _I32P, _I32P, _I32P, C.c_double, C.c_int, _DP, _DP]),
 }
@@ -281,4 +282,12 @@ def tex2d(rgba: np.ndarray, u: float, v: float, wrap_s=10497, wrap_t=10497):
     rgba = np.ascontiguousarray(rgba, np.float32)
     out = np.zeros(4, np.float32)
     lib().oracle_tex2d(_f(rgba), rgba.shape[1], rgba.shape[0], wrap_s, wrap_t, u, v, _f(out))
+    return out
+
+
+def microfacet(inputs, f0, wo, n) -> np.ndarray:
+    """The oracle's microfacet / sampling / frame pieces (27 floats, see oracle.h)."""
+    a = [np.ascontiguousarray(x, np.float32) for x in (inputs, f0, wo, n)]
+    out = np.zeros(27, np.float32)
+    lib().oracle_microfacet(_f(a[0]), _f(a[1]), _f(a[2]), _f(a[3]), _f(out))
     return out

@@ -1720,6 +1720,34 @@ void oracle_render(const oracle_scene* s, const pt_camera* cam, int spf, int max
     }
 }
 
+void oracle_microfacet(const float* in, const float* f0, const float* wo3, const float* n3, float* out) {
+    using R = float;
+    const R noh = in[0], hx = in[1], hy = in[2], a = in[3], ax = in[4], ay = in[5], nov = in[6], nol = in[7],
+            c = in[8], eta = in[9], u1 = in[10], u2 = in[11];
+    const V3<R> wo(wo3[0], wo3[1], wo3[2]), n(n3[0], n3[1], n3[2]), F0(f0[0], f0[1], f0[2]);
+    out[0] = D_GGX(noh, a);
+    out[1] = D_GGX_Aniso(noh, hx, hy, ax, ay);
+    out[2] = G1_Smith(nov, a);
+    out[3] = G2_Smith(nol, nov, a);
+    out[4] = F_SchlickS(c, R(0.04));
+    V3<R> fs3 = F_SchlickV(c, F0);
+    out[5] = fs3.x; out[6] = fs3.y; out[7] = fs3.z;
+    out[8] = F_Dielectric(c, eta);
+    V3<R> h = sampleVNDF(wo, ax, ay, u1, u2);
+    out[9] = h.x; out[10] = h.y; out[11] = h.z;
+    out[12] = pdfVNDF(wo, h, ax, ay);
+    V3<R> wt(0);
+    out[13] = refractDir(wo, V3<R>(0, 0, 1), eta, wt) ? 1.0f : 0.0f;
+    out[14] = wt.x; out[15] = wt.y; out[16] = wt.z;
+    V3<R> cs = sampleCosineHemisphere(u1, u2);
+    out[17] = cs.x; out[18] = cs.y; out[19] = cs.z;
+    out[20] = powerHeuristic(nov, nol);
+    Frame<R> fr(n);
+    V3<R> l = fr.toLocal(wo), w = fr.toWorld(wo);
+    out[21] = l.x; out[22] = l.y; out[23] = l.z;
+    out[24] = w.x; out[25] = w.y; out[26] = w.z;
+}
+
 void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
                        uint32_t frame, const float* weights, int nparams, const int* kind, const int* pa, const int* pb,
                        const int* pc, double h, int threads, double* grad, double* value) {

@@ -69,6 +69,13 @@ void oracle_bsdf_eval(const pt_material* m, const double wo[3], const double wi[
 int oracle_bsdf_sample(const pt_material* m, const double wo[3], const double n[3], int entering, const float u[3],
                        double wi[3], double weight[3], double* pdf, int* delta);
 
+/* Microfacet and sampling pieces (materials/microfacet.cuh, math/sampling.h,
+ * math/frame.h) in float for the inputs in[12] = {noh, hx, hy, alpha, ax, ay,
+ * nov, nol, cos, eta, u1, u2}: out[27] = D, D_aniso, G1, G2, Schlick(0.04),
+ * Schlick(f0) x3, F_dielectric, VNDF half vector x3, its pdf, refract ok +
+ * direction x3 (normal +z), cosine sample x3, power heuristic(nov, nol),
+ * Frame(n) toLocal(wo) x3 and toWorld(wo) x3. */
+void oracle_microfacet(const float* in, const float* f0, const float* wo, const float* n, float* out);
 /* Equirect mapping and env pdf (envmap.h:15-32, light_pdf.cuh:12-27). */
 void oracle_dir_to_uv(const float d[3], float uv[2]);
 void oracle_uv_to_dir(const float uv[2], float d[3]);

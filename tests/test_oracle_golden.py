@@ -270,6 +270,18 @@ def test_envmap_matches_reference(golden):
         assert oracle.uv_to_dir(np.array(uv, np.float32)).tolist() == [f32(x) for x in d]
 
 
+def test_microfacet_pieces_match_reference(golden):
+    # materials/microfacet.cuh, math/sampling.h and math/frame.h compiled as-is
+    for r in golden["microfacet"]:
+        o = oracle.microfacet(r["in"], r["f0"], r["wo"], r["n"])
+        ref = [r["d_ggx"], r["d_aniso"], r["g1"], r["g2"], r["fs"], *r["fs3"], r["fd"], *r["vndf"], r["pdf_vndf"],
+               float(r["refract_ok"]), *r["refract"], *r["cosine"], r["power"], *r["local"], *r["world"]]
+        ref = np.array(ref, np.float32)
+        # the same float expressions: equal to the last bit but for libm
+        # (sqrt/cos/sin of the host library on both sides)
+        assert np.allclose(o, ref, rtol=2e-6, atol=1e-7), (r["in"], o - ref)
+
+
 def test_env_pdf_integrates_to_one():
     # environmentPdf is a density over the sphere: its integral is 1
     rng = np.random.default_rng(5)
