@@ -169,6 +169,60 @@ def backward_leg(steps: int = 3) -> dict:
             "params": "256x256 albedo texture, roughness + metallic of 8 materials, 512x256 env"}
 
 
+def batched_leg(njobs: int = 64, seed: int = 1000) -> dict:
+    """The batched workload: njobs seeded scenes (icospheres of 1e3..5e5
+    triangles, 128^2..1024^2 pixels, 4..64 spp, depth 12, own 512x256 sky) in
+    one wavefront schedule, against the same scenes one at a time through the
+    plain megakernel.  GPU time from HIP events; rays from in-kernel counters
+    of a counted re-run."""
+    import pathtracingtutorial_amd as pt
+    t0 = time.perf_counter()
+    scenes = []
+    for j in range(njobs):
+        sm = pt.SceneManager()
+        sm.generate(4, seed + j)
+        sm.buildAccel()
+        sm.uploadToDevice()
+        scenes.append(sm)
+    infos = [sm.info() for sm in scenes]
+    log(f"batched workload: {njobs} scenes built in {time.perf_counter() - t0:.1f} s; "
+        f"{sum(i.num_faces for i in infos)} triangles, {sum(i.width * i.height for i in infos)} pixels, "
+        f"{sum(i.width * i.height * i.sample_per_frame for i in infos)} paths")
+    # one at a time through the megakernel
+    plain = [pt.Renderer(sm, pt.PathIntegrator()) for sm in scenes]
+    for r in plain:
+        r.renderBatch()  # warm-up
+    for sm in scenes:
+        sm.uploadToDevice()
+    plain_ms = 0.0
+    for r in plain:
+        r.renderBatch()
+        plain_ms += r.lastBatchMs()
+    # batched wavefront (the first scene's engine and autotuned-by-default config)
+    for sm in scenes:
+        sm.uploadToDevice()
+    wf = [pt.Renderer(sm, pt.WavefrontIntegrator()) for sm in scenes]
+    pt.render_jobs(wf)  # warm-up + graph capture
+    for sm in scenes:
+        sm.uploadToDevice()
+    batch_ms = pt.render_jobs(wf)
+    # rays (the same frames, counted)
+    for sm in scenes:
+        sm.uploadToDevice()
+    scenes[0].enableCounters(True)
+    scenes[0].resetCounters()
+    pt.render_jobs(wf)
+    c = scenes[0].counters()
+    scenes[0].enableCounters(False)
+    rays = c["closest_rays"] + c["shadow_rays"]
+    return {"workload": f"{njobs} seeded icosphere scenes (seeds {seed}..{seed + njobs - 1}), depth 12",
+            "value": round(rays / (batch_ms / 1e3) / 1e6, 2), "unit": "Mrays/s",
+            "batched_ms": round(batch_ms, 2), "sequential_megakernel_ms": round(plain_ms, 2),
+            "speedup": round(plain_ms / batch_ms, 3),
+            "sequential_megakernel_mrays": round(rays / (plain_ms / 1e3) / 1e6, 2), "rays": rays,
+            "paths": c["paths"]}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +233,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the wavefront autotuner (use the default config)")
     ap.add_argument("--no-backward", action="store_true", help="skip the differentiable (backward) measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--tune-trials", type=int, default=16)
     ap.add_argument("--wavefront-config", default="",
                     help='JSON object of wavefront config fields to use instead of autotuning, '
@@ -309,6 +364,12 @@ def main(argv=None) -> int:
                         # same random numbers per path: the images are the same estimate
                         "image_channels_differing_1e-4": round(differ, 6),
                         "rays_per_step": prays // a.steps}
+    if not a.no_batched:
+        try:
+            out["batched"] = batched_leg()
+            log(f"batched: {out['batched']}")
+        except Exception as e:  # reported beside the forward number, never required for it
+            out["batched"] = {"value": None, "error": str(e)}
     if not a.no_backward:
         try:
             out["backward"] = backward_leg()

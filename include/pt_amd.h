@@ -415,6 +415,13 @@ pt_status pt_probe_sampler(int sampler, uint32_t seed, const int32_t* xy_idx, in
  * does (core/ray.h:13).  t_out gets the hit distance (FLT_MAX on a miss; any-
  * hit: 1 occluded / 0 not), face_out the ORIGINAL face index or -1,
  * bary_out (may be NULL) 2 floats per ray. */
+/* Disney BSDF (disney_bsdf.cuh:53-122 fetch + clamps without textures,
+ * :209-321 eval, :325-410 sample) of n materials on the device: dirs9 holds
+ * wo, wi, n per case, u3 the three uniforms a sample consumes (lobe choice,
+ * then the warp pair).  out12 per case: f.xyz, pdf, sampled wi.xyz,
+ * weight.xyz, pdf, flags (1 valid, 2 delta). */
+pt_status pt_probe_bsdf(const pt_material* mats, int n, const float* dirs9, const int32_t* entering, const float* u3,
+                        float* out12);
 pt_status pt_trace_rays(pt_scene* scene, const float* rays8, int n, int any_hit,
                         float* t_out, int32_t* face_out, float* bary_out);
 

@@ -1263,6 +1263,28 @@ pt_status pt_trace_rays(pt_scene* s, const float* rays8, int n, int any_hit, flo
     });
 }
 
+pt_status pt_probe_bsdf(const pt_material* mats, int n, const float* dirs9, const int32_t* entering, const float* u3,
+                        float* out12) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!mats || !dirs9 || !entering || !u3 || !out12)))
+            throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        if (n == 0) return;
+        ensure_device();
+        DBuf<pt_material> dm;
+        DBuf<float> dd, du, dout;
+        DBuf<int32_t> de;
+        dm.upload(mats, n, nullptr);
+        dd.upload(dirs9, static_cast<size_t>(n) * 9, nullptr);
+        du.upload(u3, static_cast<size_t>(n) * 3, nullptr);
+        de.upload(entering, n, nullptr);
+        dout.alloc(static_cast<size_t>(n) * 12);
+        hip_check(ptd::launch_probe_bsdf(dm.p, n, dd.p, de.p, du.p, dout.p, nullptr), "bsdf probe launch");
+        hip_check(hipDeviceSynchronize(), "bsdf probe");
+        hip_check(hipMemcpy(out12, dout.p, sizeof(float) * 12 * static_cast<size_t>(n), hipMemcpyDeviceToHost),
+                  "download");
+    });
+}
+
 pt_status pt_probe_copy(size_t bytes, int iters, float* gbps) {
     return guarded([&] {
         if (bytes < 16 || iters <= 0 || !gbps) throw Error(PT_ERR_INVALID_ARG, "bad argument");

@@ -467,7 +467,8 @@ __device__ __forceinline__ BsdfEvalT<S> eval_bsdf(const MatEval<S>& m, F3 wo, F3
 }
 
 // sampleDisneyBsdf (disney_bsdf.cuh:325-410), float.
-__device__ __forceinline__ BsdfSample sample_bsdf(const MatEval<float>& m, F3 wo, F3 n, Sampler& rng, bool entering) {
+template <class Rng>
+__device__ __forceinline__ BsdfSample sample_bsdf(const MatEval<float>& m, F3 wo, F3 n, Rng& rng, bool entering) {
     BsdfSample res;
     res.wi = f3(0.0f);
     res.weight = f3(0.0f);

@@ -9,6 +9,8 @@ namespace ptd {
 
 hipError_t launch_render_plain(const DevScene& s, const RenderParams& p, float4* film_new, float4* film_acc,
                                DevCounters* ctr, bool count, hipStream_t stream);
+hipError_t launch_probe_bsdf(const pt_material* d_mats, int n, const float* d_dirs, const int32_t* d_entering,
+                             const float* d_u, float* d_out, hipStream_t stream);
 hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, int n, int dims, uint32_t* d_u,
                                 float* d_f, hipStream_t stream);
 hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, float* d_t, int32_t* d_face,

@@ -150,6 +150,44 @@ __global__ __launch_bounds__(256) void k_trace_rays(DevScene s, const float* __r
     }
 }
 
+// Disney BSDF of one material (fetch + clamps, no textures): eval at (wo, wi)
+// and a sample drawn with three given uniform numbers (choice, then the two
+// warp numbers), as sampleDisneyBsdf<FixedRng> consumes them.
+struct FixedRng {
+    const float* u;
+    int k;
+    __device__ __forceinline__ float next() { return u[k++]; }
+};
+
+__global__ void k_probe_bsdf(const pt_material* __restrict__ mats, int n, const float* __restrict__ dirs,
+                             const int32_t* __restrict__ entering, const float* __restrict__ us,
+                             float* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DevScene empty;
+    memset(&empty, 0, sizeof(empty));
+    MatEval<float> me = fetch_material(empty, mats[i], 0.0f, 0.0f);
+    const float* d = dirs + 9 * static_cast<size_t>(i);
+    F3 wo = f3(d[0], d[1], d[2]), wi = f3(d[3], d[4], d[5]), nn = f3(d[6], d[7], d[8]);
+    const bool ent = entering[i] != 0;
+    BsdfEvalT<float> e = eval_bsdf(me, wo, wi, nn, ent);
+    FixedRng rng{us + 3 * static_cast<size_t>(i), 0};
+    BsdfSample s = sample_bsdf(me, wo, nn, rng, ent);
+    float* o = out + 12 * static_cast<size_t>(i);
+    o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf;
+    o[4] = s.wi.x; o[5] = s.wi.y; o[6] = s.wi.z;
+    o[7] = s.weight.x; o[8] = s.weight.y; o[9] = s.weight.z;
+    o[10] = s.pdf;
+    o[11] = static_cast<float>((s.valid ? 1 : 0) | (s.delta ? 2 : 0));
+}
+
+hipError_t launch_probe_bsdf(const pt_material* d_mats, int n, const float* d_dirs, const int32_t* d_entering,
+                             const float* d_u, float* d_out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_probe_bsdf, dim3((n + 127) / 128), dim3(128), 0, stream, d_mats, n, d_dirs, d_entering, d_u,
+                       d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, int n, int dims, uint32_t* d_u,
                                 float* d_f, hipStream_t stream) {
     hipLaunchKernelGGL(k_probe_sampler, dim3((n + 255) / 256), dim3(256), 0, stream, type, seed, d_xyi, n, dims, d_u,

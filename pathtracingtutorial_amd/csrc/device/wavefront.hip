@@ -975,7 +975,8 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
         return hipErrorInvalidValue;
     }
     const unsigned long long want = std::max(1ull, static_cast<unsigned long long>(std::max(1, cfg.chunk_spp)) * pixels);
-    const unsigned long long cap = std::min<unsigned long long>(1ull << 31, want);
+    // at most 2^26 paths in flight (~15 GB of path state and queues)
+    const unsigned long long cap = std::min<unsigned long long>(1ull << 26, want);
     const uint32_t C = static_cast<uint32_t>(std::min(cap, paths));
     const int nchunks = static_cast<int>((paths + C - 1) / C);
     if (wf->num_cus == 0) {
