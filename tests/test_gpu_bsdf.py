@@ -64,6 +64,7 @@ def test_bsdf_eval_and_sample_match_oracle(golden):
     L = oracle.lib()
     D3 = C.c_double * 3
     bad_eval, bad_sample, flags_off = 0, 0, 0
+    examples = []
     for i in range(n):
         f = D3()
         pdf = C.c_double()
@@ -83,12 +84,25 @@ def test_bsdf_eval_and_sample_match_oracle(golden):
             flags_off += 1
             continue
         if valid:
-            ok = np.allclose(out[i, 4:7], list(swi), atol=1e-4)
-            ok = ok and np.allclose(out[i, 10], spdf.value, rtol=1e-4, atol=1e-6)
-            ok = ok and np.allclose(out[i, 7:10], list(sw), rtol=1e-3, atol=1e-5)
-            bad_sample += 0 if ok else 1
+            # the sampled direction agrees; weight and pdf are those of the
+            # oracle's BSDF at the GPU's direction (a peaked lobe turns a
+            # last-bit difference of the direction into a large change)
+            bad = not np.allclose(out[i, 4:7], list(swi), atol=1e-4)
+            if sdelta.value:
+                bad = bad or not np.allclose(out[i, 7:10], list(sw), rtol=1e-4, atol=1e-6)
+            else:
+                g_wi = D3(*map(float, out[i, 4:7]))
+                L.oracle_bsdf_eval(C.c_void_p(C.addressof(mats[i])), D3(*map(float, wo[i])), g_wi,
+                                   D3(*map(float, nrm[i])), int(ent[i]), 0, f, C.byref(pdf))
+                w_ref = np.array(list(f)) / (pdf.value + 1e-7)
+                bad = bad or not np.allclose(out[i, 10], pdf.value, rtol=1e-3, atol=1e-6)
+                bad = bad or not np.allclose(out[i, 7:10], w_ref, rtol=1e-3, atol=1e-5)
+            if bad:
+                bad_sample += 1
+                if len(examples) < 5:
+                    examples.append((i, out[i].tolist(), list(swi), list(sw), spdf.value))
     assert bad_eval <= 0.002 * n, bad_eval
-    assert bad_sample <= 0.005 * n, bad_sample
+    assert bad_sample <= 0.005 * n, (bad_sample, examples)
     assert flags_off <= 0.002 * n, flags_off
     # every lobe was exercised
     fl = out[:, 11].astype(int)
