@@ -198,6 +198,7 @@ typedef struct pt_wavefront_config {
     int32_t use_graph;        /* replay one captured hipGraph per chunk */
     int32_t time_stages;      /* bracket every launch with HIP events (no graph): pt_scene_wavefront_stats */
     int32_t shade_waves;      /* shading kernel variant built for >= this many waves per SIMD: 0, 3 or 4 */
+    int32_t stack_lds;        /* traversal stack entries held in LDS, 16 or 32 (the rest in device memory) */
 } pt_wavefront_config;
 
 /* GPU time (ms) and launch count per stage of the last wavefront batch
@@ -237,7 +238,10 @@ pt_status pt_scene_create(const pt_host_scene_desc* desc, pt_scene** out);
  * materials, 256x256 albedo texture on the back wall, 512x256 environment,
  * 256x256, 64 spp, depth 6; the seeded values are a ground truth), 4 = one
  * job of the batched workload (icospheres of 1e3..5e5 triangles, 128^2..1024^2
- * pixels, 4..64 spp, depth 12, 512x256 sky; all drawn from the seed).
+ * pixels, 4..64 spp, depth 12, 512x256 sky; all drawn from the seed), 5 =
+ * the multi-view differentiable scene (2,048,000 icosphere triangles, 16
+ * materials, 1024x1024 albedo texture on the ground, 1024x512 sky; 512x512,
+ * 32 spp, depth 10).
  * No reference counterpart; seeds fix every value. */
 pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out);
 void pt_scene_destroy(pt_scene* scene);
@@ -379,6 +383,17 @@ typedef struct pt_adjoint_grads {
  * gradient; FP32 atomics accumulate (last-bit nondeterminism). */
 pt_status pt_render_adjoint(pt_scene* scene, uint32_t frame, const float* image_grad, const pt_adjoint_grads* grads,
                             float* replay_image, float* ms_out);
+/* Many cameras of one scene (no reference counterpart).  pt_render_views
+ * renders one batch per camera into films[i] (sized like cameras[i]) in one
+ * wavefront schedule, with the random numbers of frames[i] (NULL: the
+ * scene's frame_count for all; frame counts are not advanced).
+ * pt_render_adjoint_views replays all views' batches in one launch and
+ * accumulates the gradients of all of them into grads. */
+pt_status pt_render_views(pt_scene* scene, int n, const pt_camera* cameras, pt_film* const* films,
+                          const uint32_t* frames, float* ms_out);
+pt_status pt_render_adjoint_views(pt_scene* scene, int n, const pt_camera* cameras, const uint32_t* frames,
+                                  const float* const* image_grads, const pt_adjoint_grads* grads,
+                                  float* const* replay_images, float* ms_out);
 /* Sets frame_count (the sample-index offset of the next batch). */
 pt_status pt_scene_set_frame(pt_scene* scene, uint32_t frame_count);
 /* First texel and texel count of texture tex in the device texel pool. */

@@ -227,10 +227,10 @@ ptd::DevScene make_view(const pt_scene* s) {
     return v;
 }
 
-ptd::RenderParams make_params(const pt_scene* s) {
+ptd::RenderParams make_params_cam(const pt_scene* s, const pt_camera& cam) {
     ptd::RenderParams p;
     std::memset(&p, 0, sizeof(p));
-    pt_camera_frame f = ptamd::camera_frame(s->host.camera);
+    pt_camera_frame f = ptamd::camera_frame(cam);
     for (int i = 0; i < 3; ++i) {
         p.cam_pos[i] = f.position[i];
         p.cam_fwd[i] = f.forward[i];
@@ -241,8 +241,8 @@ ptd::RenderParams make_params(const pt_scene* s) {
     p.aspect = f.aspect;
     p.aperture = f.aperture;
     p.focal_distance = f.focal_distance;
-    p.width = s->host.camera.width;
-    p.height = s->host.camera.height;
+    p.width = cam.width;
+    p.height = cam.height;
     p.spf = s->host.settings.sample_per_frame;
     p.max_depth = s->host.settings.max_depth;
     p.sampler = s->host.settings.sampler;
@@ -250,6 +250,8 @@ ptd::RenderParams make_params(const pt_scene* s) {
     p.frame_count = s->frame_count;
     return p;
 }
+
+ptd::RenderParams make_params(const pt_scene* s) { return make_params_cam(s, s->host.camera); }
 
 // Depth-first (reference) node order from the build topology.
 void dfs_layout(const ptd::BvhHostTopology& t, std::vector<float>& out) {
@@ -337,6 +339,7 @@ void check_wavefront_config(const pt_wavefront_config& c) {
     if (c.waves_per_simd < 0 || c.waves_per_simd > 8) throw Error(PT_ERR_INVALID_ARG, "waves_per_simd must be in [0, 8]");
     if (c.shade_waves != 0 && c.shade_waves != 3 && c.shade_waves != 4)
         throw Error(PT_ERR_INVALID_ARG, "shade_waves must be 0, 3 or 4");
+    if (c.stack_lds != 16 && c.stack_lds != 32) throw Error(PT_ERR_INVALID_ARG, "stack_lds must be 16 or 32");
 }
 
 ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
@@ -349,6 +352,7 @@ ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
     d.use_graph = c.use_graph ? 1 : 0;
     d.time_stages = c.time_stages ? 1 : 0;
     d.shade_waves = c.shade_waves;
+    d.stack_lds = c.stack_lds;
     return d;
 }
 
@@ -362,6 +366,7 @@ pt_wavefront_config to_abi_config(const ptd::WavefrontConfig& d) {
     c.use_graph = d.use_graph;
     c.time_stages = d.time_stages;
     c.shade_waves = d.shade_waves;
+    c.stack_lds = d.stack_lds;
     return c;
 }
 
@@ -471,6 +476,7 @@ pt_status pt_scene_generate(int which, uint64_t seed, pt_scene** out) {
         else if (which == 2) ptamd::generate_spheres(sc->host, seed, 8, 3, 256, 128);  // small variant for tests
         else if (which == 3) ptamd::generate_diff_cornell(sc->host, seed);
         else if (which == 4) ptamd::generate_batch_job(sc->host, seed);
+        else if (which == 5) ptamd::generate_diff_views(sc->host, seed);
         else throw Error(PT_ERR_INVALID_ARG, "unknown workload id " + std::to_string(which));
         validate_scene(sc->host);
         *out = sc.release();
@@ -970,35 +976,98 @@ pt_status pt_scene_get_wavefront_config(const pt_scene* s, pt_wavefront_config* 
     });
 }
 
+namespace {
+
+ptd::AdjointGrads to_grads(const pt_adjoint_grads* g) {
+    ptd::AdjointGrads G;
+    G.base = g->base_color;
+    G.rough = g->roughness;
+    G.metal = g->metallic;
+    G.tex = g->textures;
+    G.env = g->env;
+    return G;
+}
+
+float run_adjoint(pt_scene* s, const std::vector<ptd::AdjointView>& views, const pt_adjoint_grads* g) {
+    ptd::DevScene v = make_view(s);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
+    hipError_t le = ptd::launch_adjoint(v, views, to_grads(g), s->counters.p, s->counting, s->stream);
+    hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
+    hipError_t se = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hip_check(le, "adjoint launch");
+    hip_check(se, "adjoint");
+    return ms;
+}
+
+void check_camera(const pt_camera& c) {
+    if (c.width <= 0 || c.height <= 0 || c.width > 65536 || c.height > 65536)
+        throw Error(PT_ERR_INVALID_ARG, "camera resolution out of range");
+}
+
+}  // namespace
+
 pt_status pt_render_adjoint(pt_scene* s, uint32_t frame, const float* image_grad, const pt_adjoint_grads* g,
                             float* replay_image, float* ms_out) {
     return guarded([&] {
         if (!s || !image_grad || !g) throw Error(PT_ERR_INVALID_ARG, "null argument");
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
-        ptd::DevScene v = make_view(s);
-        ptd::RenderParams p = make_params(s);
-        p.frame_count = frame;
-        ptd::AdjointGrads G;
-        G.base = g->base_color;
-        G.rough = g->roughness;
-        G.metal = g->metallic;
-        G.tex = g->textures;
-        G.env = g->env;
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
-        hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
-        hipError_t le = ptd::launch_adjoint(v, p, reinterpret_cast<const float4*>(image_grad), G,
-                                            reinterpret_cast<float4*>(replay_image), s->counters.p, s->counting,
-                                            s->stream);
-        hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
-        hipError_t se = hipEventSynchronize(e1);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        hip_check(le, "adjoint launch");
-        hip_check(se, "adjoint");
+        ptd::AdjointView view;
+        view.params = make_params(s);
+        view.params.frame_count = frame;
+        view.img_grad = reinterpret_cast<const float4*>(image_grad);
+        view.replay = reinterpret_cast<float4*>(replay_image);
+        float ms = run_adjoint(s, std::vector<ptd::AdjointView>{view}, g);
+        if (ms_out) *ms_out = ms;
+    });
+}
+
+pt_status pt_render_adjoint_views(pt_scene* s, int n, const pt_camera* cameras, const uint32_t* frames,
+                                  const float* const* image_grads, const pt_adjoint_grads* g,
+                                  float* const* replay_images, float* ms_out) {
+    return guarded([&] {
+        if (!s || n <= 0 || !cameras || !frames || !image_grads || !g) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        std::vector<ptd::AdjointView> views(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            check_camera(cameras[i]);
+            if (!image_grads[i]) throw Error(PT_ERR_INVALID_ARG, "null image gradient");
+            views[i].params = make_params_cam(s, cameras[i]);
+            views[i].params.frame_count = frames[i];
+            views[i].img_grad = reinterpret_cast<const float4*>(image_grads[i]);
+            views[i].replay = replay_images ? reinterpret_cast<float4*>(replay_images[i]) : nullptr;
+        }
+        float ms = run_adjoint(s, views, g);
+        if (ms_out) *ms_out = ms;
+    });
+}
+
+pt_status pt_render_views(pt_scene* s, int n, const pt_camera* cameras, pt_film* const* films,
+                          const uint32_t* frames, float* ms_out) {
+    return guarded([&] {
+        if (!s || n <= 0 || !cameras || !films) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        std::vector<ptd::WavefrontJob> jobs(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            check_camera(cameras[i]);
+            const pt_film* f = films[i];
+            if (!f || f->w != cameras[i].width || f->h != cameras[i].height)
+                throw Error(PT_ERR_INVALID_ARG, "film size does not match the view's camera");
+            for (int k = 0; k < i; ++k)
+                if (films[k] == f) throw Error(PT_ERR_INVALID_ARG, "two views share a film");
+            jobs[i].scene = make_view(s);
+            jobs[i].params = make_params_cam(s, cameras[i]);
+            jobs[i].params.frame_count = frames ? frames[i] : s->frame_count;
+            jobs[i].film_new = f->film_new.p;
+            jobs[i].film_acc = f->film_acc.p;
+        }
+        float ms = render_wavefront_jobs(s, jobs, s->wf_cfg);
         if (ms_out) *ms_out = ms;
     });
 }
@@ -1164,6 +1233,7 @@ pt_status pt_scene_tune_wavefront(pt_scene* s, int max_trials, pt_wavefront_conf
             try_axis([](ptd::WavefrontConfig& c, int v) { c.sort_by_material = v; }, {0, 1});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.waves_per_simd = v; }, {0, 1, 2, 4});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.shade_waves = v; }, {0, 3, 4});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.stack_lds = v; }, {16, 32});
         } catch (...) {
             s->counting = counting;
             throw;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "dev_scene.h"
 
 namespace ptd {
@@ -16,11 +18,19 @@ struct AdjointGrads {
     float* env;    // env_w x env_h x 4
 };
 
-// Replays batch `p.frame_count` (p.spf samples per pixel, the forward's
-// random numbers) and accumulates d loss / d parameter, given img_grad =
-// d loss / d (batch mean image), W*H float4.  replay (optional, W*H float4)
-// receives the batch mean re-computed by the replay pass.
-hipError_t launch_adjoint(const DevScene& s, const RenderParams& p, const float4* img_grad, const AdjointGrads& g,
-                          float4* replay, DevCounters* ctr, bool count, hipStream_t stream);
+// One view of a scene: camera and batch parameters (params.frame_count,
+// params.spf), img_grad = d loss / d (batch mean image), W*H float4, and an
+// optional replay image (W*H float4) that receives the batch mean
+// re-computed by the replay pass.
+struct AdjointView {
+    RenderParams params;
+    const float4* img_grad;
+    float4* replay;
+};
+
+// Replays every view's batch (the forward's random numbers) in one launch
+// and accumulates d loss / d parameter into g.  Synchronous.
+hipError_t launch_adjoint(const DevScene& s, const std::vector<AdjointView>& views, const AdjointGrads& g,
+                          DevCounters* ctr, bool count, hipStream_t stream);
 
 }  // namespace ptd

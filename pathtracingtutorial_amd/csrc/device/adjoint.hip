@@ -21,6 +21,9 @@
 // FP32 atomics (their order is not fixed: results vary in the last bits).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <vector>
+
 #include "adjoint.h"
 #include "dev_dual.h"
 #include "dev_integrator.h"
@@ -219,16 +222,37 @@ struct AdjointNee {
     }
 };
 
+// A view of the batch: camera/batch parameters, its image gradient and
+// (optionally) its replay image, and its first 16x16 tile in the grid.
+struct AdjView {
+    RenderParams p;
+    const float4* img_grad;
+    float4* replay;
+    uint32_t tile_offset;
+    uint32_t _pad;
+};
+
 template <bool COUNT>
-__global__ __launch_bounds__(kThreads) void k_adjoint(DevScene s, RenderParams p, const float4* __restrict__ img_grad,
-                                                      AdjointGrads G, float4* __restrict__ replay, DevCounters* ctr) {
+__global__ __launch_bounds__(kThreads) void k_adjoint(DevScene s, const AdjView* __restrict__ views, uint32_t nviews,
+                                                      AdjointGrads G, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * kThreads];
     __shared__ float macc[kLdsMaterials * kDP];
     const bool use_lds = s.num_materials <= kLdsMaterials;
     if (use_lds)
         for (uint32_t k = threadIdx.x; k < s.num_materials * kDP; k += blockDim.x) macc[k] = 0.0f;
     __syncthreads();
-    uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t vlo = 0, vhi = nviews;  // the view owning this tile
+    while (vhi - vlo > 1) {
+        uint32_t mid = (vlo + vhi) >> 1;
+        if (views[mid].tile_offset <= gtile) vlo = mid;
+        else vhi = mid;
+    }
+    const AdjView& view = views[vlo];
+    const RenderParams& p = view.p;
+    const float4* __restrict__ img_grad = view.img_grad;
+    float4* __restrict__ replay = view.replay;
+    const uint32_t tile = gtile - view.tile_offset;
     int tx = static_cast<int>(tile % static_cast<uint32_t>(p.tiles_x));
     int ty = static_cast<int>(tile / static_cast<uint32_t>(p.tiles_x));
     int x = tx * kTile + static_cast<int>(threadIdx.x & (kTile - 1));
@@ -302,18 +326,39 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(DevScene s, RenderParams p
 
 }  // namespace
 
-hipError_t launch_adjoint(const DevScene& s, const RenderParams& p, const float4* img_grad, const AdjointGrads& g,
-                          float4* replay, DevCounters* ctr, bool count, hipStream_t stream) {
-    uint32_t tiles_x = static_cast<uint32_t>((p.width + kTile - 1) / kTile);
-    uint32_t tiles_y = static_cast<uint32_t>((p.height + kTile - 1) / kTile);
-    RenderParams q = p;
-    q.tiles_x = static_cast<int32_t>(tiles_x);
-    dim3 grid(tiles_x * tiles_y), block(kThreads);
-    if (count)
-        hipLaunchKernelGGL(k_adjoint<true>, grid, block, 0, stream, s, q, img_grad, g, replay, ctr);
-    else
-        hipLaunchKernelGGL(k_adjoint<false>, grid, block, 0, stream, s, q, img_grad, g, replay, ctr);
-    return hipGetLastError();
+hipError_t launch_adjoint(const DevScene& s, const std::vector<AdjointView>& in, const AdjointGrads& g,
+                          DevCounters* ctr, bool count, hipStream_t stream) {
+    if (in.empty()) return hipSuccess;
+    std::vector<AdjView> views(in.size());
+    uint32_t tiles = 0;
+    for (size_t i = 0; i < in.size(); ++i) {
+        AdjView& v = views[i];
+        std::memset(&v, 0, sizeof(v));
+        v.p = in[i].params;
+        const uint32_t tx = static_cast<uint32_t>((v.p.width + kTile - 1) / kTile);
+        const uint32_t ty = static_cast<uint32_t>((v.p.height + kTile - 1) / kTile);
+        v.p.tiles_x = static_cast<int32_t>(tx);
+        v.img_grad = in[i].img_grad;
+        v.replay = in[i].replay;
+        v.tile_offset = tiles;
+        tiles += tx * ty;
+    }
+    AdjView* d_views = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_views), sizeof(AdjView) * views.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(d_views, views.data(), sizeof(AdjView) * views.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) {
+        dim3 grid(tiles), block(kThreads);
+        const uint32_t nv = static_cast<uint32_t>(views.size());
+        if (count)
+            hipLaunchKernelGGL(k_adjoint<true>, grid, block, 0, stream, s, d_views, nv, g, ctr);
+        else
+            hipLaunchKernelGGL(k_adjoint<false>, grid, block, 0, stream, s, d_views, nv, g, ctr);
+        e = hipGetLastError();
+    }
+    const hipError_t es = hipStreamSynchronize(stream);
+    (void)hipFree(d_views);
+    return e != hipSuccess ? e : es;
 }
 
 }  // namespace ptd

@@ -224,6 +224,12 @@ __device__ __forceinline__ S g2_smith(float nol, float nov, S alpha) {  // :31-3
     S gl = nov * ssqrt(a2 + (1.0f - a2) * nol * nol);
     return 2.0f * nol * nov / (gv + gl + 1e-7f);
 }
+// x^5 by multiplication (the reference writes powf(x, 5.0f); the two agree
+// to a few ulp and the product needs no log/exp)
+__device__ __forceinline__ float pow5(float x) {
+    float x2 = x * x;
+    return x2 * x2 * x;
+}
 __device__ __forceinline__ float schlick_t5(float cos_theta) {  // :39-51
     float t = clampf(1.0f - cos_theta, 0.0f, 1.0f);
     float t2 = t * t;
@@ -408,7 +414,7 @@ __device__ __forceinline__ BsdfEvalT<S> eval_bsdf(const MatEval<S>& m, F3 wo, F3
     S diff_w = (1.0f - m.metallic) * (1.0f - m.spec_trans);
     if (sval(diff_w) > 0.0f) {  // evalDisneyDiffuse, :135-159
         S fd90 = 0.5f + 2.0f * hol * hol * m.roughness;
-        float pl = powf(1.0f - nol, 5.0f), pv = powf(1.0f - nov, 5.0f);
+        float pl = pow5(1.0f - nol), pv = pow5(1.0f - nov);
         S fl = 1.0f + (fd90 - 1.0f) * pl;
         S fv = 1.0f + (fd90 - 1.0f) * pv;
         S dk = kInvPi * fl * fv;
@@ -431,7 +437,7 @@ __device__ __forceinline__ BsdfEvalT<S> eval_bsdf(const MatEval<S>& m, F3 wo, F3
         S ot = 1.0f - m.sheen_tint;
         SV3<S> bs = sv3(ot * 1.0f + m.sheen_tint * tint.x, ot * 1.0f + m.sheen_tint * tint.y,
                         ot * 1.0f + m.sheen_tint * tint.z);
-        float fh = powf(1.0f - hol, 5.0f);
+        float fh = pow5(1.0f - hol);
         S om = 1.0f - m.metallic;
         ft = sv3(ft.x + om * (m.sheen * (bs.x * m.sheen_color.x) * fh),
                  ft.y + om * (m.sheen * (bs.y * m.sheen_color.y) * fh),

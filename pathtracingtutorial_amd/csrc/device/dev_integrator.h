@@ -44,25 +44,33 @@ __device__ __forceinline__ F3 uv_to_dir(float u, float v) {
     return normalize(f3(cosf(phi) * st, cosf(theta), sinf(phi) * st));
 }
 __device__ __forceinline__ bool has_env_texture(const DevScene& s) { return s.env != nullptr; }
-// sampleEnvironmentRadiance (envmap.h:34-45): bilinear lookup times 2 pi.
+// sampleEnvironmentRadiance (envmap.h:34-45): bilinear lookup times 2 pi, at
+// the direction's equirect coordinates (u, v) = dir_to_uv(d).
+__device__ __forceinline__ F3 env_radiance_uv(const DevScene& s, float u, float v) {
+    if (s.env == nullptr) return f3(0.0f);
+    float4 c = tex_bilinear(s.env, s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
+    return f3(c.x, c.y, c.z) * kTwoPi;
+}
 __device__ __forceinline__ F3 env_radiance(const DevScene& s, F3 d) {
     if (s.env == nullptr) return f3(0.0f);
     float u, v;
     dir_to_uv(d, u, v);
-    float4 c = tex_bilinear(s.env, s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
-    return f3(c.x, c.y, c.z) * kTwoPi;
+    return env_radiance_uv(s, u, v);
 }
-// environmentPdf (light_pdf.cuh:12-27)
-__device__ __forceinline__ float env_pdf(const DevScene& s, F3 d) {
+// environmentPdf (light_pdf.cuh:12-27) of direction d with (u, v) = dir_to_uv(d)
+__device__ __forceinline__ float env_pdf_uv(const DevScene& s, F3 d, float u, float v) {
     if (s.env == nullptr || s.env_pmf == nullptr || s.num_env_texels == 0 || s.env_w <= 0 || s.env_h <= 0) return 0.0f;
-    float u, v;
-    dir_to_uv(d, u, v);
     int x = static_cast<int>(clampf(u * s.env_w, 0.0f, static_cast<float>(s.env_w - 1)));
     int y = static_cast<int>(clampf(v * s.env_h, 0.0f, static_cast<float>(s.env_h - 1)));
     float st = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
     if (st <= 1e-5f) return 0.0f;
     int idx = y * s.env_w + x;
     return s.env_pmf[idx] * static_cast<float>(s.env_w * s.env_h) / (kTwoPi * kPi * st);
+}
+__device__ __forceinline__ float env_pdf(const DevScene& s, F3 d) {
+    float u, v;
+    dir_to_uv(d, u, v);
+    return env_pdf_uv(s, d, u, v);
 }
 
 // ---- lights ---------------------------------------------------------------
@@ -169,11 +177,13 @@ __device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3
     F3 wi = uv_to_dir(u, v);
     float cs = fmaxf(0.0f, dot(n, wi));
     if (cs <= 0.0f) return ls;
-    float pdf = env_pdf(s, wi);
+    float wu, wv;  // the equirect coordinates of wi, shared by the pdf and the radiance lookup
+    dir_to_uv(wi, wu, wv);
+    float pdf = env_pdf_uv(s, wi, wu, wv);
     if (pdf <= 0.0f) return ls;
     sh = make_shadow(p, n, wi, kInf);
     ls.wi = wi;
-    ls.radiance = env_radiance(s, wi);
+    ls.radiance = env_radiance_uv(s, wu, wv);
     ls.pdf = pdf;
     ls.valid = true;
     return ls;
@@ -398,10 +408,12 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
                                        Sampler& rng, Nee& nee) {
     const int depth = st.depth;
     if (h.tri == 0xFFFFFFFFu) {
-        F3 env = env_radiance(s, st.d);
+        float eu = 0.0f, ev = 0.0f;
+        if (s.env != nullptr) dir_to_uv(st.d, eu, ev);
+        F3 env = env_radiance_uv(s, eu, ev);
         if (!is_zero(env)) {
             float mis = 1.0f;
-            if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf(s, st.d));
+            if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf_uv(s, st.d, eu, ev));
             nee.on_env_miss(s, st, mis);
             st.lo += st.tp * env * mis;
         }

@@ -75,10 +75,30 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
     return dot_rn(e2, qvec) * inv_det;
 }
 
-// Closest hit.  stk points at this thread's first LDS stack slot.
-template <bool COUNT>
-__device__ __forceinline__ Hit trace_closest(const DevScene& s, F3 o, F3 d, uint32_t* stk, int stride,
-                                             TraceCounters& cnt, uint32_t& overflow) {
+// Per-thread traversal stack: the first LDS entries live in LDS
+// (stk[level * stride]), deeper ones in global memory
+// (spill[(level - LDS) * spill_stride]); kStackSize entries in all.  With
+// LDS == kStackSize there is no spill area.
+template <int LDS>
+struct TraceStack {
+    uint32_t* stk;
+    int stride;
+    uint32_t* spill;
+    uint32_t spill_stride;
+    __device__ __forceinline__ void put(int level, uint32_t v) const {
+        if (LDS >= kStackSize || level < LDS) stk[level * stride] = v;
+        else spill[static_cast<size_t>(level - LDS) * spill_stride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(int level) const {
+        if (LDS >= kStackSize || level < LDS) return stk[level * stride];
+        return spill[static_cast<size_t>(level - LDS) * spill_stride];
+    }
+};
+
+// Closest hit.
+template <bool COUNT, int LDS>
+__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS>& sk,
+                                               TraceCounters& cnt, uint32_t& overflow) {
     Hit h;
     h.t = kInf;
     h.u = h.v = 0.0f;
@@ -117,7 +137,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, F3 o, F3 d, uint
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < kStackSize) stk[(top++) * stride] = far;
+                if (top < kStackSize) sk.put(top++, far);
                 else overflow++;
                 cur = near;
                 continue;
@@ -126,15 +146,15 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& s, F3 o, F3 d, uint
             if (tb >= 0.0f) { cur = db; continue; }
         }
         if (top == 0) break;
-        cur = stk[(--top) * stride];
+        cur = sk.get(--top);
     }
     return h;
 }
 
 // Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).
-template <bool COUNT>
-__device__ __forceinline__ bool trace_any(const DevScene& s, F3 o, F3 d, float tmax, uint32_t* stk, int stride,
-                                          TraceCounters& cnt, uint32_t& overflow) {
+template <bool COUNT, int LDS>
+__device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax, const TraceStack<LDS>& sk,
+                                            TraceCounters& cnt, uint32_t& overflow) {
     if (s.num_nodes == 0) return false;
     RayInv r = make_ray_inv(o, d);
     float4 rlo = s.nodes[0], rhi = s.nodes[1];
@@ -163,7 +183,7 @@ __device__ __forceinline__ bool trace_any(const DevScene& s, F3 o, F3 d, float t
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < kStackSize) stk[(top++) * stride] = far;
+                if (top < kStackSize) sk.put(top++, far);
                 else overflow++;
                 cur = near;
                 continue;
@@ -172,9 +192,23 @@ __device__ __forceinline__ bool trace_any(const DevScene& s, F3 o, F3 d, float t
             if (tb >= 0.0f) { cur = db; continue; }
         }
         if (top == 0) break;
-        cur = stk[(--top) * stride];
+        cur = sk.get(--top);
     }
     return false;
+}
+
+// The all-LDS forms (stk: this thread's first LDS slot, kStackSize entries).
+template <bool COUNT>
+__device__ __forceinline__ Hit trace_closest(const DevScene& s, F3 o, F3 d, uint32_t* stk, int stride,
+                                             TraceCounters& cnt, uint32_t& overflow) {
+    TraceStack<kStackSize> sk{stk, stride, nullptr, 0};
+    return trace_closest_s<COUNT>(s, o, d, sk, cnt, overflow);
+}
+template <bool COUNT>
+__device__ __forceinline__ bool trace_any(const DevScene& s, F3 o, F3 d, float tmax, uint32_t* stk, int stride,
+                                          TraceCounters& cnt, uint32_t& overflow) {
+    TraceStack<kStackSize> sk{stk, stride, nullptr, 0};
+    return trace_any_s<COUNT>(s, o, d, tmax, sk, cnt, overflow);
 }
 
 }  // namespace ptd

@@ -75,6 +75,7 @@ struct WfState {
     uint32_t* blockhist;         // material sort: [segment][part][bin] counts, then offsets
     float4* sum;                 // per global pixel sum over the batch
     uint32_t* ctrl;
+    uint32_t* spill;  // traversal stack entries beyond the LDS part, [level][thread of the launch]
     const JobDesc* jobs;
     uint32_t njobs;
     uint32_t capacity, sp_capacity;
@@ -286,9 +287,15 @@ __global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
     for (uint32_t k = C_CUR_EXT + t; k < C_WORDS; k += blockDim.x) ctrl[k] = 0;
 }
 
-template <bool COUNT, int BLOCK>
+template <int LDS, int BLOCK>
+__device__ __forceinline__ TraceStack<LDS> thread_stack(const WfState& w, uint32_t* lds) {
+    return TraceStack<LDS>{lds + threadIdx.x, BLOCK, w.spill + blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
+}
+
+template <bool COUNT, int BLOCK, int LDS>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ uint32_t stack[LDS * BLOCK];
+    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
             uint32_t p = q[i];
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             float4 o = w.o[p], d = w.d[p];
-            Hit h = trace_closest<COUNT>(s, f3(o), f3(d), stack + threadIdx.x, BLOCK, cnt, ovf);
+            Hit h = trace_closest_s<COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
             w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
             rays++;
         }
@@ -398,26 +405,33 @@ __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot,
     }
 }
 
-// Visibility policy of the wavefront: area and environment shadow rays are
-// kept in registers and appended after the bounce (convergently); punctual
-// shadow rays are appended at once.
+// Visibility policy of the wavefront: shadow rays go to the queues at once.
+// Area and environment rays are appended with one atomic per wave (the lanes
+// that reach this point; each path adds at most one entry per queue) to the
+// segment the path's wave is working on; punctual rays with one atomic each.
 struct DeferNee {
-    bool va = false, ve = false;
-    ShadowRay sa, se;
-    F3 ca, ce;
     uint32_t path;
+    uint32_t base;  // first slot of the segment in the shadow queues
+    uint32_t *sa_count, *se_count;
+    float4 *sa_o, *sa_d, *sa_c, *se_o, *se_d, *se_c;
     uint32_t* sp_count;
     uint32_t sp_capacity;
     float4 *sp_o, *sp_d, *sp_c;
     __device__ __forceinline__ void direct(const DevScene&, PathState&, bool a, const ShadowRay& ra, F3 xa, bool e,
                                            const ShadowRay& re, F3 xe, const LightSample&, const LightSample&,
                                            const ShadeCtx&) {
-        va = a;
-        sa = ra;
-        ca = xa;
-        ve = e;
-        se = re;
-        ce = xe;
+        const uint32_t ka = base + wave_append(sa_count, a);
+        if (a) {
+            sa_o[ka] = make_float4(ra.o.x, ra.o.y, ra.o.z, ra.tmax);
+            sa_d[ka] = make_float4(ra.d.x, ra.d.y, ra.d.z, __uint_as_float(path));
+            sa_c[ka] = make_float4(xa.x, xa.y, xa.z, 0.0f);
+        }
+        const uint32_t ke = base + wave_append(se_count, e);
+        if (e) {
+            se_o[ke] = make_float4(re.o.x, re.o.y, re.o.z, re.tmax);
+            se_d[ke] = make_float4(re.d.x, re.d.y, re.d.z, __uint_as_float(path));
+            se_c[ke] = make_float4(xe.x, xe.y, xe.z, 0.0f);
+        }
     }
     __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState&, float) {}
     __device__ __forceinline__ void on_bsdf_sample(const DevScene&, const PathState&, const BsdfSample&,
@@ -463,6 +477,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         uint4 r = make_uint4(0, 0, 0, 0);
         bool cont = false;
         DeferNee nee;
+        nee.base = seg_base(n, seg);
+        nee.sa_count = &w.ctrl[C_SACNT + seg];
+        nee.se_count = &w.ctrl[C_SECNT + seg];
+        nee.sa_o = w.sa_o;
+        nee.sa_d = w.sa_d;
+        nee.sa_c = w.sa_c;
+        nee.se_o = w.se_o;
+        nee.se_d = w.se_d;
+        nee.se_c = w.se_c;
         nee.sp_count = &w.ctrl[C_SP];
         nee.sp_capacity = w.sp_capacity;
         nee.sp_o = w.sp_o;
@@ -489,22 +512,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
                 store_path(w, p, st, r);
             }
         }
-        // everything this wave spawns goes to the segment its items came from
-        const uint32_t base = seg_base(n, seg);
-        const bool pa = active && nee.va, pe = active && nee.ve;
-        uint32_t ka = base + wave_append(&w.ctrl[C_SACNT + seg], pa);
-        uint32_t ke = base + wave_append(&w.ctrl[C_SECNT + seg], pe);
-        uint32_t ko = base + wave_append(&out_counts[seg], cont);
-        if (pa) {
-            w.sa_o[ka] = make_float4(nee.sa.o.x, nee.sa.o.y, nee.sa.o.z, nee.sa.tmax);
-            w.sa_d[ka] = make_float4(nee.sa.d.x, nee.sa.d.y, nee.sa.d.z, __uint_as_float(p));
-            w.sa_c[ka] = make_float4(nee.ca.x, nee.ca.y, nee.ca.z, 0.0f);
-        }
-        if (pe) {
-            w.se_o[ke] = make_float4(nee.se.o.x, nee.se.o.y, nee.se.o.z, nee.se.tmax);
-            w.se_d[ke] = make_float4(nee.se.d.x, nee.se.d.y, nee.se.d.z, __uint_as_float(p));
-            w.se_c[ke] = make_float4(nee.ce.x, nee.ce.y, nee.ce.z, 0.0f);
-        }
+        // the continuation goes to the segment its item came from
+        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg], cont);
         if (cont) qout[ko] = p;
     }
 }
@@ -522,11 +531,12 @@ __device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, flo
 
 // Any-hit over a segmented shadow queue (area or environment): each path has
 // at most one entry, so the contribution is added without atomics.
-template <bool COUNT, int BLOCK>
+template <bool COUNT, int BLOCK, int LDS>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, int cur_slot,
                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                      const float4* __restrict__ qc, DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ uint32_t stack[LDS * BLOCK];
+    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
@@ -541,8 +551,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
             const uint32_t p = __float_as_uint(d.w);
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
-                add_unoccluded(w, p, qc[i], false);
+            if (!trace_any_s<COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, qc[i], false);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -550,9 +559,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
 
 // Any-hit over the flat punctual-light queue (several entries per path:
 // contributions are added with float atomics).
-template <bool COUNT, int BLOCK>
+template <bool COUNT, int BLOCK, int LDS>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ uint32_t stack[LDS * BLOCK];
+    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
     int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
     uint32_t i, end;
@@ -565,8 +575,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCoun
             const uint32_t p = __float_as_uint(d.w);
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any<COUNT>(s, f3(o), f3(d), o.w, stack + threadIdx.x, BLOCK, cnt, ovf))
-                add_unoccluded(w, p, w.sp_c[i], true);
+            if (!trace_any_s<COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, w.sp_c[i], true);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -682,6 +691,7 @@ public:
     uint32_t stage_launches[WF_STAGES] = {};
     JobDesc* d_jobs = nullptr;
     uint32_t job_capacity = 0;
+    size_t spill_capacity = 0;
     uint32_t capacity = 0, sp_capacity = 0, pixels = 0, nbins = 0;
     int num_cus = 0;
     // graph cache
@@ -701,12 +711,25 @@ public:
         release_graph();
         void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
-                      d_jobs};
+                      d_jobs, st.spill};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         st = WfState{};
         d_jobs = nullptr;
         job_capacity = capacity = sp_capacity = pixels = nbins = 0;
+        spill_capacity = 0;
+    }
+
+    bool ensure_spill(size_t entries) {
+        if (entries <= spill_capacity) return true;
+        if (!dalloc(st.spill, entries)) {
+            release();
+            (void)hipGetLastError();
+            return false;
+        }
+        spill_capacity = entries;
+        release_graph();
+        return true;
     }
 
     bool ensure(uint32_t npaths, uint32_t npix, uint32_t nlights, uint32_t nmat, uint32_t njobs) {
@@ -788,6 +811,21 @@ struct Grids {
     int extend, shade;
 };
 
+// The traversal kernels of one stack variant (LDS = stack entries in LDS).
+template <bool COUNT, int EB, int LDS>
+void launch_traversal_extend(const Grids& g, hipStream_t stream, const WfState& w, int in, DevCounters* ctr) {
+    hipLaunchKernelGGL((k_wf_extend<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
+}
+template <bool COUNT, int EB, int LDS>
+void launch_traversal_shadow(const Grids& g, hipStream_t stream, const WfState& w, bool lights, DevCounters* ctr) {
+    hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
+                       w.sa_o, w.sa_d, w.sa_c, ctr);
+    hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
+                       w.se_o, w.se_d, w.se_c, ctr);
+    if (lights)
+        hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
+}
+
 template <bool COUNT, int EB, int SB>
 void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
                    DevCounters* ctr, hipStream_t stream, StageTimer* tm, uint32_t gp0, uint32_t gp1) {
@@ -809,7 +847,8 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_EXTEND);
-            hipLaunchKernelGGL((k_wf_extend<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
+            if (cfg.stack_lds == 16) launch_traversal_extend<COUNT, EB, 16>(g, stream, w, in, ctr);
+            else launch_traversal_extend<COUNT, EB, kStackSize>(g, stream, w, in, ctr);
         }
         if (cs.sort) {
             Timed t(tm, WF_SORT);
@@ -831,12 +870,8 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_SHADOW);
-            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
-                               w.sa_o, w.sa_d, w.sa_c, ctr);
-            hipLaunchKernelGGL((k_wf_shadow<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
-                               w.se_o, w.se_d, w.se_c, ctr);
-            if (cs.lights)
-                hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
+            if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs.lights, ctr);
+            else launch_traversal_shadow<COUNT, EB, kStackSize>(g, stream, w, cs.lights, ctr);
         }
     }
     {
@@ -851,7 +886,13 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
 template <bool COUNT, int EB, int SB>
 Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
     Grids g;
-    g.extend = grid_for(reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB>), EB, wf.num_cus, cfg.waves_per_simd);
+    // extend and shadow kernels share the grid: take the smaller occupancy
+    const void* ext = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, 16>)
+                                          : reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, kStackSize>);
+    const void* shd = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, 16>)
+                                          : reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, kStackSize>);
+    g.extend = std::min(grid_for(ext, EB, wf.num_cus, cfg.waves_per_simd),
+                        grid_for(shd, EB, wf.num_cus, cfg.waves_per_simd));
     const void* shade = cfg.shade_waves == 4   ? reinterpret_cast<const void*>(&k_wf_shade<SB, 4>)
                         : cfg.shade_waves == 3 ? reinterpret_cast<const void*>(&k_wf_shade<SB, 3>)
                                                : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
@@ -898,9 +939,9 @@ struct ChunkEnqueuer {
     hipStream_t stream;
     StageTimer* tm;
     int nchunks;
+    const Grids& g;
     template <int EB, int SB>
     void operator()() {
-        Grids g = grids_for<COUNT, EB, SB>(wf, cfg);
         for (int c = 0; c < nchunks; ++c) {
             const unsigned long long c0 = static_cast<unsigned long long>(c) * wf.st.chunk_paths;
             const unsigned long long c1 = std::min(c0 + wf.st.chunk_paths, wf.st.total_paths);
@@ -1015,23 +1056,30 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
-                        static_cast<uint64_t>(cs.lights), static_cast<uint64_t>(cs.nbins), 0};
+                        static_cast<uint64_t>(cs.lights), static_cast<uint64_t>(cs.nbins),
+                        static_cast<uint64_t>(cfg.stack_lds)};
     StageTimer timer;
     timer.stream = stream;
     StageTimer* tm = cfg.time_stages ? &timer : nullptr;
+    Grids g;
+    if (count) {
+        GridQuery<true> q{*wf, cfg, {}};
+        with_blocks(cfg, q);
+        g = q.out;
+    } else {
+        GridQuery<false> q{*wf, cfg, {}};
+        with_blocks(cfg, q);
+        g = q.out;
+    }
+    // traversal stack entries kept in global memory: one column per thread of the traversal grid
+    if (cfg.stack_lds < kStackSize &&
+        !wf->ensure_spill(static_cast<size_t>(kStackSize - cfg.stack_lds) * g.extend * cfg.extend_block)) {
+        if (err) *err = "wavefront: out of device memory for the traversal stacks";
+        return hipErrorOutOfMemory;
+    }
     if (use_graph) {
         if (!wf->exec || std::memcmp(key, wf->key, sizeof(key)) != 0) {
             wf->release_graph();
-            Grids g;
-            if (count) {
-                GridQuery<true> q{*wf, cfg, {}};
-                with_blocks(cfg, q);
-                g = q.out;
-            } else {
-                GridQuery<false> q{*wf, cfg, {}};
-                with_blocks(cfg, q);
-                g = q.out;
-            }
             hipStream_t cap_stream;
             if ((e = hipStreamCreate(&cap_stream)) != hipSuccess) return e;
             e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
@@ -1057,10 +1105,10 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
         for (int c = 0; c < nchunks; ++c)
             if ((e = hipGraphLaunch(wf->exec, stream)) != hipSuccess) return e;
     } else if (count) {
-        ChunkEnqueuer<true> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<true> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks, g};
         with_blocks(cfg, f);
     } else {
-        ChunkEnqueuer<false> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks};
+        ChunkEnqueuer<false> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks, g};
         with_blocks(cfg, f);
     }
     {

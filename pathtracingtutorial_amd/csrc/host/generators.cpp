@@ -480,4 +480,107 @@ bool generate_batch_job(HostScene& s, uint64_t seed) {
     return true;
 }
 
+// Differentiable multi-view scene: 100 displaced level-5 icospheres
+// (2,048,000 triangles) of radius U[0.25, 0.55] with centres in [-4, 4]^3
+// sharing 16 random Disney materials, a 20x20 ground quad at y = -4.6 with a
+// linear 1024x1024 RGB albedo texture (value noise), and a 1024x512 sky
+// (smooth gradient, no sun).  512x512, 32 spp, depth 10, PCG.  Cameras for
+// the 16-view ring are set by the caller.
+bool generate_diff_views(HostScene& s, uint64_t seed) {
+    init_scene_defaults(s);
+    s.path = "<diff-views>";
+    Rng rng(seed ^ 0x1637ull);
+    for (int m = 0; m < 16; ++m) {
+        pt_material mat = default_material();
+        mat.base_color = v3p(rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f), rng.uniform(0.1f, 0.9f));
+        mat.roughness = rng.uniform(0.2f, 0.9f);
+        mat.metallic = rng.uniform(0.0f, 0.7f);
+        s.materials.push_back(mat);
+    }
+    pt_material ground = default_material();
+    ground.base_color = v3p(1.0f, 1.0f, 1.0f);
+    ground.roughness = 0.8f;
+    ground.base_color_tex_id = 0;
+    s.materials.push_back(ground);  // 16
+    std::vector<P3> unit;
+    std::vector<uint32_t> tris;
+    icosphere(5, unit, tris);
+    const uint32_t ns = 100;
+    const uint32_t noise_seed = static_cast<uint32_t>(rng.next());
+    for (uint32_t i = 0; i < ns; ++i) {
+        const float cx = rng.uniform(-4.0f, 4.0f), cy = rng.uniform(-3.5f, 3.5f), cz = rng.uniform(-4.0f, 4.0f);
+        const float r = rng.uniform(0.25f, 0.55f);
+        const uint32_t base = static_cast<uint32_t>(s.vertices.size());
+        for (const P3& u : unit) {
+            const float disp = 0.1f * r * value_noise(static_cast<float>(u.x) * 3.0f + i, static_cast<float>(u.y) * 3.0f,
+                                                      static_cast<float>(u.z) * 3.0f, noise_seed);
+            const float rr = r + disp;
+            pt_vertex v;
+            std::memset(&v, 0, sizeof(v));
+            v.position = v3p(cx + rr * static_cast<float>(u.x), cy + rr * static_cast<float>(u.y),
+                             cz + rr * static_cast<float>(u.z));
+            v.normal = v3p(static_cast<float>(u.x), static_cast<float>(u.y), static_cast<float>(u.z));
+            s.vertices.push_back(v);
+        }
+        for (size_t t = 0; t + 2 < tris.size(); t += 3)
+            s.faces.push_back({base + tris[t], base + tris[t + 1], base + tris[t + 2], static_cast<int32_t>(i % 16)});
+    }
+    {
+        float a[3] = {-10.0f, -4.6f, 10.0f}, b[3] = {10.0f, -4.6f, 10.0f}, c[3] = {10.0f, -4.6f, -10.0f},
+              d[3] = {-10.0f, -4.6f, -10.0f};
+        build_quad(a, b, c, d, 16, s.vertices, s.faces);
+    }
+    Image tex;
+    tex.width = tex.height = 1024;
+    tex.wrap_s = tex.wrap_t = 10497;
+    tex.linear = 1;
+    tex.rgba.resize(static_cast<size_t>(1024) * 1024 * 4);
+    for (int y = 0; y < 1024; ++y)
+        for (int x = 0; x < 1024; ++x) {
+            float* o = &tex.rgba[(static_cast<size_t>(y) * 1024 + x) * 4];
+            for (int ch = 0; ch < 3; ++ch) {
+                float n = 0.6f * value_noise(x / 96.0f, y / 96.0f, ch * 5.0f, noise_seed + 1) +
+                          0.4f * value_noise(x / 23.0f, y / 23.0f, ch * 5.0f + 2.0f, noise_seed + 1);
+                o[ch] = std::fmin(0.9f, std::fmax(0.1f, 0.5f + 0.45f * n));
+            }
+            o[3] = 1.0f;
+        }
+    s.textures.push_back(std::move(tex));
+    Image& env = s.env;
+    env.width = 1024;
+    env.height = 512;
+    env.wrap_s = env.wrap_t = 10497;
+    env.linear = 1;
+    env.rgba.assign(static_cast<size_t>(1024) * 512 * 4, 0.0f);
+    float top[3], bot[3];
+    for (int ch = 0; ch < 3; ++ch) {
+        top[ch] = rng.uniform(0.5f, 1.5f);
+        bot[ch] = rng.uniform(0.1f, 0.6f);
+    }
+    for (int y = 0; y < 512; ++y)
+        for (int x = 0; x < 1024; ++x) {
+            const float t = (y + 0.5f) / 512.0f;
+            const float wob = 1.0f + 0.2f * std::sin(6.2831853f * (x + 0.5f) / 1024.0f);
+            float* o = &env.rgba[(static_cast<size_t>(y) * 1024 + x) * 4];
+            for (int ch = 0; ch < 3; ++ch) o[ch] = (top[ch] + (bot[ch] - top[ch]) * t) * wob;
+            o[3] = 1.0f;
+        }
+    s.camera = default_camera();
+    s.camera.position = v3p(8.0f, 1.5f, 0.0f);
+    s.camera.look_at = v3p(0.0f, 0.0f, 0.0f);
+    s.camera.up = v3p(0.0f, 1.0f, 0.0f);
+    s.camera.fov = 35.0f;
+    s.camera.width = 512;
+    s.camera.height = 512;
+    s.settings.sample_per_frame = 32;
+    s.settings.target_samples = 32;
+    s.settings.max_depth = 10;
+    s.settings.sampler = 0;
+    s.settings.seed = 0;
+    s.output_file = "diff_views.hdr";
+    s.bvh_opts = {32, 5, 4, 0};
+    s.has_bvh_opts = true;
+    return true;
+}
+
 }  // namespace ptamd

@@ -66,6 +66,7 @@ bool mat4_inverse(const float m[16], float out[16]);
 bool generate_cornell(HostScene& s);
 bool generate_diff_cornell(HostScene& s, uint64_t seed);
 bool generate_batch_job(HostScene& s, uint64_t seed);
+bool generate_diff_views(HostScene& s, uint64_t seed);
 bool generate_spheres(HostScene& s, uint64_t seed, int instances, int level, int env_w, int env_h);
 void generate_sky_env(Image& env, int w, int h, uint64_t seed);
 

@@ -163,3 +163,116 @@ class _Render(torch.autograd.Function):
             t = g[key] if w else None
             outs.append(t.reshape(shape) if t is not None else None)
         return (None, None, *outs)
+
+
+class MultiViewRenderer:
+    """Several cameras of one scene: forward in one wavefront schedule
+    (pt_render_views), backward in one adjoint launch (pt_render_adjoint_views).
+    All cameras must have the same resolution (the views are stacked)."""
+
+    def __init__(self, scene, cameras, texture_id: int = 0):
+        self.base = DiffRenderer(scene, texture_id)
+        self.scene = scene
+        n = len(cameras)
+        assert n > 0 and len({(c.width, c.height) for c in cameras}) == 1
+        self.n = n
+        self.cams = (_capi.Camera * n)(*cameras)
+        self.width, self.height = cameras[0].width, cameras[0].height
+        self.films = []
+        for c in cameras:
+            f = C.c_void_p()
+            check(lib().pt_film_create(c.width, c.height, C.byref(f)))
+            self.films.append(f)
+        self._film_arr = (C.c_void_p * n)(*[f.value for f in self.films])
+        self.last_forward_ms = 0.0
+        self.last_backward_ms = 0.0
+
+    def push(self, *args, **kw):
+        self.base.push(*args, **kw)
+
+    def _frames(self, frame):
+        return (C.c_uint32 * self.n)(*[int(frame) * self.n + i for i in range(self.n)])
+
+    def forward_images(self, frame: int) -> torch.Tensor:
+        ms = C.c_float(0.0)
+        check(lib().pt_render_views(self.scene.handle, self.n, self.cams, self._film_arr, self._frames(frame),
+                                    C.byref(ms)))
+        self.last_forward_ms = ms.value
+        out = torch.empty((self.n, self.height, self.width, 4), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        for i, f in enumerate(self.films):
+            check(lib().pt_film_copy_device(f, 1, C.c_void_p(out[i].data_ptr())))
+        return out
+
+    def adjoint(self, frame: int, image_grads: torch.Tensor, want=(True, True, True, True, True)):
+        g = image_grads.detach().float().contiguous()
+        assert g.is_cuda and tuple(g.shape) == (self.n, self.height, self.width, 4)
+        b = self.base
+        M = b.num_materials
+        z = lambda *s: torch.zeros(s, dtype=torch.float32, device="cuda")
+        d_base = z(M, 3) if want[0] else None
+        d_rough = z(M) if want[1] else None
+        d_metal = z(M) if want[2] else None
+        d_pool = z(max(1, b.pool_texels), 4) if (want[3] and b.texture_id >= 0) else None
+        d_env = z(*b.env_shape) if (want[4] and b.env_shape is not None) else None
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        grads = _capi.AdjointGrads(ptr(d_base), ptr(d_rough), ptr(d_metal), ptr(d_pool), ptr(d_env))
+        gp = (C.c_void_p * self.n)(*[g[i].data_ptr() for i in range(self.n)])
+        torch.cuda.synchronize()
+        ms = C.c_float(0.0)
+        check(lib().pt_render_adjoint_views(self.scene.handle, self.n, self.cams, self._frames(frame), gp,
+                                            C.byref(grads), None, C.byref(ms)))
+        self.last_backward_ms = ms.value
+        d_tex = d_pool[b.tex_offset:b.tex_offset + b.tex_count] if d_pool is not None else None
+        return {"base_color": d_base, "roughness": d_rough, "metallic": d_metal, "texture": d_tex, "env": d_env}
+
+    def render(self, frame: int, base_color=None, roughness=None, metallic=None, texture=None, env=None):
+        """Differentiable (views, H, W, 4) images of batch `frame` (view i uses frame * views + i)."""
+        return _RenderViews.apply(self, int(frame), base_color, roughness, metallic, texture, env)
+
+    def __del__(self):
+        try:
+            for f in self.films:
+                if f:
+                    lib().pt_film_destroy(f)
+        except Exception:
+            pass
+
+
+class _RenderViews(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mv, frame, base_color, roughness, metallic, texture, env):
+        mv.push(base_color, roughness, metallic, texture, env)
+        ctx.mv, ctx.frame = mv, frame
+        ctx.shapes = [None if t is None else tuple(t.shape) for t in (base_color, roughness, metallic, texture, env)]
+        ctx.want = [t is not None and t.requires_grad for t in (base_color, roughness, metallic, texture, env)]
+        return mv.forward_images(frame)
+
+    @staticmethod
+    def backward(ctx, grad):
+        if not any(ctx.want):
+            return (None,) * 7
+        g = ctx.mv.adjoint(ctx.frame, grad, want=ctx.want)
+        outs = []
+        for key, shape, w in zip(("base_color", "roughness", "metallic", "texture", "env"), ctx.shapes, ctx.want):
+            t = g[key] if w else None
+            outs.append(t.reshape(shape) if t is not None else None)
+        return (None, None, *outs)
+
+
+def ring_cameras(n: int = 16, radius: float = 8.0, height: float = 1.5, fov: float = 35.0, size: int = 512):
+    """n cameras on a horizontal ring around the origin, looking at it."""
+    import math
+    cams = []
+    for i in range(n):
+        a = 2.0 * math.pi * i / n
+        c = _capi.Camera()
+        c.position.x, c.position.y, c.position.z = radius * math.cos(a), height, radius * math.sin(a)
+        c.look_at.x = c.look_at.y = c.look_at.z = 0.0
+        c.up.x, c.up.y, c.up.z = 0.0, 1.0, 0.0
+        c.fov = fov
+        c.width = c.height = size
+        c.aperture = 0.0
+        c.focal_distance = radius
+        cams.append(c)
+    return cams

@@ -117,6 +117,8 @@ CONFIGS = [
     dict(extend_block=64, shade_block=256),
     dict(extend_block=256, shade_block=64),
     dict(waves_per_simd=1),
+    dict(stack_lds=16),
+    dict(stack_lds=16, extend_block=256, waves_per_simd=2),
 ]
 
 
@@ -225,3 +227,29 @@ def test_wavefront_bad_config_rejected():
         sm.setWavefrontConfig(extend_block=100)
     with pytest.raises(pt.PtError):
         sm.setWavefrontConfig(chunk_spp=0)
+    with pytest.raises(pt.PtError):
+        sm.setWavefrontConfig(stack_lds=8)
+
+
+def test_wavefront_split_stack_deep_bvh_bit_identical():
+    # One triangle per leaf over 2M triangles: a BVH deeper than the 16 LDS
+    # stack entries, so traversal stacks spill into device memory.
+    sm = pt.SceneManager()
+    sm.generate(5)
+    cam = sm.camera()
+    cam.width, cam.height = 96, 64
+    sm.updateCamera(cam)
+    sm.setSamplePerFrame(2)
+    sm.buildAccel(24, 1, 1)
+    assert sm.info().bvh_depth > 20
+    sm.uploadToDevice()
+    sm.enableCounters(True)
+    out = {}
+    for lds in (32, 16):
+        sm.setWavefrontConfig(stack_lds=lds)
+        sm.uploadToDevice()
+        sm.resetCounters()
+        out[lds] = _render(sm, pt.WavefrontIntegrator(), 1)[0], sm.counters()
+    assert np.array_equal(out[16][0], out[32][0])
+    for k in ("node_visits", "tri_tests", "shadow_node_visits", "shadow_tri_tests", "stack_overflows"):
+        assert out[16][1][k] == out[32][1][k], k
