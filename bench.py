@@ -274,7 +274,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-tune", action="store_true", help="skip the wavefront autotuner (use the default config)")
     ap.add_argument("--no-backward", action="store_true", help="skip the differentiable (backward) measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched measurement")
-    ap.add_argument("--tune-trials", type=int, default=16)
+    ap.add_argument("--tune-trials", type=int, default=24)
     ap.add_argument("--wavefront-config", default="",
                     help='JSON object of wavefront config fields to use instead of autotuning, '
                          'e.g. \'{"chunk_spp": 32, "extend_block": 64}\'')

@@ -190,7 +190,7 @@ enum { PT_INTEGRATOR_PATH = 0, PT_INTEGRATOR_WAVEFRONT = 1 };
  * integrators/path.cu:209-255).  Every field changes speed only, never the
  * image: the estimator and its random numbers are the megakernel's. */
 typedef struct pt_wavefront_config {
-    int32_t chunk_spp;        /* samples per pixel in flight (a divisor of sample_per_frame is used) */
+    int32_t chunk_spp;        /* paths in flight = chunk_spp x all jobs' pixels, at most 2^26 (default 64) */
     int32_t extend_block;     /* threads per block of the traversal kernels: 64, 128 or 256 */
     int32_t shade_block;      /* threads per block of the shading kernel: 64, 128 or 256 */
     int32_t waves_per_simd;   /* persistent grid: waves per SIMD, 0 = occupancy limit */
@@ -199,6 +199,9 @@ typedef struct pt_wavefront_config {
     int32_t time_stages;      /* bracket every launch with HIP events (no graph): pt_scene_wavefront_stats */
     int32_t shade_waves;      /* shading kernel variant built for >= this many waves per SIMD: 0, 3 or 4 */
     int32_t stack_lds;        /* traversal stack entries held in LDS, 16 or 32 (the rest in device memory) */
+    int32_t grab_extend;      /* queue items a wave takes per atomic in the closest-hit kernel: 64..1024, power of 2 */
+    int32_t grab_shade;       /* the same for the shading kernel */
+    int32_t grab_shadow;      /* the same for the shadow-ray kernels */
 } pt_wavefront_config;
 
 /* GPU time (ms) and launch count per stage of the last wavefront batch
@@ -383,12 +386,20 @@ typedef struct pt_adjoint_grads {
  * gradient; FP32 atomics accumulate (last-bit nondeterminism). */
 pt_status pt_render_adjoint(pt_scene* scene, uint32_t frame, const float* image_grad, const pt_adjoint_grads* grads,
                             float* replay_image, float* ms_out);
+/* Form of the adjoint passes: PT_INTEGRATOR_WAVEFRONT (default: per chunk
+ * of the persistent schedule a primal pass, then the replay with
+ * material-sorted queues and shadow rays carrying their sensitivities) or
+ * PT_INTEGRATOR_PATH (one thread per pixel, primal and replay in one kernel).
+ * Both compute the same estimator. */
+pt_status pt_scene_set_adjoint_integrator(pt_scene* scene, int integrator);
+pt_status pt_scene_get_adjoint_integrator(const pt_scene* scene, int* integrator);
 /* Many cameras of one scene (no reference counterpart).  pt_render_views
  * renders one batch per camera into films[i] (sized like cameras[i]) in one
  * wavefront schedule, with the random numbers of frames[i] (NULL: the
  * scene's frame_count for all; frame counts are not advanced).
- * pt_render_adjoint_views replays all views' batches in one launch and
- * accumulates the gradients of all of them into grads. */
+ * pt_render_adjoint_views replays all views' batches in one persistent
+ * schedule (one launch with PT_INTEGRATOR_PATH) and accumulates the
+ * gradients of all of them into grads. */
 pt_status pt_render_views(pt_scene* scene, int n, const pt_camera* cameras, pt_film* const* films,
                           const uint32_t* frames, float* ms_out);
 pt_status pt_render_adjoint_views(pt_scene* scene, int n, const pt_camera* cameras, const uint32_t* frames,

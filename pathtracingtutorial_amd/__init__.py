@@ -204,6 +204,15 @@ class SceneManager:
         return {name: {"ms": float(st.stage_ms[i]), "launches": int(st.launches[i])}
                 for i, name in enumerate(_capi.WAVEFRONT_STAGES)}
 
+    def setAdjointIntegrator(self, integrator: Integrator) -> None:
+        """Form of the path-replay adjoint: WavefrontIntegrator() (default) or PathIntegrator()."""
+        check(lib().pt_scene_set_adjoint_integrator(self.handle, integrator.kind))
+
+    def adjointIntegrator(self) -> str:
+        k = C.c_int(0)
+        check(lib().pt_scene_get_adjoint_integrator(self.handle, C.byref(k)))
+        return "Wavefront" if k.value == _capi.PT_INTEGRATOR_WAVEFRONT else "Path"
+
     def enableCounters(self, on: bool = True) -> None:
         check(lib().pt_counters_enable(self.handle, 1 if on else 0))
 

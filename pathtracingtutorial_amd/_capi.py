@@ -154,7 +154,8 @@ class Counters(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("chunk_spp", C.c_int32), ("extend_block", C.c_int32), ("shade_block", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
-                ("time_stages", C.c_int32), ("shade_waves", C.c_int32), ("stack_lds", C.c_int32)]
+                ("time_stages", C.c_int32), ("shade_waves", C.c_int32), ("stack_lds", C.c_int32),
+                ("grab_extend", C.c_int32), ("grab_shade", C.c_int32), ("grab_shadow", C.c_int32)]
 
 
 class WavefrontStats(C.Structure):
@@ -222,6 +223,8 @@ SIGNATURES = {
     "pt_scene_wavefront_stats": (C.c_int, [_P, C.POINTER(WavefrontStats)]),
     "pt_render_adjoint": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.POINTER(AdjointGrads), C.c_void_p, _FP]),
     "pt_scene_set_frame": (C.c_int, [_P, C.c_uint32]),
+    "pt_scene_set_adjoint_integrator": (C.c_int, [_P, C.c_int]),
+    "pt_scene_get_adjoint_integrator": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "pt_render_views": (C.c_int, [_P, C.c_int, C.POINTER(Camera), C.POINTER(C.c_void_p), _U32P, _FP]),
     "pt_render_adjoint_views": (C.c_int, [_P, C.c_int, C.POINTER(Camera), _U32P, C.POINTER(C.c_void_p),
                                           C.POINTER(AdjointGrads), C.POINTER(C.c_void_p), _FP]),

@@ -134,6 +134,7 @@ struct pt_scene {
     // wavefront integrator state (created at its first batch)
     ptd::Wavefront* wf = nullptr;
     ptd::WavefrontConfig wf_cfg;
+    int adjoint_integrator = PT_INTEGRATOR_WAVEFRONT;
     uint64_t generation = 0;  // bumped by every upload; keys the captured graph
 
     ~pt_scene() {
@@ -340,6 +341,9 @@ void check_wavefront_config(const pt_wavefront_config& c) {
     if (c.shade_waves != 0 && c.shade_waves != 3 && c.shade_waves != 4)
         throw Error(PT_ERR_INVALID_ARG, "shade_waves must be 0, 3 or 4");
     if (c.stack_lds != 16 && c.stack_lds != 32) throw Error(PT_ERR_INVALID_ARG, "stack_lds must be 16 or 32");
+    auto grab_ok = [](int g) { return g >= 64 && g <= 1024 && (g & (g - 1)) == 0; };
+    if (!grab_ok(c.grab_extend) || !grab_ok(c.grab_shade) || !grab_ok(c.grab_shadow))
+        throw Error(PT_ERR_INVALID_ARG, "grab sizes must be 64, 128, 256, 512 or 1024");
 }
 
 ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
@@ -353,6 +357,9 @@ ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
     d.time_stages = c.time_stages ? 1 : 0;
     d.shade_waves = c.shade_waves;
     d.stack_lds = c.stack_lds;
+    d.grab_extend = c.grab_extend;
+    d.grab_shade = c.grab_shade;
+    d.grab_shadow = c.grab_shadow;
     return d;
 }
 
@@ -367,6 +374,9 @@ pt_wavefront_config to_abi_config(const ptd::WavefrontConfig& d) {
     c.time_stages = d.time_stages;
     c.shade_waves = d.shade_waves;
     c.stack_lds = d.stack_lds;
+    c.grab_extend = d.grab_extend;
+    c.grab_shade = d.grab_shade;
+    c.grab_shadow = d.grab_shadow;
     return c;
 }
 
@@ -990,6 +1000,26 @@ ptd::AdjointGrads to_grads(const pt_adjoint_grads* g) {
 
 float run_adjoint(pt_scene* s, const std::vector<ptd::AdjointView>& views, const pt_adjoint_grads* g) {
     ptd::DevScene v = make_view(s);
+    if (s->adjoint_integrator == PT_INTEGRATOR_WAVEFRONT) {
+        std::vector<ptd::WavefrontJob> jobs(views.size());
+        for (size_t i = 0; i < views.size(); ++i) {
+            jobs[i].scene = v;
+            jobs[i].params = views[i].params;
+            jobs[i].film_new = jobs[i].film_acc = nullptr;
+            jobs[i].img_grad = views[i].img_grad;
+            jobs[i].replay = views[i].replay;
+        }
+        if (!s->wf) s->wf = ptd::wavefront_create();
+        float ms = 0.0f;
+        std::string err;
+        hipError_t e = ptd::wavefront_adjoint(s->wf, jobs, to_grads(g), s->wf_cfg, s->counters.p, s->counting,
+                                              s->stream, &ms, &err);
+        if (e == hipErrorOutOfMemory) throw Error(PT_ERR_OUT_OF_MEMORY, err.empty() ? "adjoint: out of memory" : err);
+        if (e != hipSuccess && !err.empty())
+            throw Error(e == hipErrorInvalidValue ? PT_ERR_INVALID_ARG : PT_ERR_DEVICE, err);
+        hip_check(e, "wavefront adjoint");
+        return ms;
+    }
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -1069,6 +1099,22 @@ pt_status pt_render_views(pt_scene* s, int n, const pt_camera* cameras, pt_film*
         }
         float ms = render_wavefront_jobs(s, jobs, s->wf_cfg);
         if (ms_out) *ms_out = ms;
+    });
+}
+
+pt_status pt_scene_set_adjoint_integrator(pt_scene* s, int integrator) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (integrator != PT_INTEGRATOR_PATH && integrator != PT_INTEGRATOR_WAVEFRONT)
+            throw Error(PT_ERR_INVALID_ARG, "unknown integrator");
+        s->adjoint_integrator = integrator;
+    });
+}
+
+pt_status pt_scene_get_adjoint_integrator(const pt_scene* s, int* integrator) {
+    return guarded([&] {
+        if (!s || !integrator) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        *integrator = s->adjoint_integrator;
     });
 }
 
@@ -1227,7 +1273,10 @@ pt_status pt_scene_tune_wavefront(pt_scene* s, int max_trials, pt_wavefront_conf
                     }
                 }
             };
-            try_axis([](ptd::WavefrontConfig& c, int v) { c.chunk_spp = v; }, {2, 4, 8, 16, 32});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_extend = v; }, {64, 128, 256});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_shade = v; }, {128, 256, 512, 1024});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_shadow = v; }, {64, 128});
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.chunk_spp = v; }, {4, 8, 16, 32, 64});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.extend_block = v; }, {64, 128, 256});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.shade_block = v; }, {64, 128, 256});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.sort_by_material = v; }, {0, 1});

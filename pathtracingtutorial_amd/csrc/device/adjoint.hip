@@ -25,8 +25,7 @@
 #include <vector>
 
 #include "adjoint.h"
-#include "dev_dual.h"
-#include "dev_integrator.h"
+#include "adjoint_common.h"
 
 namespace ptd {
 
@@ -47,13 +46,6 @@ struct AdjPath {
     F3 L;  // primal radiance of the path (before the clamp)
     F3 g;  // d loss / d L per channel (zero where the clamp is active)
 };
-
-// Bilinear environment taps of direction d (sampleEnvironmentRadiance's lookup).
-__device__ __forceinline__ TexTaps env_taps(const DevScene& s, F3 d) {
-    float u, v;
-    dir_to_uv(d, u, v);
-    return tex_taps(s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
-}
 
 __device__ __forceinline__ void add_env_grad(const DevScene& s, const AdjointGrads& G, F3 d, F3 coef) {
     if (G.env == nullptr || s.env == nullptr) return;
@@ -85,87 +77,45 @@ struct AdjointNee {
 #pragma unroll
         for (int k = 0; k < kDP; ++k) any = any || gv[k] != 0.0f;
         if (!any) return;
-        const pt_material& m = *x.mat;
         const uint32_t mi = static_cast<uint32_t>(x.mat - s.materials);
-        F3 T = f3(1.0f);
-        const bool has_tex = s.num_textures > 0 && s.textures != nullptr && tex_valid(s, m.base_color_tex_id);
-        float mr_y = 1.0f, mr_z = 1.0f;
-        if (s.num_textures > 0 && s.textures != nullptr && tex_valid(s, m.metallic_roughness_tex_id)) {
-            float4 mr = tex_fetch(s, m.metallic_roughness_tex_id, x.tu, x.tv);
-            mr_y = mr.y;
-            mr_z = mr.z;
-        }
-        if (has_tex) {
-            const TexDesc& td = s.textures[m.base_color_tex_id];
-            TexTaps t = tex_taps(td.w, td.h, td.wrap_s, td.wrap_t, x.tu, x.tv);
-            float4 raw = tex_fetch(s, m.base_color_tex_id, x.tu, x.tv);
-            F3 dec = td.linear ? f3(1.0f)
-                               : f3(2.2f * powf(raw.x, 1.2f), 2.2f * powf(raw.y, 1.2f), 2.2f * powf(raw.z, 1.2f));
-            T = td.linear ? f3(raw.x, raw.y, raw.z) : f3(powf(raw.x, 2.2f), powf(raw.y, 2.2f), powf(raw.z, 2.2f));
-            if (G.tex != nullptr) {
-                const F3 k3 = f3(gv[0] * m.base_color.x * dec.x, gv[1] * m.base_color.y * dec.y,
-                                 gv[2] * m.base_color.z * dec.z);
+        ParamChain pc;
+        param_chain(s, *x.mat, x.tu, x.tv, gv, pc);
+        if (pc.tex && G.tex != nullptr) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (t.w[k] == 0.0f) continue;
-                    float* dst = G.tex + 4 * (static_cast<size_t>(td.offset) + t.idx[k]);
-                    if (k3.x != 0.0f) atomicAdd(dst + 0, k3.x * t.w[k]);
-                    if (k3.y != 0.0f) atomicAdd(dst + 1, k3.y * t.w[k]);
-                    if (k3.z != 0.0f) atomicAdd(dst + 2, k3.z * t.w[k]);
-                }
+            for (int k = 0; k < 4; ++k) {
+                if (pc.tw[k] == 0.0f) continue;
+                float* dst = G.tex + 4 * static_cast<size_t>(pc.tidx[k]);
+                if (pc.k3.x != 0.0f) atomicAdd(dst + 0, pc.k3.x * pc.tw[k]);
+                if (pc.k3.y != 0.0f) atomicAdd(dst + 1, pc.k3.y * pc.tw[k]);
+                if (pc.k3.z != 0.0f) atomicAdd(dst + 2, pc.k3.z * pc.tw[k]);
             }
         }
-        float out[kDP];
-        out[0] = gv[0] * T.x;
-        out[1] = gv[1] * T.y;
-        out[2] = gv[2] * T.z;
-        const float rough_raw = m.roughness * mr_y, metal_raw = m.metallic * mr_z;
-        // the clamps of fetchMaterial (disney_bsdf.cuh:118-120) stop the gradient outside their range
-        out[3] = (rough_raw >= 0.001f && rough_raw <= 1.0f) ? gv[kDRough] * mr_y : 0.0f;
-        out[4] = (metal_raw >= 0.0f && metal_raw <= 1.0f) ? gv[kDMetal] * mr_z : 0.0f;
         if (macc != nullptr && mi < kLdsMaterials) {
 #pragma unroll
             for (int k = 0; k < kDP; ++k)
-                if (out[k] != 0.0f) atomicAdd(&macc[mi * kDP + k], out[k]);
+                if (pc.m[k] != 0.0f) atomicAdd(&macc[mi * kDP + k], pc.m[k]);
         } else {
             if (G.base) {
                 for (int k = 0; k < 3; ++k)
-                    if (out[k] != 0.0f) atomicAdd(&G.base[3 * mi + k], out[k]);
+                    if (pc.m[k] != 0.0f) atomicAdd(&G.base[3 * mi + k], pc.m[k]);
             }
-            if (G.rough && out[3] != 0.0f) atomicAdd(&G.rough[mi], out[3]);
-            if (G.metal && out[4] != 0.0f) atomicAdd(&G.metal[mi], out[4]);
+            if (G.rough && pc.m[3] != 0.0f) atomicAdd(&G.rough[mi], pc.m[3]);
+            if (G.metal && pc.m[4] != 0.0f) atomicAdd(&G.metal[mi], pc.m[4]);
         }
-    }
-
-    // d/dtheta of T * Le * f * mis / (pdf + 1e-7) at the light direction;
-    // environment samples also pass d/dLe to the env texels.
-    __device__ void light_grad(const DevScene& s, const PathState& st, const LightSample& ls, const ShadeCtx& x,
-                               bool env, float gv[kDP]) {
-        BsdfEvalT<D5> bd = eval_bsdf(to_dual(*x.me), x.wo, ls.wi, x.n, true);
-        const float pdf_b = bd.pdf.v;
-        if (pdf_b <= 0.0f) return;
-        const float k = power_heuristic(ls.pdf, pdf_b) / (ls.pdf + 1e-7f);
-        const F3 w = f3(a.g.x * st.tp.x * ls.radiance.x * k, a.g.y * st.tp.y * ls.radiance.y * k,
-                        a.g.z * st.tp.z * ls.radiance.z * k);
-#pragma unroll
-        for (int j = 0; j < kDP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
-        if (env)
-            add_env_grad(s, G, ls.wi,
-                         f3(a.g.x * st.tp.x * bd.f.x.v * k, a.g.y * st.tp.y * bd.f.y.v * k,
-                            a.g.z * st.tp.z * bd.f.z.v * k));
     }
 
     __device__ void direct(const DevScene& s, PathState& st, bool va, const ShadowRay& sa, F3 ca, bool ve,
                            const ShadowRay& se, F3 ce, const LightSample& la, const LightSample& le,
                            const ShadeCtx& x) {
         float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        F3 ec;
         if (va && !occluded<COUNT>(s, sa, c)) {
             st.lo += ca;
-            light_grad(s, st, la, x, false, gv);
+            light_grad(st, a.g, la, x, gv, ec);
         }
         if (ve && !occluded<COUNT>(s, se, c)) {
             st.lo += ce;
-            light_grad(s, st, le, x, true, gv);
+            if (light_grad(st, a.g, le, x, gv, ec)) add_env_grad(s, G, le.wi, ec);
         }
         flush(s, x, gv);
     }
@@ -181,10 +131,7 @@ struct AdjointNee {
             if (occluded<COUNT>(s, make_shadow(x.p, x.n, wi, dist), c)) continue;
             BsdfEvalT<float> b = eval_bsdf(*x.me, x.wo, wi, x.n, true);
             ld += inten * f3(b.f.x, b.f.y, b.f.z);
-            BsdfEvalT<D5> bd = eval_bsdf(to_dual(*x.me), x.wo, wi, x.n, true);
-            const F3 w = f3(a.g.x * st.tp.x * inten.x, a.g.y * st.tp.y * inten.y, a.g.z * st.tp.z * inten.z);
-#pragma unroll
-            for (int j = 0; j < kDP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
+            punctual_grad(st, a.g, wi, inten, x, gv);
         }
         st.lo += st.tp * ld;
         flush(s, x, gv);
@@ -199,20 +146,7 @@ struct AdjointNee {
         const F3 coef = f3(a.g.x * (a.L.x - st.lo.x), a.g.y * (a.L.y - st.lo.y), a.g.z * (a.L.z - st.lo.z));
         if (coef.x == 0.0f && coef.y == 0.0f && coef.z == 0.0f) return;
         float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        const MatEval<D5> md = to_dual(*x.me);
-        SV3<D5> f;
-        if (bs.delta) {  // disney_bsdf.cuh:391-403: the analytic weight
-            if (dot(x.n, bs.wi) < 0.0f) {
-                f = md.base;
-            } else {
-                SV3<D5> f0 = compute_f0(md);
-                F3 H = normalize(x.wo + bs.wi);
-                float t5 = schlick_t5(fmaxf(dot(H, bs.wi), 0.0f));
-                f = sv3(f0.x + (1.0f - f0.x) * t5, f0.y + (1.0f - f0.y) * t5, f0.z + (1.0f - f0.z) * t5);
-            }
-        } else {
-            f = eval_bsdf(md, x.wo, bs.wi, x.n, x.entering).f;
-        }
+        const SV3<D5> f = continuation_f(x, bs);
         const float rx = f.x.v != 0.0f ? coef.x / f.x.v : 0.0f;
         const float ry = f.y.v != 0.0f ? coef.y / f.y.v : 0.0f;
         const float rz = f.z.v != 0.0f ? coef.z / f.z.v : 0.0f;

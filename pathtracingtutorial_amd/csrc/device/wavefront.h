@@ -7,12 +7,13 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.h"
 #include "dev_scene.h"
 
 namespace ptd {
 
 struct WavefrontConfig {
-    int chunk_spp = 8;          // paths in flight = chunk_spp x (pixels of all jobs)
+    int chunk_spp = 64;         // paths in flight = chunk_spp x (pixels of all jobs), at most 2^26
     int extend_block = 128;     // threads per block of the traversal kernels
     int shade_block = 128;      // threads per block of the shading kernel
     int waves_per_simd = 0;     // persistent grid size: 0 = from the occupancy calculator
@@ -21,6 +22,12 @@ struct WavefrontConfig {
     int time_stages = 0;        // bracket every launch with HIP events (no graph); see wavefront_stage_times
     int shade_waves = 0;        // shading kernel built for at least this many waves per SIMD (0, 3 or 4)
     int stack_lds = 32;         // traversal stack entries in LDS (16 or 32); the rest live in device memory
+    // queue items a wave takes per atomic (64 .. 1024, a power of two) in the
+    // closest-hit, shading and shadow kernels: larger grabs cost fewer
+    // same-address atomics, smaller ones keep the rays in flight coherent
+    int grab_extend = 128;
+    int grab_shade = 512;
+    int grab_shadow = 64;
 };
 
 // Stages timed when WavefrontConfig::time_stages is set.
@@ -40,6 +47,8 @@ struct WavefrontJob {
     RenderParams params;  // params.spf samples per pixel of frame params.frame_count
     float4* film_new;
     float4* film_acc;
+    const float4* img_grad = nullptr;  // adjoint only: d loss / d batch mean image (W*H float4)
+    float4* replay = nullptr;          // adjoint only: receives the replay pass's batch mean, or null
 };
 
 class Wavefront;
@@ -53,6 +62,13 @@ void wavefront_destroy(Wavefront* w);
 // (batch mean + running mean).  ms = GPU time from HIP events on `stream`.
 hipError_t wavefront_render(Wavefront* w, const std::vector<WavefrontJob>& jobs, const WavefrontConfig& cfg,
                             DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err);
+// Path-replay adjoint of one batch of every job (views of one scene, each
+// with its img_grad) in the persistent schedule: per chunk the primal pass
+// (each path's radiance), then the replay from the same sampler state, which
+// accumulates d loss / d parameter into g.  Films are not touched.
+hipError_t wavefront_adjoint(Wavefront* w, const std::vector<WavefrontJob>& jobs, const AdjointGrads& g,
+                             const WavefrontConfig& cfg, DevCounters* ctr, bool count, hipStream_t stream, float* ms,
+                             std::string* err);
 // GPU milliseconds and launch counts per stage of the last batch rendered
 // with time_stages set (zeros otherwise).
 void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches);

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <vector>
 
+#include "adjoint_common.h"
 #include "dev_integrator.h"
 #include "wavefront.h"
 
@@ -55,6 +56,8 @@ struct JobDesc {
     RenderParams params;
     float4* film_new;
     float4* film_acc;
+    const float4* img_grad;          // adjoint: d loss / d batch mean image
+    float4* replay;                  // adjoint: batch mean of the replay pass, or null
     unsigned long long path_offset;  // first global path
     uint32_t pixel_offset;           // first global pixel (per-pixel sums)
     uint32_t npix;
@@ -76,10 +79,17 @@ struct WfState {
     float4* sum;                 // per global pixel sum over the batch
     uint32_t* ctrl;
     uint32_t* spill;  // traversal stack entries beyond the LDS part, [level][thread of the launch]
+    // adjoint (replay) state
+    float4* adj_L;                // primal radiance of each path
+    float4* adj_g;                // d loss / d path radiance (zero above the clamp)
+    float4* pend;                 // kPend per path: the last continuation's sensitivities
+    float4 *sa_x, *se_x, *sp_x;   // kPayload per shadow entry: its NEE sensitivities
+    const AdjointGrads* grads;    // parameter gradient buffers (device copy)
     const JobDesc* jobs;
     uint32_t njobs;
     uint32_t capacity, sp_capacity;
     uint32_t chunk_paths;              // C
+    uint32_t grab_extend, grab_shade, grab_shadow;  // queue items a wave takes per atomic
     unsigned long long total_paths;    // all jobs' paths of the batch
     uint32_t total_pixels;
 };
@@ -134,46 +144,48 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + static_cast<uint32_t>(__popcll(below));
 }
 
-// Grabs the next 64 items of a segmented queue for this wave: its own
-// segment first, then the others.  Segment k holds [base(k), base(k) +
-// counts[k]).  Returns false when every segment is exhausted.
-__device__ __forceinline__ bool grab_seg(uint32_t* cursors, const uint32_t* counts, uint32_t n, int& seg,
-                                         int& tried, uint32_t& i, uint32_t& end) {
-    while (tried < kSeg) {
-        uint32_t s0 = seg_base(n, seg);
-        uint32_t s1 = s0 + counts[seg];
-        uint32_t off = 0;
-        if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
-        off = __shfl(off, 0, 64);
-        if (s0 + off < s1) {
-            i = s0 + off + static_cast<uint32_t>(lane_id());
-            end = s1;
-            return true;
-        }
-        seg = (seg + 1) % kSeg;
-        ++tried;
-    }
-    return false;
-}
+// A wave's walk over a queue split in kSeg segments (one per XCD): it grabs
+// `step` items (a multiple of 64) with one atomic on the segment's cursor,
+// its own segment first, then the others, and hands them out 64 at a time.
+// Segment k holds [base(k), base(k) + counts[k]) (counts == null: a flat
+// queue of n items in kSeg equal ranges).  Grabbing several rows of 64 per
+// atomic keeps the same-address atomics on the 8 cursors, which the device
+// serializes, off the critical path.
+struct QueueWalk {
+    uint32_t* cursors;
+    const uint32_t* counts;
+    uint32_t n, step;
+    int seg, tried;
+    uint32_t cur, lim;  // the wave's current grab [cur, lim)
 
-// The same over a flat queue of n items split in kSeg equal ranges.
-__device__ __forceinline__ bool grab_flat(uint32_t* cursors, uint32_t n, int& seg, int& tried, uint32_t& i,
-                                          uint32_t& end) {
-    while (tried < kSeg) {
-        uint32_t s0 = seg_base(n, seg), s1 = seg_base(n, seg + 1);
-        uint32_t off = 0;
-        if (lane_id() == 0) off = atomicAdd(&cursors[seg], 64u);
-        off = __shfl(off, 0, 64);
-        if (s0 + off < s1) {
-            i = s0 + off + static_cast<uint32_t>(lane_id());
-            end = s1;
-            return true;
+    __device__ __forceinline__ QueueWalk(uint32_t* c, const uint32_t* cnt, uint32_t items, uint32_t grab)
+        : cursors(c), counts(cnt), n(items), step(grab), seg(static_cast<int>(blockIdx.x % kSeg)), tried(0), cur(0),
+          lim(0) {}
+
+    // The next 64 items of the wave: lane item i, valid while i < end.
+    // Returns false (for the whole wave) once every segment is drained.
+    __device__ __forceinline__ bool next(uint32_t& i, uint32_t& end) {
+        while (cur >= lim) {
+            if (tried >= kSeg) return false;
+            const uint32_t s0 = seg_base(n, seg);
+            const uint32_t s1 = counts ? s0 + counts[seg] : seg_base(n, seg + 1);
+            uint32_t off = 0;
+            if (lane_id() == 0) off = atomicAdd(&cursors[seg], step);
+            off = __shfl(off, 0, 64);
+            if (s0 + off < s1) {
+                cur = s0 + off;
+                lim = min(s1, cur + step);
+            } else {
+                seg = (seg + 1) % kSeg;
+                ++tried;
+            }
         }
-        seg = (seg + 1) % kSeg;
-        ++tried;
+        i = cur + static_cast<uint32_t>(lane_id());
+        end = lim;
+        cur += 64;
+        return true;
     }
-    return false;
-}
+};
 
 __device__ __forceinline__ void load_path(const WfState& w, uint32_t p, PathState& st, uint4& r) {
     float4 o = w.o[p], d = w.d[p], tp = w.tp[p], lo = w.lo[p];
@@ -300,12 +312,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = w.q[in_slot];
-    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
+    QueueWalk qw(w.ctrl + C_CUR_EXT, counts, n, w.grab_extend);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab_seg(w.ctrl + C_CUR_EXT, counts, n, seg, tried, i, end)) {
+    while (qw.next(i, end)) {
         if (i < end) {
             uint32_t p = q[i];
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
@@ -468,9 +480,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
-    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
+    QueueWalk qw(w.ctrl + C_CUR_SHADE, counts, n, w.grab_shade);
     uint32_t i, end;
-    while (grab_seg(w.ctrl + C_CUR_SHADE, counts, n, seg, tried, i, end)) {
+    while (qw.next(i, end)) {
+        const int seg = qw.seg;
         const bool active = i < end;
         uint32_t p = 0;
         PathState st;
@@ -540,12 +553,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
-    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
+    QueueWalk qw(w.ctrl + cur_slot, counts, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab_seg(w.ctrl + cur_slot, counts, n, seg, tried, i, end)) {
+    while (qw.next(i, end)) {
         if (i < end) {
             float4 o = qo[i], d = qd[i];
             const uint32_t p = __float_as_uint(d.w);
@@ -564,12 +577,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCoun
     __shared__ uint32_t stack[LDS * BLOCK];
     const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
-    int seg = static_cast<int>(blockIdx.x % kSeg), tried = 0;
+    QueueWalk qw(w.ctrl + C_CUR_SP, nullptr, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
-    while (grab_flat(w.ctrl + C_CUR_SP, n, seg, tried, i, end)) {
+    while (qw.next(i, end)) {
         if (i < end) {
             float4 o = w.sp_o[i], d = w.sp_d[i];
             const uint32_t p = __float_as_uint(d.w);
@@ -618,6 +631,456 @@ __global__ void k_wf_film(WfState w, DevCounters* ctr, uint32_t count_paths) {
     a.w = a.w * (1.0f - inv) + nw.w * inv;
     job.film_acc[px] = a;
     if (count_paths && gp == 0) atomicAdd(&ctr->paths, static_cast<unsigned long long>(w.total_paths));
+}
+
+// ------------------------------------------------------- adjoint (replay)
+// The wavefront form of the path-replay adjoint.  A chunk first runs the
+// primal bounce loop (k_wf_shade: each path's radiance L ends in lo);
+// k_wfa_seed keeps L and the path's adjoint g = d loss / d L; the chunk's
+// paths are then generated again from the same sampler state and replayed
+// by k_wfa_shade and the k_wfa_shadow kernels:
+//   * the replay shade appends NEE shadow rays with their sensitivities
+//     (d/dtheta of T Le f w / p at the light direction and, for environment
+//     samples, the weight of d/dLe); a shadow kernel applies them to the
+//     parameter buffers when the ray is unoccluded;
+//   * a miss applies the gradient of T env w to the environment texels;
+//   * a sampled continuation stores rho_c = g_c (df_c/dtheta) / f_c for the
+//     path; the next vertex's shade, when everything the path gathered up to
+//     the previous vertex is in lo (its shadow rays have been resolved),
+//     applies (L - lo) . rho: the radiance still to come times the relative
+//     change of the throughput.  A path that ends at a vertex (Russian
+//     roulette, maximum depth) has nothing still to come.
+// Pdfs, MIS weights and Russian-roulette factors are detached, as in the
+// megakernel adjoint.  All jobs are views of one scene.
+
+constexpr int kPend = 5;     // float4 per path: rho[3][5], material | kNoVertex, tu, tv
+constexpr int kPayload = 3;  // float4 per shadow entry: gv[5], material, tu, tv, env weight rgb
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
+constexpr uint32_t kWfLdsMaterials = 64;  // material sensitivities summed in LDS up to this many materials
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add3(float* dst, F3 v) {
+    if (v.x != 0.0f) atomicAdd(dst + 0, v.x);
+    if (v.y != 0.0f) atomicAdd(dst + 1, v.y);
+    if (v.z != 0.0f) atomicAdd(dst + 2, v.z);
+}
+
+// dst[4 idx + 0..2] += v on every lane with `valid`.  Lanes of the wave that
+// add to the same texel are summed first and add with one atomic (a pixel's
+// samples are neighbours in the queues and mostly touch the same texels).
+// Every lane of the wave calls it.
+__device__ void wave_add_texel(float* dst, uint32_t idx, F3 v, bool valid) {
+    unsigned long long pending = __ballot(valid);
+    const int lane = lane_id();
+    for (int round = 0; pending != 0ull && round < 8; ++round) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const uint32_t lidx = __shfl(idx, leader, 64);
+        const bool mine = ((pending >> lane) & 1ull) != 0ull && idx == lidx;
+        const unsigned long long m = __ballot(mine);
+        pending &= ~m;
+        if (__popcll(m) == 1) {
+            if (lane == leader) atomic_add3(dst + 4 * static_cast<size_t>(idx), v);
+            continue;
+        }
+        const F3 sum = f3(wave_sum_f(mine ? v.x : 0.0f), wave_sum_f(mine ? v.y : 0.0f), wave_sum_f(mine ? v.z : 0.0f));
+        if (lane == leader) atomic_add3(dst + 4 * static_cast<size_t>(lidx), sum);
+    }
+    if ((pending >> lane) & 1ull) atomic_add3(dst + 4 * static_cast<size_t>(idx), v);
+}
+
+__device__ __forceinline__ void add_material(float* macc, const AdjointGrads& G, uint32_t mi, const float m[kDP]) {
+    if (mi < kWfLdsMaterials) {
+#pragma unroll
+        for (int k = 0; k < kDP; ++k)
+            if (m[k] != 0.0f) atomicAdd(&macc[mi * kDP + k], m[k]);
+        return;
+    }
+    if (G.base)
+        for (int k = 0; k < 3; ++k)
+            if (m[k] != 0.0f) atomicAdd(&G.base[3 * mi + k], m[k]);
+    if (G.rough && m[kDRough] != 0.0f) atomicAdd(&G.rough[mi], m[kDRough]);
+    if (G.metal && m[kDMetal] != 0.0f) atomicAdd(&G.metal[mi], m[kDMetal]);
+}
+
+__device__ __forceinline__ void clear_materials(float* macc) {
+    for (uint32_t k = threadIdx.x; k < kWfLdsMaterials * kDP; k += blockDim.x) macc[k] = 0.0f;
+    __syncthreads();
+}
+
+// The block's LDS material sums to the parameter buffers (kernel end).
+__device__ __forceinline__ void flush_materials(const float* macc, const AdjointGrads& G, uint32_t nmat) {
+    __syncthreads();
+    const uint32_t n = min(nmat, kWfLdsMaterials) * kDP;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const float v = macc[k];
+        if (v == 0.0f) continue;
+        const uint32_t mi = k / kDP, j = k % kDP;
+        if (j < 3) {
+            if (G.base) atomicAdd(&G.base[3 * mi + j], v);
+        } else if (j == kDRough) {
+            if (G.rough) atomicAdd(&G.rough[mi], v);
+        } else if (G.metal) {
+            atomicAdd(&G.metal[mi], v);
+        }
+    }
+}
+
+// A vertex's sensitivities gv (lanes with `valid`) through the parameter
+// chain to the buffers.  Every lane of the wave calls it.
+__device__ void apply_vertex(const DevScene& s, const AdjointGrads& G, float* macc, bool valid, uint32_t mi,
+                             float tu, float tv, const float gv[kDP]) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < kDP; ++k) any = any || gv[k] != 0.0f;
+    valid = valid && any;
+    ParamChain pc{};
+    if (valid) {
+        param_chain(s, s.materials[mi], tu, tv, gv, pc);
+        add_material(macc, G, mi, pc.m);
+    }
+    const bool tex = valid && pc.tex && G.tex != nullptr;
+    if (__ballot(tex) == 0ull) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        wave_add_texel(G.tex, tex ? pc.tidx[k] : 0u, tex ? pc.k3 * pc.tw[k] : f3(0.0f), tex && pc.tw[k] != 0.0f);
+}
+
+// d/d(env texels) of coef . env(d) (lanes with `valid`).  Every lane of the wave calls it.
+__device__ void apply_env(const DevScene& s, const AdjointGrads& G, bool valid, F3 d, F3 coef) {
+    valid = valid && G.env != nullptr && s.env != nullptr && !is_zero(coef);
+    if (__ballot(valid) == 0ull) return;
+    TexTaps t{};
+    if (valid) t = env_taps(s, d);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool on = valid && t.w[k] != 0.0f;
+        wave_add_texel(G.env, on ? t.idx[k] : 0u, on ? coef * (t.w[k] * kTwoPi) : f3(0.0f), on);
+    }
+}
+
+__global__ void k_wfa_seed(WfState w) {
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long g = c0 + p;
+    const JobDesc& job = w.jobs[job_of_path(w, g)];
+    const uint32_t spf = static_cast<uint32_t>(job.params.spf);
+    const uint32_t pixel = static_cast<uint32_t>((g - job.path_offset) / spf);
+    const float4 L = w.lo[p];
+    const float4 gp = job.img_grad[pixel];
+    const float inv = 1.0f / static_cast<float>(spf);
+    w.adj_L[p] = L;
+    // fminf(L, 100) passes the gradient below the clamp (path.cu:201-204)
+    w.adj_g[p] = make_float4(L.x <= 100.0f ? gp.x * inv : 0.0f, L.y <= 100.0f ? gp.y * inv : 0.0f,
+                             L.z <= 100.0f ? gp.z * inv : 0.0f, 0.0f);
+}
+
+// Visibility policy of the replay: DeferNee's queues, each entry with its
+// sensitivities; the continuation's rho goes to the path's pending record.
+struct AdjDeferNee {
+    uint32_t path;
+    uint32_t base;
+    uint32_t *sa_count, *se_count;
+    float4 *sa_o, *sa_d, *sa_c, *sa_x, *se_o, *se_d, *se_c, *se_x;
+    uint32_t* sp_count;
+    uint32_t sp_capacity;
+    float4 *sp_o, *sp_d, *sp_c, *sp_x;
+    float4* pend;  // this path's kPend records
+    F3 g;
+    bool wrote_pend;
+    bool miss;
+    F3 miss_coef;
+
+    __device__ __forceinline__ static void put_payload(float4* x, const float gv[kDP], uint32_t mi, const ShadeCtx& c,
+                                                       F3 ec) {
+        x[0] = make_float4(gv[0], gv[1], gv[2], gv[3]);
+        x[1] = make_float4(gv[4], __uint_as_float(mi), c.tu, c.tv);
+        x[2] = make_float4(ec.x, ec.y, ec.z, 0.0f);
+    }
+
+    __device__ void direct(const DevScene& s, PathState& st, bool a, const ShadowRay& ra, F3 xa, bool e,
+                           const ShadowRay& re, F3 xe, const LightSample& la, const LightSample& le,
+                           const ShadeCtx& x) {
+        const uint32_t mi = static_cast<uint32_t>(x.mat - s.materials);
+        const uint32_t ka = base + wave_append(sa_count, a);
+        if (a) {
+            sa_o[ka] = make_float4(ra.o.x, ra.o.y, ra.o.z, ra.tmax);
+            sa_d[ka] = make_float4(ra.d.x, ra.d.y, ra.d.z, __uint_as_float(path));
+            sa_c[ka] = make_float4(xa.x, xa.y, xa.z, 0.0f);
+            float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            F3 ec = f3(0.0f);
+            light_grad(st, g, la, x, gv, ec);
+            put_payload(sa_x + kPayload * static_cast<size_t>(ka), gv, mi, x, f3(0.0f));
+        }
+        const uint32_t ke = base + wave_append(se_count, e);
+        if (e) {
+            se_o[ke] = make_float4(re.o.x, re.o.y, re.o.z, re.tmax);
+            se_d[ke] = make_float4(re.d.x, re.d.y, re.d.z, __uint_as_float(path));
+            se_c[ke] = make_float4(xe.x, xe.y, xe.z, 0.0f);
+            float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            F3 ec = f3(0.0f);
+            if (!light_grad(st, g, le, x, gv, ec)) ec = f3(0.0f);
+            put_payload(se_x + kPayload * static_cast<size_t>(ke), gv, mi, x, ec);
+        }
+    }
+    __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState& st, float mis) {
+        miss = true;
+        miss_coef = f3(g.x * st.tp.x * mis, g.y * st.tp.y * mis, g.z * st.tp.z * mis);
+    }
+    __device__ void on_bsdf_sample(const DevScene& s, const PathState&, const BsdfSample& bs, const ShadeCtx& x) {
+        const SV3<D5> f = continuation_f(x, bs);
+        const float rx = f.x.v != 0.0f ? g.x / f.x.v : 0.0f;
+        const float ry = f.y.v != 0.0f ? g.y / f.y.v : 0.0f;
+        const float rz = f.z.v != 0.0f ? g.z / f.z.v : 0.0f;
+        float r[16];
+#pragma unroll
+        for (int j = 0; j < kDP; ++j) {
+            r[j] = rx * f.x.d[j];
+            r[kDP + j] = ry * f.y.d[j];
+            r[2 * kDP + j] = rz * f.z.d[j];
+        }
+        r[15] = __uint_as_float(static_cast<uint32_t>(x.mat - s.materials));
+        pend[0] = make_float4(r[0], r[1], r[2], r[3]);
+        pend[1] = make_float4(r[4], r[5], r[6], r[7]);
+        pend[2] = make_float4(r[8], r[9], r[10], r[11]);
+        pend[3] = make_float4(r[12], r[13], r[14], r[15]);
+        pend[4] = make_float4(x.tu, x.tv, 0.0f, 0.0f);
+        wrote_pend = true;
+    }
+    __device__ void punctual_lights(const DevScene& s, PathState& st, const ShadeCtx& x) {
+        const uint32_t mi = static_cast<uint32_t>(x.mat - s.materials);
+        for (uint32_t i = 0; i < s.num_lights; ++i) {
+            F3 wi, inten;
+            float dist;
+            if (!punctual_light_sample(s, i, x.p, x.n, wi, inten, dist)) continue;
+            BsdfEvalT<float> b = eval_bsdf(*x.me, x.wo, wi, x.n, true);
+            F3 c = st.tp * (inten * f3(b.f.x, b.f.y, b.f.z));
+            if (is_zero(c)) continue;
+            uint32_t k = atomicAdd(sp_count, 1u);
+            if (k >= sp_capacity) continue;  // cannot happen: capacity = paths x lights
+            ShadowRay r = make_shadow(x.p, x.n, wi, dist);
+            sp_o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
+            sp_d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(path));
+            sp_c[k] = make_float4(c.x, c.y, c.z, 0.0f);
+            float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            punctual_grad(st, g, wi, inten, x, gv);
+            put_payload(sp_x + kPayload * static_cast<size_t>(k), gv, mi, x, f3(0.0f));
+        }
+    }
+};
+
+// Built for 2 waves per SIMD (registers + accumulation registers within 256).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_wfa_shade(WfState w,
+                                                                                                 int in_slot,
+                                                                                                 int use_sorted) {
+    __shared__ float macc[kWfLdsMaterials * kDP];
+    clear_materials(macc);
+    const AdjointGrads G = *w.grads;
+    const DevScene& s0 = w.jobs[0].scene;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
+    const int out_slot = in_slot ^ 1;
+    uint32_t* qout = w.q[out_slot];
+    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
+    QueueWalk qw(w.ctrl + C_CUR_SHADE, counts, n, w.grab_shade);
+    uint32_t i, end;
+    while (qw.next(i, end)) {
+        const int seg = qw.seg;
+        const bool active = i < end;
+        uint32_t p = 0;
+        PathState st{};
+        uint4 r = make_uint4(0, 0, 0, 0);
+        bool cont = false;
+        AdjDeferNee nee;
+        nee.base = seg_base(n, seg);
+        nee.sa_count = &w.ctrl[C_SACNT + seg];
+        nee.se_count = &w.ctrl[C_SECNT + seg];
+        nee.sa_o = w.sa_o;
+        nee.sa_d = w.sa_d;
+        nee.sa_c = w.sa_c;
+        nee.sa_x = w.sa_x;
+        nee.se_o = w.se_o;
+        nee.se_d = w.se_d;
+        nee.se_c = w.se_c;
+        nee.se_x = w.se_x;
+        nee.sp_count = &w.ctrl[C_SP];
+        nee.sp_capacity = w.sp_capacity;
+        nee.sp_o = w.sp_o;
+        nee.sp_d = w.sp_d;
+        nee.sp_c = w.sp_c;
+        nee.sp_x = w.sp_x;
+        nee.wrote_pend = false;
+        nee.miss = false;
+        nee.miss_coef = f3(0.0f);
+        // the previous vertex's continuation, applied with (L - lo)
+        bool pv = false;
+        uint32_t pmi = 0;
+        float ptu = 0.0f, ptv = 0.0f;
+        float pgv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (active) {
+            p = q[i];
+            nee.path = p;
+            nee.pend = w.pend + kPend * static_cast<size_t>(p);
+            load_path(w, p, st, r);
+            nee.g = f3(w.adj_g[p]);
+            if (st.depth > 0) {
+                const float4 p3 = nee.pend[3];
+                const uint32_t m = __float_as_uint(p3.w);
+                if (m != kNoVertex) {
+                    const float4 p0 = nee.pend[0], p1 = nee.pend[1], p2 = nee.pend[2], p4 = nee.pend[4];
+                    const float rho[15] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w,
+                                           p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z};
+                    const F3 rem = f3(w.adj_L[p]) - st.lo;
+#pragma unroll
+                    for (int j = 0; j < kDP; ++j)
+                        pgv[j] = rem.x * rho[j] + rem.y * rho[kDP + j] + rem.z * rho[2 * kDP + j];
+                    pv = true;
+                    pmi = m;
+                    ptu = p4.x;
+                    ptv = p4.y;
+                }
+            }
+            const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
+            const RenderParams& rp = job.params;
+            float4 hv = w.hit[p];
+            Hit h;
+            h.t = hv.x;
+            h.u = hv.y;
+            h.v = hv.z;
+            h.tri = __float_as_uint(hv.w);
+            if (st.depth < rp.max_depth) {
+                Sampler rng = sampler_of(rp.sampler, r);
+                cont = bounce(job.scene, rp, h, st, rng, nee);
+                r = sampler_state(rng, r);
+                if (cont) st.depth += 1;
+                cont = cont && st.depth < rp.max_depth;
+                store_path(w, p, st, r);
+                if (cont && !nee.wrote_pend) nee.pend[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoVertex));
+            }
+        }
+        apply_vertex(s0, G, macc, pv, pmi, ptu, ptv, pgv);
+        apply_env(s0, G, nee.miss, st.d, nee.miss_coef);
+        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg], cont);
+        if (cont) qout[ko] = p;
+    }
+    flush_materials(macc, G, s0.num_materials);
+}
+
+// Any-hit over a segmented shadow queue of the replay: an unoccluded ray
+// adds its contribution and applies its sensitivities.
+template <bool COUNT, int BLOCK, bool ENV>
+__global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot, int cur_slot,
+                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                      const float4* __restrict__ qc, const float4* __restrict__ qx,
+                                                      DevCounters* ctr) {
+    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ float macc[kWfLdsMaterials * kDP];
+    clear_materials(macc);
+    const TraceStack<kStackSize> sk = thread_stack<kStackSize, BLOCK>(w, stack);
+    const AdjointGrads G = *w.grads;
+    const DevScene& s0 = w.jobs[0].scene;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t* counts = w.ctrl + count_slot;
+    QueueWalk qw(w.ctrl + cur_slot, counts, n, w.grab_shadow);
+    uint32_t i, end;
+    TraceCounters cnt{0, 0};
+    uint32_t ovf = 0;
+    unsigned long long rays = 0;
+    while (qw.next(i, end)) {
+        bool vis = false;
+        uint32_t mi = 0;
+        float tu = 0.0f, tv = 0.0f;
+        float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        F3 ec = f3(0.0f), dir = f3(0.0f);
+        if (i < end) {
+            float4 o = qo[i], d = qd[i];
+            const uint32_t p = __float_as_uint(d.w);
+            rays++;
+            if (!trace_any_s<COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
+                add_unoccluded(w, p, qc[i], false);
+                const float4 x0 = qx[kPayload * static_cast<size_t>(i)];
+                const float4 x1 = qx[kPayload * static_cast<size_t>(i) + 1];
+                gv[0] = x0.x;
+                gv[1] = x0.y;
+                gv[2] = x0.z;
+                gv[3] = x0.w;
+                gv[4] = x1.x;
+                mi = __float_as_uint(x1.y);
+                tu = x1.z;
+                tv = x1.w;
+                if (ENV) ec = f3(qx[kPayload * static_cast<size_t>(i) + 2]);
+                dir = f3(d);
+                vis = true;
+            }
+        }
+        apply_vertex(s0, G, macc, vis, mi, tu, tv, gv);
+        if (ENV) apply_env(s0, G, vis, dir, ec);
+    }
+    flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
+    flush_materials(macc, G, s0.num_materials);
+}
+
+template <bool COUNT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wfa_shadow_punctual(WfState w, DevCounters* ctr) {
+    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ float macc[kWfLdsMaterials * kDP];
+    clear_materials(macc);
+    const TraceStack<kStackSize> sk = thread_stack<kStackSize, BLOCK>(w, stack);
+    const AdjointGrads G = *w.grads;
+    const DevScene& s0 = w.jobs[0].scene;
+    const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
+    QueueWalk qw(w.ctrl + C_CUR_SP, nullptr, n, w.grab_shadow);
+    uint32_t i, end;
+    TraceCounters cnt{0, 0};
+    uint32_t ovf = 0;
+    unsigned long long rays = 0;
+    while (qw.next(i, end)) {
+        bool vis = false;
+        uint32_t mi = 0;
+        float tu = 0.0f, tv = 0.0f;
+        float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (i < end) {
+            float4 o = w.sp_o[i], d = w.sp_d[i];
+            const uint32_t p = __float_as_uint(d.w);
+            rays++;
+            if (!trace_any_s<COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
+                add_unoccluded(w, p, w.sp_c[i], true);
+                const float4 x0 = w.sp_x[kPayload * static_cast<size_t>(i)];
+                const float4 x1 = w.sp_x[kPayload * static_cast<size_t>(i) + 1];
+                gv[0] = x0.x;
+                gv[1] = x0.y;
+                gv[2] = x0.z;
+                gv[3] = x0.w;
+                gv[4] = x1.x;
+                mi = __float_as_uint(x1.y);
+                tu = x1.z;
+                tv = x1.w;
+                vis = true;
+            }
+        }
+        apply_vertex(s0, G, macc, vis, mi, tu, tv, gv);
+    }
+    flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
+    flush_materials(macc, G, s0.num_materials);
+}
+
+// Batch mean of the replay pass to each job's replay image (when given).
+__global__ void k_wfa_replay_out(WfState w) {
+    const uint32_t gp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= w.total_pixels) return;
+    const JobDesc& job = w.jobs[job_of_pixel(w, gp)];
+    if (job.replay == nullptr) return;
+    const F3 acc = f3(w.sum[gp]) / static_cast<float>(job.params.spf);
+    job.replay[gp - job.pixel_offset] = make_float4(acc.x, acc.y, acc.z, 1.0f);
 }
 
 template <class T>
@@ -693,31 +1156,69 @@ public:
     uint32_t job_capacity = 0;
     size_t spill_capacity = 0;
     uint32_t capacity = 0, sp_capacity = 0, pixels = 0, nbins = 0;
+    uint32_t adj_capacity = 0, adj_sp_capacity = 0;
+    AdjointGrads* d_grads = nullptr;
     int num_cus = 0;
-    // graph cache
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    uint64_t key[12] = {};
+    // graph cache: [0] the forward render, [1] the adjoint (alternating
+    // forward and backward batches keep both)
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    uint64_t key[2][12] = {};
 
     ~Wavefront() { release(); }
 
+    void release_graph(int k) {
+        if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+        if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        exec[k] = nullptr;
+        graph[k] = nullptr;
+    }
     void release_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        release_graph(0);
+        release_graph(1);
     }
     void release() {
         release_graph();
         void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
-                      d_jobs, st.spill};
+                      d_jobs, st.spill, st.adj_L, st.adj_g, st.pend, st.sa_x, st.se_x, st.sp_x, d_grads};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         st = WfState{};
         d_jobs = nullptr;
+        d_grads = nullptr;
         job_capacity = capacity = sp_capacity = pixels = nbins = 0;
+        adj_capacity = adj_sp_capacity = 0;
         spill_capacity = 0;
+    }
+
+    // Replay state of the adjoint for npaths paths in flight (after ensure()).
+    bool ensure_adjoint(uint32_t npaths) {
+        bool ok = true;
+        if (npaths > adj_capacity) {
+            ok = ok && dalloc(st.adj_L, npaths) && dalloc(st.adj_g, npaths) &&
+                 dalloc(st.pend, static_cast<size_t>(kPend) * npaths) &&
+                 dalloc(st.sa_x, static_cast<size_t>(kPayload) * npaths) &&
+                 dalloc(st.se_x, static_cast<size_t>(kPayload) * npaths);
+            adj_capacity = npaths;
+            release_graph();
+        }
+        if (ok && sp_capacity > adj_sp_capacity) {
+            ok = dalloc(st.sp_x, static_cast<size_t>(kPayload) * sp_capacity);
+            adj_sp_capacity = sp_capacity;
+            release_graph();
+        }
+        if (ok && !d_grads) {
+            ok = dalloc(d_grads, 1);
+            release_graph();
+        }
+        if (!ok) {
+            release();
+            (void)hipGetLastError();
+            return false;
+        }
+        st.grads = d_grads;
+        return true;
     }
 
     bool ensure_spill(size_t entries) {
@@ -803,12 +1304,14 @@ struct ChunkShape {
     int max_depth;
     uint32_t nbins;
     bool sort, lights;
+    bool adjoint;  // primal pass, then the replay of the path-replay adjoint
 };
 
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
     int extend, shade;
+    int ashade, ashadow;  // the adjoint's replay kernels
 };
 
 // The traversal kernels of one stack variant (LDS = stack entries in LDS).
@@ -826,9 +1329,21 @@ void launch_traversal_shadow(const Grids& g, hipStream_t stream, const WfState& 
         hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
 }
 
+template <bool COUNT, int EB>
+void launch_replay_shadow(const Grids& g, hipStream_t stream, const WfState& w, bool lights, DevCounters* ctr) {
+    hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, false>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
+                       w.sa_o, w.sa_d, w.sa_c, w.sa_x, ctr);
+    hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, true>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
+                       w.se_o, w.se_d, w.se_c, w.se_x, ctr);
+    if (lights)
+        hipLaunchKernelGGL((k_wfa_shadow_punctual<COUNT, EB>), dim3(g.ashadow), dim3(EB), 0, stream, w, ctr);
+}
+
+// Camera rays and max_depth bounces of the chunk's paths; `replay` runs the
+// adjoint's shading and shadow kernels.
 template <bool COUNT, int EB, int SB>
-void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
-                   DevCounters* ctr, hipStream_t stream, StageTimer* tm, uint32_t gp0, uint32_t gp1) {
+void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
+                   DevCounters* ctr, hipStream_t stream, StageTimer* tm, bool replay) {
     const WfState& w = wf.st;
     const size_t sort_lds = sizeof(uint32_t) * cs.nbins;
     {
@@ -861,7 +1376,9 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         {
             Timed t(tm, WF_SHADE);
             const int us = cs.sort ? 1 : 0;
-            if (cfg.shade_waves == 4)
+            if (replay)
+                hipLaunchKernelGGL((k_wfa_shade<SB>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us);
+            else if (cfg.shade_waves == 4)
                 hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
             else if (cfg.shade_waves == 3)
                 hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
@@ -870,9 +1387,26 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_SHADOW);
-            if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs.lights, ctr);
+            if (replay) launch_replay_shadow<COUNT, EB>(g, stream, w, cs.lights, ctr);
+            else if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs.lights, ctr);
             else launch_traversal_shadow<COUNT, EB, kStackSize>(g, stream, w, cs.lights, ctr);
         }
+    }
+}
+
+// One chunk: its paths (for the adjoint: the primal pass, the seed of the
+// replay and the replay), then the per-pixel sums.
+template <bool COUNT, int EB, int SB>
+void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
+                   DevCounters* ctr, hipStream_t stream, StageTimer* tm, uint32_t gp0, uint32_t gp1) {
+    const WfState& w = wf.st;
+    enqueue_paths<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, tm, false);
+    if (cs.adjoint) {
+        {
+            Timed t(tm, WF_CONTROL);
+            hipLaunchKernelGGL(k_wfa_seed, dim3((w.chunk_paths + 255) / 256), dim3(256), 0, stream, w);
+        }
+        enqueue_paths<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, tm, true);
     }
     {
         Timed t(tm, WF_CONTROL);
@@ -897,6 +1431,11 @@ Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
                         : cfg.shade_waves == 3 ? reinterpret_cast<const void*>(&k_wf_shade<SB, 3>)
                                                : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
     g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
+    g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
+    g.ashadow = std::min(
+        grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true>), EB, wf.num_cus, cfg.waves_per_simd),
+        grid_for(reinterpret_cast<const void*>(&k_wfa_shadow_punctual<COUNT, EB>), EB, wf.num_cus,
+                 cfg.waves_per_simd));
     return g;
 }
 
@@ -981,8 +1520,12 @@ struct CaptureEnqueuer {
 
 }  // namespace
 
-hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, const WavefrontConfig& cfg,
-                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err) {
+namespace {
+
+// One batch of every job: the forward render (G == null) or the adjoint.
+hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, const WavefrontConfig& cfg,
+                     DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err,
+                     const AdjointGrads* G) {
     if (jobs_in.empty()) return hipSuccess;
     // job table: global path and pixel offsets
     std::vector<JobDesc> jobs(jobs_in.size());
@@ -997,6 +1540,12 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
         d.params = in.params;
         d.film_new = in.film_new;
         d.film_acc = in.film_acc;
+        d.img_grad = in.img_grad;
+        d.replay = in.replay;
+        if (G && (in.img_grad == nullptr || std::memcmp(&in.scene, &jobs_in[0].scene, sizeof(DevScene)) != 0)) {
+            if (err) *err = "adjoint jobs need an image gradient each and must be views of one scene";
+            return hipErrorInvalidValue;
+        }
         const unsigned long long npix = static_cast<unsigned long long>(in.params.width) * in.params.height;
         if (in.params.spf <= 0 || npix == 0) {
             if (err) *err = "wavefront job with no pixels or samples";
@@ -1028,12 +1577,16 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
             wf->num_cus = 256;
     }
     const uint32_t njobs = static_cast<uint32_t>(jobs.size());
-    if (!wf->ensure(C, static_cast<uint32_t>(pixels), max_lights, max_mat, njobs)) {
+    if (!wf->ensure(C, static_cast<uint32_t>(pixels), max_lights, max_mat, njobs) ||
+        (G && !wf->ensure_adjoint(C))) {
         if (err) *err = "wavefront: out of device memory for path state";
         return hipErrorOutOfMemory;
     }
     wf->st.njobs = njobs;
     wf->st.chunk_paths = C;
+    wf->st.grab_extend = static_cast<uint32_t>(cfg.grab_extend);
+    wf->st.grab_shade = static_cast<uint32_t>(cfg.grab_shade);
+    wf->st.grab_shadow = static_cast<uint32_t>(cfg.grab_shadow);
     wf->st.total_paths = paths;
     wf->st.total_pixels = static_cast<uint32_t>(pixels);
     ChunkShape cs;
@@ -1041,9 +1594,12 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
     cs.nbins = max_mat + 1;
     cs.sort = cfg.sort_by_material && cs.nbins <= kMaxSortBins;
     cs.lights = max_lights > 0;
+    cs.adjoint = G != nullptr;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
         hipSuccess)
+        return e;
+    if (G && (e = hipMemcpyAsync(wf->d_grads, G, sizeof(AdjointGrads), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return e;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
@@ -1057,7 +1613,9 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
                         static_cast<uint64_t>(cs.lights), static_cast<uint64_t>(cs.nbins),
-                        static_cast<uint64_t>(cfg.stack_lds)};
+                        static_cast<uint64_t>(cfg.stack_lds + (cs.adjoint ? 1000 : 0)) |
+                            (static_cast<uint64_t>(cfg.grab_extend) << 16) | (static_cast<uint64_t>(cfg.grab_shade) << 32) |
+                            (static_cast<uint64_t>(cfg.grab_shadow) << 48)};
     StageTimer timer;
     timer.stream = stream;
     StageTimer* tm = cfg.time_stages ? &timer : nullptr;
@@ -1078,8 +1636,9 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
         return hipErrorOutOfMemory;
     }
     if (use_graph) {
-        if (!wf->exec || std::memcmp(key, wf->key, sizeof(key)) != 0) {
-            wf->release_graph();
+        const int gk = cs.adjoint ? 1 : 0;
+        if (!wf->exec[gk] || std::memcmp(key, wf->key[gk], sizeof(key)) != 0) {
+            wf->release_graph(gk);
             hipStream_t cap_stream;
             if ((e = hipStreamCreate(&cap_stream)) != hipSuccess) return e;
             e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
@@ -1091,19 +1650,19 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
                     CaptureEnqueuer<false> f{*wf, cfg, cs, g, ctr, cap_stream};
                     with_blocks(cfg, f);
                 }
-                e = hipStreamEndCapture(cap_stream, &wf->graph);
+                e = hipStreamEndCapture(cap_stream, &wf->graph[gk]);
             }
-            if (e == hipSuccess) e = hipGraphInstantiate(&wf->exec, wf->graph, nullptr, nullptr, 0);
+            if (e == hipSuccess) e = hipGraphInstantiate(&wf->exec[gk], wf->graph[gk], nullptr, nullptr, 0);
             (void)hipStreamDestroy(cap_stream);
             if (e != hipSuccess) {
-                wf->release_graph();
+                wf->release_graph(gk);
                 if (err) *err = std::string("wavefront graph capture: ") + hipGetErrorString(e);
                 return e;
             }
-            std::memcpy(wf->key, key, sizeof(key));
+            std::memcpy(wf->key[gk], key, sizeof(key));
         }
         for (int c = 0; c < nchunks; ++c)
-            if ((e = hipGraphLaunch(wf->exec, stream)) != hipSuccess) return e;
+            if ((e = hipGraphLaunch(wf->exec[gk], stream)) != hipSuccess) return e;
     } else if (count) {
         ChunkEnqueuer<true> f{*wf, cfg, cs, jobs, ctr, stream, tm, nchunks, g};
         with_blocks(cfg, f);
@@ -1113,8 +1672,9 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
     }
     {
         Timed t(tm, WF_CONTROL);
-        hipLaunchKernelGGL(k_wf_film, dim3((static_cast<uint32_t>(pixels) + 255) / 256), dim3(256), 0, stream,
-                           wf->st, ctr, count ? 1u : 0u);
+        const dim3 grid((static_cast<uint32_t>(pixels) + 255) / 256);
+        if (cs.adjoint) hipLaunchKernelGGL(k_wfa_replay_out, grid, dim3(256), 0, stream, wf->st);
+        else hipLaunchKernelGGL(k_wf_film, grid, dim3(256), 0, stream, wf->st, ctr, count ? 1u : 0u);
     }
     (void)hipEventRecord(e1, stream);
     e = hipEventSynchronize(e1);
@@ -1133,6 +1693,19 @@ hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs
     }
     if (e != hipSuccess) return e;
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs, const WavefrontConfig& cfg,
+                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err) {
+    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, nullptr);
+}
+
+hipError_t wavefront_adjoint(Wavefront* wf, const std::vector<WavefrontJob>& jobs, const AdjointGrads& g,
+                             const WavefrontConfig& cfg, DevCounters* ctr, bool count, hipStream_t stream, float* ms,
+                             std::string* err) {
+    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, &g);
 }
 
 void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches) {

@@ -6,8 +6,12 @@ of the same batch (same draws, same decisions) and takes central differences
 of the detached estimator (directions, pdfs, MIS weights and Russian
 roulette held) in double precision: the two agree path by path, so the
 tolerance is the 1e-3 relative bar, plus a small absolute floor for
-parameters whose gradient is (nearly) zero."""
+parameters whose gradient is (nearly) zero.  Both forms of the adjoint (one
+thread per pixel, and the wavefront schedule's primal + replay passes) are
+checked; they compute the same estimator and agree to float summation order."""
 import ctypes as C
+import json
+import os
 
 import numpy as np
 import pytest
@@ -20,6 +24,9 @@ from test_gpu_parity import host_view
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
+
+FORMS = {"path": pt.PathIntegrator, "wavefront": pt.WavefrontIntegrator}
+CORNELL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "scenes", "cornell_box.json")
 
 
 def _oracle_with_textures(sm):
@@ -53,9 +60,11 @@ def test_diff_cornell_workload():
     assert info.num_faces == 36
 
 
-def test_adjoint_replay_reproduces_forward():
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_adjoint_replay_reproduces_forward(form):
     from pathtracingtutorial_amd.diff import DiffRenderer
     sm = _diff_cornell()
+    sm.setAdjointIntegrator(FORMS[form]())
     dr = DiffRenderer(sm)
     img = dr.forward_image(5)
     g = torch.ones_like(img)
@@ -98,9 +107,11 @@ def _gpu_grad(out, p, tex_w=256):
     return float(out["env"].reshape(-1, 4)[b, c])
 
 
-def test_adjoint_matches_fp64_detached_fd():
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_adjoint_matches_fp64_detached_fd(form):
     from pathtracingtutorial_amd.diff import DiffRenderer
     sm = _diff_cornell(w=48, h=48, spf=4)
+    sm.setAdjointIntegrator(FORMS[form]())
     cam, info = sm.camera(), sm.info()
     rng = np.random.default_rng(11)
     weights = rng.uniform(-1.0, 1.0, size=(info.height, info.width, 3)).astype(np.float32)
@@ -160,3 +171,74 @@ def test_autograd_function_and_descent():
             bc.clamp_(0.0, 1.0)
     err1 = float((bc.detach()[learn] - gt[learn]).abs().mean())
     assert err1 < 0.6 * err0, (err0, err1)
+
+
+def _both_forms(sm, frame, g, replay=False):
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    dr = DiffRenderer(sm)
+    out = {}
+    for form in ("path", "wavefront"):
+        sm.setAdjointIntegrator(FORMS[form]())
+        out[form] = dr.adjoint(frame, g, replay=replay)
+    return out
+
+
+def _assert_same_grads(a, b, keys=("base_color", "roughness", "metallic", "texture", "env")):
+    for k in keys:
+        if a[k] is None:
+            assert b[k] is None
+            continue
+        scale = float(a[k].abs().max()) + 1e-12
+        diff = float((a[k] - b[k]).abs().max())
+        assert diff <= 2e-4 * scale, (k, diff, scale)
+
+
+def test_wavefront_adjoint_equals_megakernel_adjoint():
+    sm = _diff_cornell(w=64, h=48, spf=8)
+    rng = np.random.default_rng(5)
+    g = torch.from_numpy(rng.uniform(-1, 1, (48, 64, 4)).astype(np.float32)).cuda()
+    out = _both_forms(sm, 3, g, replay=True)
+    assert sm.adjointIntegrator() == "Wavefront"
+    _assert_same_grads(out["path"], out["wavefront"])
+    # both replays are the forward batch
+    assert torch.allclose(out["path"]["replay"], out["wavefront"]["replay"], rtol=1e-5, atol=1e-6)
+
+
+def test_wavefront_adjoint_chunked_equals_single_chunk():
+    # one chunk per sample (chunk_spp=1) against all samples in flight
+    sm = _diff_cornell(w=40, h=40, spf=8)
+    g = torch.ones((40, 40, 4), dtype=torch.float32, device="cuda")
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    dr = DiffRenderer(sm)
+    sm.setAdjointIntegrator(pt.WavefrontIntegrator())
+    outs = []
+    for cfg in (dict(chunk_spp=8), dict(chunk_spp=1, use_graph=0), dict(chunk_spp=2, sort_by_material=0)):
+        sm.setWavefrontConfig(**cfg)
+        outs.append(dr.adjoint(1, g))
+    for o in outs[1:]:
+        _assert_same_grads(outs[0], o)
+
+
+def test_wavefront_adjoint_punctual_lights(tmp_path):
+    scene = json.load(open(CORNELL))
+    scene["camera"]["resolution"] = [40, 40]
+    scene["render"]["samplePerFrame"] = 4
+    scene["render"]["spp"] = 4
+    scene["lights"] = [
+        {"type": "point", "position": [0.2, 0.3, 0.1], "color": [1, 0.9, 0.8], "intensity": 0.5},
+        {"type": "spot", "position": [-0.2, 0.4, 0.2], "direction": [0.1, -1, -0.1], "color": [0.3, 0.6, 1.0],
+         "intensity": 2.0, "innerCone": 0.3, "outerCone": 0.6},
+    ]
+    path = tmp_path / "cornell_lights.json"
+    path.write_text(json.dumps(scene))
+    sm = pt.SceneManager()
+    sm.load(str(path))
+    sm.buildAccel()
+    sm.uploadToDevice()
+    assert sm.info().num_lights == 2
+    rng = np.random.default_rng(9)
+    g = torch.from_numpy(rng.uniform(0, 1, (40, 40, 4)).astype(np.float32)).cuda()
+    out = _both_forms(sm, 0, g, replay=True)
+    _assert_same_grads(out["path"], out["wavefront"], keys=("base_color", "roughness", "metallic"))
+    assert float(out["wavefront"]["base_color"].abs().sum()) > 0
+    assert torch.allclose(out["path"]["replay"], out["wavefront"]["replay"], rtol=1e-4, atol=1e-5)

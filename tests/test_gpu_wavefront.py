@@ -119,6 +119,8 @@ CONFIGS = [
     dict(waves_per_simd=1),
     dict(stack_lds=16),
     dict(stack_lds=16, extend_block=256, waves_per_simd=2),
+    dict(grab_extend=64, grab_shade=64, grab_shadow=64),
+    dict(grab_extend=1024, grab_shade=1024, grab_shadow=1024, chunk_spp=2),
 ]
 
 
@@ -229,6 +231,8 @@ def test_wavefront_bad_config_rejected():
         sm.setWavefrontConfig(chunk_spp=0)
     with pytest.raises(pt.PtError):
         sm.setWavefrontConfig(stack_lds=8)
+    with pytest.raises(pt.PtError):
+        sm.setWavefrontConfig(grab_shade=96)
 
 
 def test_wavefront_split_stack_deep_bvh_bit_identical():
