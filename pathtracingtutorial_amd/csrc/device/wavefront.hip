@@ -48,7 +48,10 @@ constexpr int C_SECNT = 24;  // [8] environment shadow queue segment counts
 constexpr int C_SP = 32;     // punctual shadow queue count (one flat queue)
 constexpr int C_CHUNK = 33;  // chunk index within the batch
 constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72;
-constexpr int C_WORDS = 80;
+// per cursor set (5 of them, in the order above): a bit mask of drained
+// segments, each mask on a 128-byte line of its own
+constexpr int C_DRAIN = 128, C_DRAIN_STRIDE = 32;
+constexpr int C_WORDS = C_DRAIN + 5 * C_DRAIN_STRIDE;
 
 // A job as the kernels see it (device memory, one entry per job).
 struct JobDesc {
@@ -150,34 +153,57 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 // Segment k holds [base(k), base(k) + counts[k]) (counts == null: a flat
 // queue of n items in kSeg equal ranges).  Grabbing several rows of 64 per
 // atomic keeps the same-address atomics on the 8 cursors, which the device
-// serializes, off the critical path.
+// serializes, off the critical path.  Draining is cheap too: an empty
+// segment is skipped without an atomic, and the one wave whose grab is the
+// first past a segment's end sets the segment's bit in the drained mask,
+// which waves read before stealing from another segment; otherwise every
+// wave of the persistent grid would end with one atomic per segment.
 struct QueueWalk {
     uint32_t* cursors;
+    uint32_t* drained;
     const uint32_t* counts;
     uint32_t n, step;
     int seg, tried;
     uint32_t cur, lim;  // the wave's current grab [cur, lim)
 
-    __device__ __forceinline__ QueueWalk(uint32_t* c, const uint32_t* cnt, uint32_t items, uint32_t grab)
-        : cursors(c), counts(cnt), n(items), step(grab), seg(static_cast<int>(blockIdx.x % kSeg)), tried(0), cur(0),
-          lim(0) {}
+    __device__ __forceinline__ QueueWalk(uint32_t* ctrl, int cur_slot, const uint32_t* cnt, uint32_t items,
+                                         uint32_t grab)
+        : cursors(ctrl + cur_slot), drained(ctrl + C_DRAIN + C_DRAIN_STRIDE * ((cur_slot - C_CUR_EXT) / kSeg)),
+          counts(cnt), n(items), step(grab), seg(static_cast<int>(blockIdx.x % kSeg)), tried(0), cur(0), lim(0) {}
+
+    __device__ __forceinline__ void skip() {
+        seg = (seg + 1) % kSeg;
+        ++tried;
+    }
 
     // The next 64 items of the wave: lane item i, valid while i < end.
     // Returns false (for the whole wave) once every segment is drained.
     __device__ __forceinline__ bool next(uint32_t& i, uint32_t& end) {
         while (cur >= lim) {
+            if (tried > 0 && tried < kSeg) {  // stealing: leave out segments known to be drained
+                uint32_t m = 0;
+                if (lane_id() == 0) m = __hip_atomic_load(drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                m = __shfl(m, 0, 64);
+                while (tried < kSeg && ((m >> seg) & 1u)) skip();
+            }
             if (tried >= kSeg) return false;
             const uint32_t s0 = seg_base(n, seg);
             const uint32_t s1 = counts ? s0 + counts[seg] : seg_base(n, seg + 1);
+            if (s1 <= s0) {
+                skip();
+                continue;
+            }
             uint32_t off = 0;
-            if (lane_id() == 0) off = atomicAdd(&cursors[seg], step);
+            if (lane_id() == 0) {
+                off = atomicAdd(&cursors[seg], step);
+                if (off >= s1 - s0 && off < s1 - s0 + step) atomicOr(drained, 1u << seg);
+            }
             off = __shfl(off, 0, 64);
             if (s0 + off < s1) {
                 cur = s0 + off;
                 lim = min(s1, cur + step);
             } else {
-                seg = (seg + 1) % kSeg;
-                ++tried;
+                skip();
             }
         }
         i = cur + static_cast<uint32_t>(lane_id());
@@ -312,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
     const uint32_t* q = w.q[in_slot];
-    QueueWalk qw(w.ctrl + C_CUR_EXT, counts, n, w.grab_extend);
+    QueueWalk qw(w.ctrl, C_CUR_EXT, counts, n, w.grab_extend);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
@@ -480,7 +506,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
-    QueueWalk qw(w.ctrl + C_CUR_SHADE, counts, n, w.grab_shade);
+    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
     uint32_t i, end;
     while (qw.next(i, end)) {
         const int seg = qw.seg;
@@ -553,7 +579,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
-    QueueWalk qw(w.ctrl + cur_slot, counts, n, w.grab_shadow);
+    QueueWalk qw(w.ctrl, cur_slot, counts, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
@@ -577,7 +603,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCoun
     __shared__ uint32_t stack[LDS * BLOCK];
     const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
-    QueueWalk qw(w.ctrl + C_CUR_SP, nullptr, n, w.grab_shadow);
+    QueueWalk qw(w.ctrl, C_CUR_SP, nullptr, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
@@ -890,7 +916,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
-    QueueWalk qw(w.ctrl + C_CUR_SHADE, counts, n, w.grab_shade);
+    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
     uint32_t i, end;
     while (qw.next(i, end)) {
         const int seg = qw.seg;
@@ -990,7 +1016,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot,
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
-    QueueWalk qw(w.ctrl + cur_slot, counts, n, w.grab_shadow);
+    QueueWalk qw(w.ctrl, cur_slot, counts, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
@@ -1038,7 +1064,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow_punctual(WfState w, DevCou
     const AdjointGrads G = *w.grads;
     const DevScene& s0 = w.jobs[0].scene;
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
-    QueueWalk qw(w.ctrl + C_CUR_SP, nullptr, n, w.grab_shadow);
+    QueueWalk qw(w.ctrl, C_CUR_SP, nullptr, n, w.grab_shadow);
     uint32_t i, end;
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
@@ -1304,7 +1330,8 @@ struct ChunkShape {
     int max_depth;
     uint32_t nbins;
     bool sort, lights;
-    bool adjoint;  // primal pass, then the replay of the path-replay adjoint
+    bool area, env;  // some job samples emissive triangles / the environment (else that shadow queue stays empty)
+    bool adjoint;    // primal pass, then the replay of the path-replay adjoint
 };
 
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
@@ -1320,22 +1347,28 @@ void launch_traversal_extend(const Grids& g, hipStream_t stream, const WfState& 
     hipLaunchKernelGGL((k_wf_extend<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
 }
 template <bool COUNT, int EB, int LDS>
-void launch_traversal_shadow(const Grids& g, hipStream_t stream, const WfState& w, bool lights, DevCounters* ctr) {
-    hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
-                       w.sa_o, w.sa_d, w.sa_c, ctr);
-    hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
-                       w.se_o, w.se_d, w.se_c, ctr);
-    if (lights)
+void launch_traversal_shadow(const Grids& g, hipStream_t stream, const WfState& w, const ChunkShape& cs,
+                             DevCounters* ctr) {
+    if (cs.area)
+        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
+                           w.sa_o, w.sa_d, w.sa_c, ctr);
+    if (cs.env)
+        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
+                           w.se_o, w.se_d, w.se_c, ctr);
+    if (cs.lights)
         hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
 }
 
 template <bool COUNT, int EB>
-void launch_replay_shadow(const Grids& g, hipStream_t stream, const WfState& w, bool lights, DevCounters* ctr) {
-    hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, false>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
-                       w.sa_o, w.sa_d, w.sa_c, w.sa_x, ctr);
-    hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, true>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
-                       w.se_o, w.se_d, w.se_c, w.se_x, ctr);
-    if (lights)
+void launch_replay_shadow(const Grids& g, hipStream_t stream, const WfState& w, const ChunkShape& cs,
+                          DevCounters* ctr) {
+    if (cs.area)
+        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, false>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SACNT,
+                           C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, w.sa_x, ctr);
+    if (cs.env)
+        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, true>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SECNT,
+                           C_CUR_SE, w.se_o, w.se_d, w.se_c, w.se_x, ctr);
+    if (cs.lights)
         hipLaunchKernelGGL((k_wfa_shadow_punctual<COUNT, EB>), dim3(g.ashadow), dim3(EB), 0, stream, w, ctr);
 }
 
@@ -1387,9 +1420,9 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_SHADOW);
-            if (replay) launch_replay_shadow<COUNT, EB>(g, stream, w, cs.lights, ctr);
-            else if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs.lights, ctr);
-            else launch_traversal_shadow<COUNT, EB, kStackSize>(g, stream, w, cs.lights, ctr);
+            if (replay) launch_replay_shadow<COUNT, EB>(g, stream, w, cs, ctr);
+            else if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs, ctr);
+            else launch_traversal_shadow<COUNT, EB, kStackSize>(g, stream, w, cs, ctr);
         }
     }
 }
@@ -1532,6 +1565,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     unsigned long long paths = 0, pixels = 0;
     uint32_t max_lights = 0, max_mat = 0;
     int max_depth = 0;
+    bool any_area = false, any_env = false;
     for (size_t j = 0; j < jobs_in.size(); ++j) {
         const WavefrontJob& in = jobs_in[j];
         JobDesc& d = jobs[j];
@@ -1557,6 +1591,10 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
         paths += npix * static_cast<unsigned long long>(in.params.spf);
         pixels += npix;
         max_lights = std::max(max_lights, in.scene.num_lights);
+        // the tests bounce() makes before sampling each light type
+        any_area = any_area || (in.scene.num_emissive > 0 && in.scene.emissive != nullptr);
+        any_env = any_env || (in.scene.env != nullptr && in.scene.env_alias != nullptr && in.scene.num_env_texels > 0 &&
+                              in.scene.env_w > 0 && in.scene.env_h > 0);
         max_mat = std::max(max_mat, in.scene.num_materials);
         max_depth = std::max(max_depth, in.params.max_depth);
     }
@@ -1594,6 +1632,8 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.nbins = max_mat + 1;
     cs.sort = cfg.sort_by_material && cs.nbins <= kMaxSortBins;
     cs.lights = max_lights > 0;
+    cs.area = any_area;
+    cs.env = any_env;
     cs.adjoint = G != nullptr;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
@@ -1612,7 +1652,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
-                        static_cast<uint64_t>(cs.lights), static_cast<uint64_t>(cs.nbins),
+                        static_cast<uint64_t>(cs.lights + 2 * cs.area + 4 * cs.env), static_cast<uint64_t>(cs.nbins),
                         static_cast<uint64_t>(cfg.stack_lds + (cs.adjoint ? 1000 : 0)) |
                             (static_cast<uint64_t>(cfg.grab_extend) << 16) | (static_cast<uint64_t>(cfg.grab_shade) << 32) |
                             (static_cast<uint64_t>(cfg.grab_shadow) << 48)};
