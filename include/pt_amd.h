@@ -318,13 +318,18 @@ pt_status pt_film_reset(pt_film* film, pt_scene* scene);
 pt_status pt_film_download(const pt_film* film, int which, float* rgba_host);
 /* Device pointer of the acc (which 0) or new (which 1) surface, W*H*4 floats. */
 pt_status pt_film_device_ptr(const pt_film* film, int which, void** dev_ptr);
-/* ImageIO::write (io/image_io.cpp:79-144) for .hdr (Radiance RGBE, rows
- * flipped so the file's first row is the image top) and .pfm: host RGBA
- * floats, row 0 = bottom.  Other extensions return PT_ERR_UNSUPPORTED. */
+/* ImageIO::write (io/image_io.cpp:79-144): host RGBA floats, row 0 = bottom,
+ * written with the rows flipped so the file's first row is the image top.
+ * .hdr (Radiance RGBE) and .pfm keep floats; .png, .jpg/.jpeg (quality 90),
+ * .bmp and .tga store clamp(pow(v, 1/2.2), 0, 1) * 255 + 0.5 per channel
+ * (image_io.cpp:119-127).  Other extensions return PT_ERR_UNSUPPORTED. */
 pt_status pt_image_write(const char* path, const float* rgba, int width, int height);
-/* ImageIO::load of a Radiance .hdr (stb_image's RGBE decode, io/image_io.cpp:
- * 25-45): call with rgba NULL for the size, then with W*H*4 floats; row 0 =
- * bottom (the same convention as films, so a saved film reads back equal). */
+/* ImageIO::load (io/image_io.cpp:17-67): .hdr (RGBE) or .png/.jpg/.jpeg
+ * (8-bit, decoded with pow(v/255, 2.2)); alpha = 1.  Call with rgba NULL for
+ * the size, then with W*H*4 floats; row 0 = bottom (the film convention, so
+ * a saved film reads back equal up to the format's precision). */
+pt_status pt_image_read(const char* path, float* rgba, int* width, int* height);
+/* The same as pt_image_read (kept for callers written against .hdr only). */
 pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* height);
 /* Copies the acc (0) or new (1) surface into device memory dst (W*H*4 floats). */
 pt_status pt_film_copy_device(const pt_film* film, int which, float* dst_device);

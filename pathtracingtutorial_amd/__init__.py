@@ -360,7 +360,7 @@ class Renderer:
         return int(p.value or 0)
 
     def saveImage(self, path: str) -> None:
-        """ImageIO::write of the accumulated film (.hdr or .pfm)."""
+        """ImageIO::write of the accumulated film (.hdr .pfm .png .jpg .jpeg .bmp .tga)."""
         from .imageio import write_image
         write_image(path, self.downloadCurrentImage())
 

@@ -214,6 +214,7 @@ SIGNATURES = {
     "pt_image_write": (C.c_int, [C.c_char_p, _FP, C.c_int, C.c_int]),
     "pt_scene_output_path": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "pt_image_read_hdr": (C.c_int, [C.c_char_p, _FP, _I32P, _I32P]),
+    "pt_image_read": (C.c_int, [C.c_char_p, _FP, _I32P, _I32P]),
     "pt_render_batch": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_all": (C.c_int, [_P, _P, C.c_int, _FP]),
     "pt_render_jobs": (C.c_int, [C.POINTER(Job), C.c_int, _FP]),

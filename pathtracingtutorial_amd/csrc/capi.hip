@@ -878,12 +878,13 @@ pt_status pt_scene_output_path(const pt_scene* s, char* buf, size_t cap) {
     });
 }
 
-pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* height) {
+pt_status pt_image_read(const char* path, float* rgba, int* width, int* height) {
     return guarded([&] {
         if (!path || !width || !height) throw Error(PT_ERR_INVALID_ARG, "null argument");
         ptamd::Image img;
         std::string err;
-        if (!ptamd::load_hdr(path, img, &err)) throw Error(PT_ERR_IO, err.empty() ? std::string("cannot read ") + path : err);
+        if (!ptamd::load_image(path, img, &err))
+            throw Error(PT_ERR_IO, err.empty() ? std::string("cannot read ") + path : err);
         if (rgba) {
             if (*width != img.width || *height != img.height) throw Error(PT_ERR_INVALID_ARG, "buffer size mismatch");
             // the loader returns the file's top row first; films keep row 0 at the bottom
@@ -897,19 +898,21 @@ pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* heig
     });
 }
 
+pt_status pt_image_read_hdr(const char* path, float* rgba, int* width, int* height) {
+    return pt_image_read(path, rgba, width, height);
+}
+
 pt_status pt_image_write(const char* path, const float* rgba, int w, int h) {
     return guarded([&] {
         if (!path || !rgba || w <= 0 || h <= 0) throw Error(PT_ERR_INVALID_ARG, "bad image arguments");
         std::string p(path), err;
-        auto ends = [&](const char* e) {
-            size_t n = std::strlen(e);
-            return p.size() >= n && p.compare(p.size() - n, n, e) == 0;
-        };
-        bool ok;
-        if (ends(".hdr")) ok = ptamd::write_hdr(p, w, h, rgba, true, &err);
-        else if (ends(".pfm")) ok = ptamd::write_pfm(p, w, h, rgba, &err);
-        else throw Error(PT_ERR_UNSUPPORTED, "image format not supported (use .hdr or .pfm): " + p);
-        if (!ok) throw Error(PT_ERR_IO, err.empty() ? "cannot write " + p : err);
+        std::string ext = p.substr(std::min(p.size(), p.find_last_of('.')));
+        for (auto& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+        static const char* known[] = {".hdr", ".pfm", ".png", ".jpg", ".jpeg", ".bmp", ".tga"};
+        if (std::none_of(std::begin(known), std::end(known), [&](const char* k) { return ext == k; }))
+            throw Error(PT_ERR_UNSUPPORTED, "image format not supported (.hdr .pfm .png .jpg .jpeg .bmp .tga): " + p);
+        if (!ptamd::write_image(p, w, h, rgba, true, &err))
+            throw Error(PT_ERR_IO, err.empty() ? "cannot write " + p : err);
     });
 }
 

@@ -52,6 +52,10 @@ bool load_obj(const std::string& path, int material_id, const float xform[16],
 bool load_hdr(const std::string& path, Image& img, std::string* err);
 bool write_hdr(const std::string& path, int w, int h, const float* rgba, bool flip_y, std::string* err);
 bool write_pfm(const std::string& path, int w, int h, const float* rgba, std::string* err);
+// ImageIO::load / ImageIO::write (io/image_io.cpp): every format the reference reads and writes.
+bool load_image(const std::string& path, Image& img, std::string* err);
+bool write_image(const std::string& path, int w, int h, const float* rgba, bool flip_y, std::string* err);
+uint8_t ldr_byte(float v);
 void build_quad(const float a[3], const float b[3], const float c[3], const float d[3], int mat,
                 std::vector<pt_vertex>& verts, std::vector<pt_face>& faces);
 void auto_fit_camera(const std::vector<pt_vertex>& verts, pt_camera& cam);

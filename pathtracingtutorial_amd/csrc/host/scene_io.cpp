@@ -14,6 +14,7 @@
 #include <unordered_map>
 
 #include "host_scene.h"
+#include "image_codec.h"
 #include "json.h"
 
 namespace ptamd {
@@ -652,6 +653,87 @@ bool write_pfm(const std::string& path, int w, int h, const float* rgba, std::st
     return static_cast<bool>(out);
 }
 
+// ----------------------------------------------------------------- ImageIO --
+
+namespace {
+std::string lower_ext(const std::string& path) {
+    const size_t dot = path.find_last_of('.');
+    std::string ext = dot == std::string::npos ? std::string() : path.substr(dot);
+    for (auto& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    return ext;
+}
+}  // namespace
+
+// ImageIO::load (io/image_io.cpp:17-67): .hdr as float RGB, .png/.jpg/.jpeg
+// as 8-bit RGB decoded with pow(v / 255, 2.2); alpha is 1 in both.
+bool load_image(const std::string& path, Image& img, std::string* err) {
+    const std::string ext = lower_ext(path);
+    if (ext == ".hdr") return load_hdr(path, img, err);
+    if (ext != ".png" && ext != ".jpg" && ext != ".jpeg") {
+        if (err) *err = "unsupported image format: '" + path + "'";
+        return false;
+    }
+    std::vector<uint8_t> bytes;
+    Image8 im;
+    std::string derr;
+    if (!read_file(path, bytes, err)) return false;
+    if (!decode_image(bytes.data(), bytes.size(), im, &derr)) {
+        if (err) *err = "'" + path + "': " + derr;
+        return false;
+    }
+    img.width = im.width;
+    img.height = im.height;
+    img.rgba.resize(static_cast<size_t>(im.width) * im.height * 4);
+    for (size_t i = 0; i < static_cast<size_t>(im.width) * im.height; ++i) {
+        for (int c = 0; c < 3; ++c) img.rgba[i * 4 + c] = std::pow(im.rgba[i * 4 + c] / 255.0f, 2.2f);
+        img.rgba[i * 4 + 3] = 1.0f;
+    }
+    return true;
+}
+
+// The 8-bit value ImageIO::write stores for a linear channel
+// (io/image_io.cpp:119-127): clamp(pow(v, 1/2.2), 0, 1) * 255 + 0.5, truncated.
+// A NaN (the power of a negative value) is stored as 0.
+uint8_t ldr_byte(float v) {
+    float g = std::pow(v, 1.0f / 2.2f);
+    if (!(g >= 0.0f)) g = 0.0f;
+    if (g > 1.0f) g = 1.0f;
+    return static_cast<uint8_t>(g * 255.0f + 0.5f);
+}
+
+// ImageIO::write (io/image_io.cpp:79-144) for .hdr, .pfm, .png, .jpg/.jpeg
+// (quality 90), .bmp and .tga; flip_y writes row h-1 of rgba first.
+bool write_image(const std::string& path, int w, int h, const float* rgba, bool flip_y, std::string* err) {
+    const std::string ext = lower_ext(path);
+    if (ext == ".hdr") return write_hdr(path, w, h, rgba, flip_y, err);
+    if (ext == ".pfm") return write_pfm(path, w, h, rgba, err);
+    if (ext != ".png" && ext != ".jpg" && ext != ".jpeg" && ext != ".bmp" && ext != ".tga") {
+        if (err) *err = "unsupported image format: '" + path + "'";
+        return false;
+    }
+    std::vector<uint8_t> rgb(static_cast<size_t>(w) * h * 3), out;
+    for (int y = 0; y < h; ++y) {
+        const float* src = rgba + static_cast<size_t>(flip_y ? h - 1 - y : y) * w * 4;
+        for (int x = 0; x < w; ++x)
+            for (int c = 0; c < 3; ++c) rgb[(static_cast<size_t>(y) * w + x) * 3 + c] = ldr_byte(src[x * 4 + c]);
+    }
+    if (ext == ".png") encode_png(w, h, rgb.data(), out);
+    else if (ext == ".bmp") encode_bmp(w, h, rgb.data(), out);
+    else if (ext == ".tga") encode_tga(w, h, rgb.data(), out);
+    else encode_jpeg(w, h, rgb.data(), 90, out);
+    std::ofstream f(path, std::ios::binary);
+    if (!f.is_open()) {
+        if (err) *err = "cannot write '" + path + "'";
+        return false;
+    }
+    f.write(reinterpret_cast<const char*>(out.data()), static_cast<std::streamsize>(out.size()));
+    if (!f) {
+        if (err) *err = "cannot write '" + path + "'";
+        return false;
+    }
+    return true;
+}
+
 // -------------------------------------------------------------- JSON scene --
 
 namespace {
@@ -767,13 +849,7 @@ bool load_scene_json(const std::string& path, HostScene& scene, std::string* err
             if (e.contains("file")) {
                 std::string hp = resolve_relative(path, e["file"].str);
                 std::string herr;
-                std::string ext = hp.size() >= 4 ? hp.substr(hp.size() - 4) : hp;
-                for (auto& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
-                if (ext != ".hdr") {
-                    if (err) *err = "environment '" + hp + "': only Radiance .hdr images are supported";
-                    return false;
-                }
-                if (!load_hdr(hp, scene.env, &herr)) {
+                if (!load_image(hp, scene.env, &herr)) {
                     // ImageIO::load failure leaves an empty image; the reference renders black
                     // (device_scene.cu:19-27).  Report it instead of rendering silently.
                     if (err) *err = herr;

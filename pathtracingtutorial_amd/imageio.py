@@ -1,9 +1,12 @@
-"""Film output: Radiance .hdr (RGBE) and .pfm writers, .hdr reader.
-
-write_image flips rows like ImageIO::write (Part_2/src/io/image_io.cpp:
-79-144): the film's row 0 is the bottom image row.
+"""Image files: write_image / read_image go through libptamd (ImageIO::write
+and ImageIO::load of Part_2/src/io/image_io.cpp: .hdr .pfm .png .jpg .jpeg
+.bmp .tga out, .hdr .png .jpg .jpeg in); rows follow the film convention
+(row 0 = bottom image row, flipped on write).  write_hdr / write_pfm /
+read_hdr are small numpy codecs of the float formats.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
@@ -36,12 +39,22 @@ def write_pfm(path: str, img: np.ndarray) -> None:
 
 
 def write_image(path: str, img: np.ndarray) -> None:
-    if path.lower().endswith(".pfm"):
-        write_pfm(path, img)
-    elif path.lower().endswith(".hdr"):
-        write_hdr(path, img)
-    else:
-        raise ValueError(f"unsupported output format: {path} (use .hdr or .pfm)")
+    """ImageIO::write of an (H, W, 3|4) float image whose row 0 is the bottom row."""
+    from . import _capi, lib
+    h, w = img.shape[:2]
+    a = np.ones((h, w, 4), np.float32)
+    a[..., :3] = img[..., :3]
+    _capi.check(lib().pt_image_write(path.encode(), a.ctypes.data_as(C.POINTER(C.c_float)), w, h))
+
+
+def read_image(path: str) -> np.ndarray:
+    """ImageIO::load: (H, W, 4) float32, linear, alpha 1, row 0 = bottom row."""
+    from . import _capi, lib
+    w, h = C.c_int32(0), C.c_int32(0)
+    _capi.check(lib().pt_image_read(path.encode(), None, C.byref(w), C.byref(h)))
+    out = np.empty((h.value, w.value, 4), np.float32)
+    _capi.check(lib().pt_image_read(path.encode(), out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(w), C.byref(h)))
+    return out
 
 
 def read_hdr(path: str) -> np.ndarray:
