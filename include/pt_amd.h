@@ -230,8 +230,12 @@ pt_status pt_struct_sizes(uint32_t* out, int n);
 /* ------------------------------------------------------------------------ */
 /* Scene                                                                      */
 /* ------------------------------------------------------------------------ */
-/* Loads a JSON scene (scene/scene_loader.cpp:240-292 SceneLoader::load;
- * scene/scene_manager.cpp:11 SceneManager::load).  Same schema and defaults. */
+/* Loads a scene by extension (scene/scene_loader.cpp:278-292 SceneLoader::load;
+ * scene/scene_manager.cpp:11 SceneManager::load): .json (the same schema and
+ * defaults, including "gltf" objects with transforms and materialOverrides,
+ * scene_loader.cpp:162-194) or a glTF 2.0 file, .gltf or .glb
+ * (io/gltf_loader.cpp:583-732: its camera, or one fitted to the bounds; its
+ * KHR_lights_punctual lights, or two default lights when nothing emits). */
 pt_status pt_scene_load(const char* path, pt_scene** out);
 /* Builds a scene from host arrays (HostScene, scene/scene.h:24-35). */
 pt_status pt_scene_create(const pt_host_scene_desc* desc, pt_scene** out);

@@ -419,16 +419,13 @@ pt_status pt_scene_load(const char* path, pt_scene** out) {
         if (!path || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");
         *out = nullptr;
         std::string p(path);
-        std::string ext = p.size() >= 5 ? p.substr(p.size() - 5) : p;
+        std::string ext = p.substr(std::min(p.size(), p.find_last_of('.')));
         for (auto& c : ext) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
-        if (ext != ".json") {
-            if (ext == ".gltf" || (p.size() >= 4 && p.substr(p.size() - 4) == ".glb"))
-                throw Error(PT_ERR_UNSUPPORTED, "glTF scenes are not supported by this build: " + p);
+        if (ext != ".json" && ext != ".gltf" && ext != ".glb")
             throw Error(PT_ERR_UNSUPPORTED, "unknown scene format: " + p);
-        }
         std::unique_ptr<pt_scene> sc(new_scene());
         std::string err;
-        if (!ptamd::load_scene_json(p, sc->host, &err)) {
+        if (!ptamd::load_scene(p, sc->host, &err)) {
             pt_status code = PT_ERR_PARSE;
             if (err.rfind("cannot open", 0) == 0) code = PT_ERR_IO;
             throw Error(code, err);

@@ -47,6 +47,7 @@ void init_scene_defaults(HostScene& s);
 
 // Scene loading (scene/scene_loader.cpp).  Return false with *err set.
 bool load_scene_json(const std::string& path, HostScene& scene, std::string* err);
+bool load_scene(const std::string& path, HostScene& scene, std::string* err);  // .json, .gltf, .glb
 bool load_obj(const std::string& path, int material_id, const float xform[16],
               std::vector<pt_vertex>& verts, std::vector<pt_face>& faces, std::string* err);
 bool load_hdr(const std::string& path, Image& img, std::string* err);

@@ -13,6 +13,7 @@
 #include <sstream>
 #include <unordered_map>
 
+#include "gltf.h"
 #include "host_scene.h"
 #include "image_codec.h"
 #include "json.h"
@@ -791,6 +792,19 @@ void parse_transform(const Json& obj, float out[16]) {
 
 }  // namespace
 
+// SceneLoader::load (scene_loader.cpp:278-292): by extension.
+bool load_scene(const std::string& path, HostScene& scene, std::string* err) {
+    const std::string ext = lower_ext(path);
+    if (ext == ".json") return load_scene_json(path, scene, err);
+    if (ext == ".gltf" || ext == ".glb") {
+        init_scene_defaults(scene);
+        scene.path = path;
+        return load_gltf(path, scene, GltfOptions{}, err);
+    }
+    if (err) *err = "unknown scene format: " + path;
+    return false;
+}
+
 // SceneLoader::loadFromJson, scene_loader.cpp:240-276.  Extensions (ignored
 // by the reference, which skips unknown keys): render.seed, environment
 // "generate", material "albedoTexture" (a linear float RGB image), objects of
@@ -940,10 +954,25 @@ bool load_scene_json(const std::string& path, HostScene& scene, std::string* err
                         if (err) *err = merr;
                         return false;
                     }
-                } else if (type == "gltf") {
-                    if (err) *err = "glTF objects are not supported by this build (object file '" +
-                                    (obj.contains("file") ? obj["file"].str : std::string()) + "')";
-                    return false;
+                } else if (type == "gltf") {  // :162-194
+                    const std::string gp = resolve_relative(path, obj["file"].str);
+                    GltfOptions go;
+                    parse_transform(obj, go.transform);
+                    go.load_camera = go.auto_fit_camera = go.add_default_lights = go.load_environment = false;
+                    std::vector<std::string> names;
+                    go.material_names = &names;
+                    const size_t first = scene.materials.size();
+                    std::string gerr;
+                    if (!load_gltf(gp, scene, go, &gerr)) {
+                        // the reference prints the loader error and keeps going (gltf_loader.cpp:600-604)
+                        if (err) *err = gerr;
+                        return false;
+                    }
+                    if (obj.contains("materialOverrides"))
+                        for (const auto& kv : obj["materialOverrides"].obj)
+                            for (size_t i = 0; i < names.size(); ++i)
+                                if (names[i] == kv.first && first + i < scene.materials.size())
+                                    apply_material_override(kv.second, scene.materials[first + i]);
                 }
             }
         }
