@@ -643,6 +643,35 @@ MatEval<R> fetchMaterial(const oracle_scene* s, const pt_material& m, float u, f
     return e;
 }
 
+// interpolateTangent + applyNormalMap (materials/shading_utils.cuh:29-68):
+// the tangent-space normal of the normal texture (scaled in x/y by
+// normalScale) in the frame of the interpolated mesh tangent with the first
+// vertex's handedness, or in Frame(N) when the mesh has no tangent there.
+template <class R>
+V3<R> normalMap(const oracle_scene* s, const pt_material& m, const V3<R>& N, const pt_vertex& va,
+                const pt_vertex& vb, const pt_vertex& vc, R bu, R bv, float u, float v) {
+    if (m.normal_tex_id < 0 || static_cast<size_t>(m.normal_tex_id) >= s->tex.size()) return N;
+    const Texture& t = s->tex[m.normal_tex_id];
+    R nm[4];
+    tex2DT<R>(t.rgba.data(), t.w, t.h, t.wrapS, t.wrapT, u, v, nm);
+    V3<R> mapN(nm[0] * R(2) - R(1), nm[1] * R(2) - R(1), nm[2] * R(2) - R(1));
+    mapN.x *= R(m.normal_scale);
+    mapN.y *= R(m.normal_scale);
+    mapN = normalize(mapN);
+    const R w = R(1) - bu - bv;
+    V3<R> T = normalize(V3<R>(w * R(va.tangent[0]) + bu * R(vb.tangent[0]) + bv * R(vc.tangent[0]),
+                              w * R(va.tangent[1]) + bu * R(vb.tangent[1]) + bv * R(vc.tangent[1]),
+                              w * R(va.tangent[2]) + bu * R(vb.tangent[2]) + bv * R(vc.tangent[2])));
+    const R sign = va.tangent[3] >= 0.0f ? R(1) : R(-1);
+    if (T.x != R(0) || T.y != R(0) || T.z != R(0)) {
+        T = normalize(T - N * dot(N, T));
+        V3<R> B = cross(N, T) * sign;
+        return normalize(T * mapN.x + B * mapN.y + N * mapN.z);
+    }
+    Frame<R> tbn(N);
+    return normalize(tbn.toWorld(mapN));
+}
+
 template <class R> void computeAlpha(R rough, R aniso, R& ax, R& ay) {  // :126-131
     R aspect = std::sqrt(R(1) - R(0.9) * aniso);
     R a = rough * rough;
@@ -1181,7 +1210,7 @@ V3<R> pathTrace(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, 
             mu = ru * mat.uv_scale_x + mat.uv_offset_x;
             mv = rv * mat.uv_scale_y + mat.uv_offset_y;
         }
-        V3<R> N = gn;  // normal maps: parity unpinned, not restated (no configured scene uses one)
+        V3<R> N = normalMap<R>(s, mat, gn, va, vb, vc, h.u, h.v, mu, mv);
         MatEval<R> me = fetchMaterial<R>(s, mat, mu, mv);
         if (isEmissive(mat)) {
             R mis = R(1);
@@ -1346,7 +1375,7 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
             mu = ru * mat.uv_scale_x + mat.uv_offset_x;
             mv = rv * mat.uv_scale_y + mat.uv_offset_y;
         }
-        V3<R> N = gn;
+        V3<R> N = normalMap<R>(s, mat, gn, va, vb, vc, h.u, h.v, mu, mv);
         MatEval<R> me = fetchMaterial<R>(s, mat, mu, mv);
         if (isEmissive(mat)) {
             R mis = R(1);
