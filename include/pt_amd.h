@@ -429,6 +429,17 @@ pt_status pt_scene_update_materials(pt_scene* scene, const float* base_color, co
                                     const float* metallic);
 pt_status pt_scene_update_texture(pt_scene* scene, uint32_t tex, const float* rgba, size_t count, int on_device);
 pt_status pt_scene_rebuild_env_tables(pt_scene* scene);
+/* The same tables built on the GPU from the device environment texels
+ * (light_sampler.cpp:112-149 weights, no host round trip): the pmf equals the
+ * host build's up to rounding and the alias table samples the same
+ * distribution, but its layout differs from the reference's Vose table, so a
+ * given random number picks another texel (the estimator is unchanged).
+ * For inverse rendering, where the environment changes every iteration. */
+pt_status pt_scene_build_env_tables_device(pt_scene* scene);
+/* Alias table of n >= 1 non-negative device weights (not all zero), built on
+ * the GPU: pmf[i] = w[i] / sum (n floats), alias = n {prob, index bits}
+ * pairs (2n floats), all device memory.  Synchronous. */
+pt_status pt_alias_build_device(const float* weights, uint32_t n, float* pmf, float* alias);
 /* Counter control: when enabled, render kernels count rays, node visits and
  * triangle tests (slower; used to price algorithmic bytes). */
 pt_status pt_counters_enable(pt_scene* scene, int enable);

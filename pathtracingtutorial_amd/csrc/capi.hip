@@ -16,6 +16,7 @@
 #include "device/adjoint.h"
 #include "device/bvh_build.h"
 #include "device/dev_scene.h"
+#include "device/light_build.h"
 #include "device/render.h"
 #include "device/wavefront.h"
 #include "host/host_scene.h"
@@ -135,6 +136,7 @@ struct pt_scene {
     ptd::Wavefront* wf = nullptr;
     ptd::WavefrontConfig wf_cfg;
     int adjoint_integrator = PT_INTEGRATOR_WAVEFRONT;
+    bool env_tables_on_device = false;  // the device env tables were built on the GPU (host copy stale)
     uint64_t generation = 0;  // bumped by every upload; keys the captured graph
 
     ~pt_scene() {
@@ -655,6 +657,7 @@ pt_status pt_scene_upload(pt_scene* s) {
         s->env_pmf.upload(L.env_pmf.data(), L.env_pmf.size(), s->stream);
         s->env_alias.upload(reinterpret_cast<const float2*>(L.env_alias.data()), L.env_alias.size(), s->stream);
         if (L.env_pmf.empty()) s->env_pmf.reset(), s->env_alias.reset();
+        s->env_tables_on_device = false;
         s->plights.upload(h.lights.data(), h.lights.size(), s->stream);
         hip_check(hipStreamSynchronize(s->stream), "upload");
         s->frame_count = 0;
@@ -812,8 +815,16 @@ pt_status pt_scene_get_light_tables(const pt_scene* s, float* tri_pmf, uint32_t*
         if (n_tri_alias) *n_tri_alias = static_cast<uint32_t>(L.tri_alias.size());
         if (env_pmf || env_alias) {
             if (!n_env || *n_env < L.env_pmf.size()) throw Error(PT_ERR_INVALID_ARG, "env buffers too small");
-            if (env_pmf) std::memcpy(env_pmf, L.env_pmf.data(), L.env_pmf.size() * 4);
-            if (env_alias) std::memcpy(env_alias, L.env_alias.data(), L.env_alias.size() * 8);
+            if (s->env_tables_on_device) {  // built on the GPU: read the device tables
+                if (env_pmf)
+                    hip_check(hipMemcpy(env_pmf, s->env_pmf.p, L.env_pmf.size() * 4, hipMemcpyDeviceToHost), "env tables");
+                if (env_alias)
+                    hip_check(hipMemcpy(env_alias, s->env_alias.p, L.env_alias.size() * 8, hipMemcpyDeviceToHost),
+                              "env tables");
+            } else {
+                if (env_pmf) std::memcpy(env_pmf, L.env_pmf.data(), L.env_pmf.size() * 4);
+                if (env_alias) std::memcpy(env_alias, L.env_alias.data(), L.env_alias.size() * 8);
+            }
         }
         if (n_env) *n_env = static_cast<uint32_t>(L.env_pmf.size());
     });
@@ -1198,12 +1209,42 @@ pt_status pt_scene_rebuild_env_tables(pt_scene* s) {
             throw Error(PT_ERR_STATE, "environment sampling tables changed size (an all-zero environment?)");
         s->lights.env_pmf = t.env_pmf;
         s->lights.env_alias = t.env_alias;
+        s->env_tables_on_device = false;
         if (!t.env_pmf.empty()) {
             hip_check(hipMemcpy(s->env_pmf.p, t.env_pmf.data(), t.env_pmf.size() * 4, hipMemcpyHostToDevice),
                       "env table update");
             hip_check(hipMemcpy(s->env_alias.p, t.env_alias.data(), t.env_alias.size() * 8, hipMemcpyHostToDevice),
                       "env table update");
         }
+    });
+}
+
+pt_status pt_scene_build_env_tables_device(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        const size_t n = static_cast<size_t>(s->env_w) * s->env_h;
+        if (n == 0 || s->env.p == nullptr) return;
+        if (s->env_pmf.n != n || s->env_alias.n != n)
+            throw Error(PT_ERR_STATE, "the uploaded environment has no sampling tables (an all-zero environment?)");
+        std::string err;
+        hipError_t e = ptd::build_env_tables_device(s->env.p, s->env_w, s->env_h, s->env_pmf.p, s->env_alias.p,
+                                                    s->stream, &err);
+        if (e == hipErrorInvalidValue) throw Error(PT_ERR_INVALID_ARG, err.empty() ? "environment is all zero" : err);
+        hip_check(e, "environment tables on the GPU");
+        s->env_tables_on_device = true;
+    });
+}
+
+pt_status pt_alias_build_device(const float* weights, uint32_t n, float* pmf, float* alias) {
+    return guarded([&] {
+        if (n == 0) return;
+        if (!weights || !pmf || !alias) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        ensure_device();
+        std::string err;
+        hipError_t e = ptd::build_alias_device(weights, n, pmf, reinterpret_cast<float2*>(alias), nullptr, &err);
+        if (e == hipErrorInvalidValue) throw Error(PT_ERR_INVALID_ARG, err.empty() ? "weights sum to zero" : err);
+        hip_check(e, "alias table on the GPU");
     });
 }
 

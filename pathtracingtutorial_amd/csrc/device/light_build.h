@@ -1,0 +1,26 @@
+// Light-selection tables built on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace ptd {
+
+// Alias table + pmf of n non-negative weights in device memory: pmf[i] =
+// w[i] / sum(w); alias[i] = {prob, index bits} in the layout sample_alias
+// reads (u2 < prob keeps bucket i, else alias).  The table gives each i the
+// probability pmf[i], like the reference's Vose construction
+// (lights/light_sampler.cpp:7-55); its layout differs (Vose is sequential;
+// this one matches light buckets to heavy buckets along prefix sums of their
+// deficits and surpluses, in parallel).  Synchronous on `stream`.
+hipError_t build_alias_device(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
+                              std::string* err);
+
+// The environment's tables (light_sampler.cpp:112-149 weights: luminance *
+// sin(theta) of the texel centre) from its device texels.
+hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, float2* alias, hipStream_t stream,
+                                   std::string* err);
+
+}  // namespace ptd

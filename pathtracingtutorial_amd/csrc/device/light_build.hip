@@ -1,0 +1,295 @@
+// Parallel alias-table construction (see light_build.h).
+//
+// With p_i = w_i * n / W, bucket i is "light" when p_i < 1 (it needs 1 - p_i
+// from another item) and "heavy" otherwise (it has p_i - 1 to give).  Laying
+// the light deficits end to end in index order gives a line [0, T); the heavy
+// surpluses, also in index order, tile the same line.  Light l whose deficit
+// starts at s_l takes its alias from the heavy h whose surplus interval
+// [S_h, S_{h+1}) contains s_l.  A light interval that runs past S_{h+1}
+// overdraws heavy h by e - S_{h+1} (e = the light interval's end); heavy h's
+// own bucket then keeps 1 - (e - S_{h+1}) of itself and takes the rest from
+// heavy h+1, whose surplus interval starts exactly where that overdraw
+// begins.  Every item ends with total mass p_i across the buckets, so the
+// table samples pmf exactly (up to the rounding of the sums, carried in
+// double precision: this is a table build, not a render kernel).
+#include "light_build.h"
+
+#include <vector>
+
+namespace ptd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kPerThread = 8;
+constexpr int kTile = kBlock * kPerThread;
+
+struct Acc {
+    uint32_t lights;  // light items
+    double def;       // light deficits
+    double sur;       // heavy surpluses
+};
+__device__ __forceinline__ Acc acc_add(const Acc& a, const Acc& b) { return Acc{a.lights + b.lights, a.def + b.def, a.sur + b.sur}; }
+
+__device__ __forceinline__ Acc item_acc(const double* p, uint32_t i, uint32_t n) {
+    if (i >= n) return Acc{0u, 0.0, 0.0};
+    const double v = p[i];
+    return v < 1.0 ? Acc{1u, 1.0 - v, 0.0} : Acc{0u, 0.0, v - 1.0};
+}
+
+// Inclusive scan of one value per thread over the block (Hillis-Steele in LDS).
+__device__ Acc block_scan(Acc v, Acc* sh) {
+    const int t = static_cast<int>(threadIdx.x);
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        Acc add = t >= off ? sh[t - off] : Acc{0u, 0.0, 0.0};
+        __syncthreads();
+        if (t >= off) sh[t] = acc_add(sh[t], add);
+        __syncthreads();
+    }
+    const Acc r = sh[t];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_sum(double v, double* sh) {
+    const int t = static_cast<int>(threadIdx.x);
+    sh[t] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_env_weights(const float4* __restrict__ env, int w, int h, float* wts) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = static_cast<uint32_t>(w) * static_cast<uint32_t>(h);
+    if (i >= n) return;
+    const int y = static_cast<int>(i / static_cast<uint32_t>(w));
+    const float theta = 3.14159265358979323846f * (static_cast<float>(y) + 0.5f) / static_cast<float>(h);
+    const float st = fmaxf(sinf(theta), 0.0f);
+    const float4 c = env[i];
+    wts[i] = fmaxf(0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z, 0.0f) * st;
+}
+
+// Partial sums of the weights per tile.
+__global__ __launch_bounds__(kBlock) void k_sum_tiles(const float* __restrict__ w, uint32_t n, double* part) {
+    __shared__ double sh[kBlock];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x;
+    double v = 0.0;
+    for (int k = 0; k < kPerThread; ++k) {
+        const uint32_t i = base + k * kBlock;
+        if (i < n) v += static_cast<double>(fmaxf(w[i], 0.0f));
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final(const double* part, uint32_t np, double* total) {
+    __shared__ double sh[kBlock];
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += kBlock) v += part[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) *total = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(const float* __restrict__ w, uint32_t n, const double* total,
+                                                  double* p, float* pmf) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double W = *total;
+    const double wi = static_cast<double>(fmaxf(w[i], 0.0f));
+    p[i] = wi * static_cast<double>(n) / W;
+    pmf[i] = static_cast<float>(wi / W);
+}
+
+// Scan phase 1: per-tile totals of (lights, deficits, surpluses).
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const double* __restrict__ p, uint32_t n, Acc* part) {
+    __shared__ Acc sh[kBlock];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPerThread;
+    Acc v{0u, 0.0, 0.0};
+    for (int k = 0; k < kPerThread; ++k) v = acc_add(v, item_acc(p, base + k, n));
+    v = block_scan(v, sh);
+    if (threadIdx.x == kBlock - 1) part[blockIdx.x] = v;
+}
+
+// Scan phase 2: exclusive offsets of the tiles (one block); the grand total in part[np].
+__global__ __launch_bounds__(kBlock) void k_scan_parts(Acc* part, uint32_t np) {
+    __shared__ Acc sh[kBlock];
+    Acc carry{0u, 0.0, 0.0};
+    for (uint32_t b0 = 0; b0 < np; b0 += kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const Acc v = i < np ? part[i] : Acc{0u, 0.0, 0.0};
+        const Acc inc = block_scan(v, sh);
+        if (i < np) {
+            Acc ex = acc_add(carry, inc);
+            ex.lights -= v.lights;
+            ex.def -= v.def;
+            ex.sur -= v.sur;
+            part[i] = ex;
+        }
+        const Acc last = sh[kBlock - 1];
+        __syncthreads();
+        carry = acc_add(carry, last);
+        (void)last;
+    }
+    if (threadIdx.x == 0) part[np] = carry;
+}
+
+// Scan phase 3: every item's exclusive prefix -> light / heavy lists.
+__global__ __launch_bounds__(kBlock) void k_scan_scatter(const double* __restrict__ p, uint32_t n, const Acc* part,
+                                                         uint32_t* light_idx, double* light_start, double* light_def,
+                                                         uint32_t* heavy_idx, double* heavy_start) {
+    __shared__ Acc sh[kBlock];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPerThread;
+    Acc own{0u, 0.0, 0.0};
+    for (int k = 0; k < kPerThread; ++k) own = acc_add(own, item_acc(p, base + k, n));
+    const Acc inc = block_scan(own, sh);
+    Acc run = acc_add(part[blockIdx.x], inc);
+    run.lights -= own.lights;
+    run.def -= own.def;
+    run.sur -= own.sur;
+    for (int k = 0; k < kPerThread; ++k) {
+        const uint32_t i = base + k;
+        if (i >= n) break;
+        const Acc a = item_acc(p, i, n);
+        if (a.lights) {
+            light_idx[run.lights] = i;
+            light_start[run.lights] = run.def;
+            light_def[run.lights] = a.def;
+        } else {
+            const uint32_t m = i - run.lights;
+            heavy_idx[m] = i;
+            heavy_start[m] = run.sur;
+        }
+        run = acc_add(run, a);
+    }
+}
+
+// Last index k in [0, m) with a[k] <= x (or, with strict, a[k] < x); -1 if none.
+__device__ __forceinline__ int last_at_most(const double* a, uint32_t m, double x, bool strict) {
+    int lo = 0, hi = static_cast<int>(m);  // answer in [lo - 1, hi - 1]
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const bool ok = strict ? a[mid] < x : a[mid] <= x;
+        if (ok) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_assign_lights(const double* __restrict__ p, const Acc* totals,
+                                                          const uint32_t* light_idx, const double* light_start,
+                                                          const uint32_t* heavy_idx, const double* heavy_start,
+                                                          uint32_t n, float2* table) {
+    const uint32_t L = totals->lights, H = n - L;
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= L) return;
+    const uint32_t i = light_idx[k];
+    if (H == 0) {  // only possible through rounding: keep the item
+        table[i] = make_float2(1.0f, __uint_as_float(i));
+        return;
+    }
+    int m = last_at_most(heavy_start, H, light_start[k], false);
+    if (m < 0) m = 0;
+    table[i] = make_float2(static_cast<float>(p[i]), __uint_as_float(heavy_idx[m]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_assign_heavies(const Acc* totals, const uint32_t* light_idx,
+                                                           const double* light_start, const double* light_def,
+                                                           const uint32_t* heavy_idx, const double* heavy_start,
+                                                           uint32_t n, float2* table) {
+    const uint32_t L = totals->lights, H = n - L;
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= H) return;
+    const uint32_t i = heavy_idx[m];
+    float prob = 1.0f;
+    uint32_t alias = i;
+    if (m + 1 < H && L > 0) {
+        const double x = heavy_start[m + 1];
+        const int k = last_at_most(light_start, L, x, true);
+        if (k >= 0) {
+            const double over = light_start[k] + light_def[k] - x;
+            if (over > 0.0) {
+                prob = static_cast<float>(fmin(fmax(1.0 - over, 0.0), 1.0));
+                alias = heavy_idx[m + 1];
+            }
+        }
+    }
+    table[i] = make_float2(prob, __uint_as_float(alias));
+}
+
+template <class T>
+hipError_t dmalloc(T*& p, size_t n) {
+    return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1));
+}
+
+}  // namespace
+
+hipError_t build_alias_device(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
+                              std::string* err) {
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+    double *part_sum = nullptr, *total = nullptr, *p = nullptr, *light_start = nullptr, *light_def = nullptr,
+           *heavy_start = nullptr;
+    Acc* part = nullptr;
+    uint32_t *light_idx = nullptr, *heavy_idx = nullptr;
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+        return e == hipSuccess;
+    };
+    if (ok(dmalloc(part_sum, tiles)) && ok(dmalloc(total, 1)) && ok(dmalloc(p, n)) && ok(dmalloc(part, tiles + 1)) &&
+        ok(dmalloc(light_idx, n)) && ok(dmalloc(light_start, n)) && ok(dmalloc(light_def, n)) &&
+        ok(dmalloc(heavy_idx, n)) && ok(dmalloc(heavy_start, n))) {
+        hipLaunchKernelGGL(k_sum_tiles, dim3(tiles), dim3(kBlock), 0, stream, weights, n, part_sum);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, part_sum, tiles, total);
+        double W = 0.0;
+        if (ok(hipMemcpyAsync(&W, total, sizeof(double), hipMemcpyDeviceToHost, stream)) && ok(hipStreamSynchronize(stream))) {
+            if (!(W > 0.0)) {
+                if (err) *err = "alias table: the weights sum to zero";
+                e = hipErrorInvalidValue;
+            }
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_scale, dim3(blocks), dim3(kBlock), 0, stream, weights, n, total, p, pmf);
+            hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, stream, p, n, part);
+            hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kBlock), 0, stream, part, tiles);
+            hipLaunchKernelGGL(k_scan_scatter, dim3(tiles), dim3(kBlock), 0, stream, p, n, part, light_idx, light_start,
+                               light_def, heavy_idx, heavy_start);
+            hipLaunchKernelGGL(k_assign_lights, dim3(blocks), dim3(kBlock), 0, stream, p, part + tiles, light_idx,
+                               light_start, heavy_idx, heavy_start, n, alias);
+            hipLaunchKernelGGL(k_assign_heavies, dim3(blocks), dim3(kBlock), 0, stream, part + tiles, light_idx,
+                               light_start, light_def, heavy_idx, heavy_start, n, alias);
+            ok(hipGetLastError());
+            ok(hipStreamSynchronize(stream));
+        }
+    }
+    void* bufs[] = {part_sum, total, p, part, light_idx, light_start, light_def, heavy_idx, heavy_start};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    return e;
+}
+
+hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, float2* alias, hipStream_t stream,
+                                   std::string* err) {
+    if (w <= 0 || h <= 0 || !env) return hipSuccess;
+    const uint32_t n = static_cast<uint32_t>(w) * static_cast<uint32_t>(h);
+    float* wts = nullptr;
+    hipError_t e = dmalloc(wts, n);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_env_weights, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, env, w, h, wts);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = build_alias_device(wts, n, pmf, alias, stream, err);
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(wts);
+    return e;
+}
+
+}  // namespace ptd

@@ -90,8 +90,14 @@ class DiffRenderer:
             assert e.is_cuda and self.env_shape is not None and tuple(e.shape) == self.env_shape
             check(lib().pt_scene_update_texture(h, ENV, C.c_void_p(e.data_ptr()), e.numel(), 1))
 
-    def rebuild_env_tables(self) -> None:
-        check(lib().pt_scene_rebuild_env_tables(self.scene.handle))
+    def rebuild_env_tables(self, on_device: bool = True) -> None:
+        """Environment sampling tables for the current texels: built on the GPU
+        (default; same distribution, another alias layout) or on the host
+        (the reference's Vose table)."""
+        if on_device:
+            check(lib().pt_scene_build_env_tables_device(self.scene.handle))
+        else:
+            check(lib().pt_scene_rebuild_env_tables(self.scene.handle))
 
     # -- the two passes ------------------------------------------------------------
     def forward_image(self, frame: int) -> torch.Tensor:
