@@ -21,7 +21,7 @@ struct WavefrontConfig {
     int use_graph = 1;          // capture the bounce loop of a chunk in a hipGraph
     int time_stages = 0;        // bracket every launch with HIP events (no graph); see wavefront_stage_times
     int shade_waves = 0;        // shading kernel built for at least this many waves per SIMD (0, 3 or 4)
-    int stack_lds = 32;         // traversal stack entries in LDS (16 or 32); the rest live in device memory
+    int stack_lds = 16;         // traversal stack entries in LDS (16 or 32); the rest live in device memory
     // queue items a wave takes per atomic (64 .. 1024, a power of two) in the
     // closest-hit, shading and shadow kernels: larger grabs cost fewer
     // same-address atomics, smaller ones keep the rays in flight coherent
