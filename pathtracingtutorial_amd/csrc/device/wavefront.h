@@ -26,7 +26,7 @@ struct WavefrontConfig {
     // closest-hit, shading and shadow kernels: larger grabs cost fewer
     // same-address atomics, smaller ones keep the rays in flight coherent
     int grab_extend = 128;
-    int grab_shade = 512;
+    int grab_shade = 1024;
     int grab_shadow = 64;
 };
 

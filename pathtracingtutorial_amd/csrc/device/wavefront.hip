@@ -66,15 +66,24 @@ struct JobDesc {
     uint32_t npix;
 };
 
+// A path's ray, throughput and sampler state: one 64-byte record, so the
+// shading kernel, which visits paths in material order (scattered), fetches
+// one line per path instead of one per field.
+struct PathRec {
+    float4 o;   // xyz origin
+    float4 d;   // xyz direction, w = last bsdf pdf
+    float4 tp;  // xyz throughput, w = bits: depth | last_delta << 31
+    uint4 rng;  // x job, y Sobol index, z Sobol dim or PCG state, w pixel hash
+};
+
 struct WfState {
-    float4* o;    // xyz origin
-    float4* d;    // xyz direction, w = last bsdf pdf
-    float4* tp;   // xyz throughput, w = bits: depth | last_delta << 31
-    float4* lo;   // xyz radiance
-    uint4* rng;   // x job, y Sobol index, z Sobol dim or PCG state, w pixel hash
+    PathRec* path;
+    float4* lo;   // xyz radiance (apart: the shadow kernels and the per-pixel sums touch only it)
     float4* hit;  // t, u, v, triangle bits
     uint32_t* q[2];
     uint32_t* sorted;
+    uint32_t* qkey;       // material key of each ray-queue item, in queue order (written by the closest-hit kernel)
+    uint32_t key_bins;    // material bins + the miss bin when sorting, else 0
     float4 *sa_o, *sa_d, *sa_c;  // area shadow queue: origin + tmax, direction + path bits, contribution
     float4 *se_o, *se_d, *se_c;  // environment shadow queue
     float4 *sp_o, *sp_d, *sp_c;  // punctual shadow queue
@@ -131,7 +140,7 @@ __device__ __forceinline__ uint32_t job_of_pixel(const WfState& w, uint32_t gp) 
 }
 
 __device__ __forceinline__ uint32_t path_job(const WfState& w, uint32_t p) {
-    return reinterpret_cast<const uint32_t*>(w.rng)[4 * static_cast<size_t>(p)];
+    return w.path[p].rng.x;
 }
 
 // Wave-aggregated append: every lane of the wave must call it.
@@ -214,7 +223,8 @@ struct QueueWalk {
 };
 
 __device__ __forceinline__ void load_path(const WfState& w, uint32_t p, PathState& st, uint4& r) {
-    float4 o = w.o[p], d = w.d[p], tp = w.tp[p], lo = w.lo[p];
+    const PathRec& R = w.path[p];
+    float4 o = R.o, d = R.d, tp = R.tp, lo = w.lo[p];
     st.o = f3(o);
     st.d = f3(d);
     st.last_pdf = d.w;
@@ -223,16 +233,17 @@ __device__ __forceinline__ void load_path(const WfState& w, uint32_t p, PathStat
     st.depth = static_cast<int>(bits & 0xFFFFu);
     st.last_delta = (bits >> 31) != 0;
     st.lo = f3(lo);
-    r = w.rng[p];
+    r = R.rng;
 }
 
 __device__ __forceinline__ void store_path(const WfState& w, uint32_t p, const PathState& st, uint4 r) {
-    w.o[p] = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
-    w.d[p] = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
+    PathRec& R = w.path[p];
+    R.o = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
+    R.d = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
     uint32_t bits = static_cast<uint32_t>(st.depth) | (st.last_delta ? 0x80000000u : 0u);
-    w.tp[p] = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
+    R.tp = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
     w.lo[p] = make_float4(st.lo.x, st.lo.y, st.lo.z, 0.0f);
-    w.rng[p] = r;
+    R.rng = r;
 }
 
 __device__ __forceinline__ Sampler sampler_of(int type, uint4 r) {
@@ -347,9 +358,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
         if (i < end) {
             uint32_t p = q[i];
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
-            float4 o = w.o[p], d = w.d[p];
+            const float4 o = w.path[p].o, d = w.path[p].d;
             Hit h = trace_closest_s<COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
             w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+            // the sort's key, stored in queue order so the sort reads it contiguously
+            if (w.key_bins)
+                w.qkey[i] = h.tri == 0xFFFFFFFFu ? w.key_bins - 1 : min(s.faces[h.tri].w, w.key_bins - 2);
             rays++;
         }
     }
@@ -360,14 +374,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
 // Block j sorts part j / kSeg of segment j % kSeg: per-block histograms in
 // LDS, one scan per segment over (bin, part), then a scatter that ranks the
 // block's items with LDS atomics.  Misses go to the last bin; the key is the
-// material index (of whichever job the path belongs to).
-
-__device__ __forceinline__ uint32_t material_key(const WfState& w, uint32_t p, uint32_t nbins) {
-    uint32_t tri = __float_as_uint(w.hit[p].w);
-    if (tri == 0xFFFFFFFFu) return nbins - 1;
-    const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
-    return min(s.faces[tri].w, nbins - 2);
-}
+// material index (of whichever job the path belongs to), written per queue
+// item by the closest-hit kernel (qkey).
 
 __device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* counts, int seg, int part, uint32_t& b0,
                                                 uint32_t& b1) {
@@ -385,8 +393,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, int in_slot, u
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
     sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
-    const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[material_key(w, q[i], nbins)], 1u);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[w.qkey[i]], 1u);
     __syncthreads();
     uint32_t* out = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
@@ -437,10 +444,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot,
     uint32_t b0, b1;
     sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
     const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-        uint32_t p = q[i];
-        w.sorted[atomicAdd(&off[material_key(w, p, nbins)], 1u)] = p;
-    }
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) w.sorted[atomicAdd(&off[w.qkey[i]], 1u)] = q[i];
 }
 
 // Visibility policy of the wavefront: shadow rays go to the queues at once.
@@ -1205,7 +1209,7 @@ public:
     }
     void release() {
         release_graph();
-        void* ps[] = {st.o, st.d, st.tp, st.lo, st.rng, st.hit, st.q[0], st.q[1], st.sorted, st.sa_o, st.sa_d,
+        void* ps[] = {st.path, st.lo, st.hit, st.q[0], st.q[1], st.sorted, st.qkey, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
                       d_jobs, st.spill, st.adj_L, st.adj_g, st.pend, st.sa_x, st.se_x, st.sp_x, d_grads};
         for (void* p : ps)
@@ -1262,9 +1266,9 @@ public:
     bool ensure(uint32_t npaths, uint32_t npix, uint32_t nlights, uint32_t nmat, uint32_t njobs) {
         bool ok = true;
         if (npaths > capacity) {
-            ok = ok && dalloc(st.o, npaths) && dalloc(st.d, npaths) && dalloc(st.tp, npaths) &&
-                 dalloc(st.lo, npaths) && dalloc(st.rng, npaths) && dalloc(st.hit, npaths) &&
+            ok = ok && dalloc(st.path, npaths) && dalloc(st.lo, npaths) && dalloc(st.hit, npaths) &&
                  dalloc(st.q[0], npaths) && dalloc(st.q[1], npaths) && dalloc(st.sorted, npaths) &&
+                 dalloc(st.qkey, npaths) &&
                  dalloc(st.sa_o, npaths) && dalloc(st.sa_d, npaths) && dalloc(st.sa_c, npaths) &&
                  dalloc(st.se_o, npaths) && dalloc(st.se_d, npaths) && dalloc(st.se_c, npaths);
             capacity = npaths;
@@ -1635,6 +1639,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.area = any_area;
     cs.env = any_env;
     cs.adjoint = G != nullptr;
+    wf->st.key_bins = cs.sort ? cs.nbins : 0u;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
         hipSuccess)
