@@ -436,6 +436,22 @@ pt_status pt_scene_rebuild_env_tables(pt_scene* scene);
  * given random number picks another texel (the estimator is unchanged).
  * For inverse rendering, where the environment changes every iteration. */
 pt_status pt_scene_build_env_tables_device(pt_scene* scene);
+/* Emission updates of an uploaded scene: material emissive color
+ * (num_materials x 3) and strength (num_materials) from host arrays (NULL
+ * keeps the field).  The emissive-triangle sampling tables keep their old
+ * distribution until one of the two rebuilds below. */
+pt_status pt_scene_update_emission(pt_scene* scene, const float* emissive, const float* strength);
+/* Rebuild of the emissive-triangle tables on the host, bit-identical to the
+ * reference's (lights/light_sampler.cpp:57-110, called from
+ * scene/scene_manager.cpp:31-34 at load), and uploaded. */
+pt_status pt_scene_rebuild_emissive_tables(pt_scene* scene);
+/* The same tables built on the GPU from the device vertices, BVH-ordered
+ * faces and materials (no host round trip): the emissive list (face indices
+ * in face order) equals the host build's, the per-face pmf equals it up to
+ * rounding, and the alias table samples the same distribution in another
+ * layout.  The number of emissive faces may change (an emission set to 0
+ * drops its faces).  For inverse rendering of emission. */
+pt_status pt_scene_build_emissive_tables_device(pt_scene* scene);
 /* Alias table of n >= 1 non-negative device weights (not all zero), built on
  * the GPU: pmf[i] = w[i] / sum (n floats), alias = n {prob, index bits}
  * pairs (2n floats), all device memory.  Synchronous. */

@@ -176,6 +176,31 @@ class SceneManager:
                                               fptr(ea), C.byref(c)))
         return {"tri_pmf": tp, "tri_alias": ta, "env_pmf": ep, "env_alias": ea}
 
+    def updateEmission(self, emissive=None, strength=None) -> None:
+        """Sets the materials' emissive color (num_materials x 3) and/or strength
+        (num_materials) of a built scene; the light tables keep their old
+        distribution until rebuildEmissiveTables."""
+        def arr(a, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1))
+            if a.size != n:
+                raise ValueError(f"expected {n} values, got {a.size}")
+            return a
+        nm = self.info().num_materials
+        e, s = arr(emissive, 3 * nm), arr(strength, nm)
+        check(lib().pt_scene_update_emission(self.handle, fptr(e) if e is not None else None,
+                                             fptr(s) if s is not None else None))
+
+    def rebuildEmissiveTables(self, on_device: bool = True) -> None:
+        """Emissive-triangle sampling tables for the current emission: built on
+        the GPU (default; same distribution, another alias layout) or on the
+        host (the reference's Vose table)."""
+        if on_device:
+            check(lib().pt_scene_build_emissive_tables_device(self.handle))
+        else:
+            check(lib().pt_scene_rebuild_emissive_tables(self.handle))
+
     def wavefrontConfig(self) -> dict:
         c = _capi.WavefrontConfig()
         check(lib().pt_scene_get_wavefront_config(self.handle, C.byref(c)))

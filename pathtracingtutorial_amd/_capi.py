@@ -234,6 +234,9 @@ SIGNATURES = {
     "pt_scene_update_texture": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_size_t, C.c_int]),
     "pt_scene_rebuild_env_tables": (C.c_int, [_P]),
     "pt_scene_build_env_tables_device": (C.c_int, [_P]),
+    "pt_scene_update_emission": (C.c_int, [_P, _FP, _FP]),
+    "pt_scene_rebuild_emissive_tables": (C.c_int, [_P]),
+    "pt_scene_build_emissive_tables_device": (C.c_int, [_P]),
     "pt_alias_build_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pt_counters_enable": (C.c_int, [_P, C.c_int]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),

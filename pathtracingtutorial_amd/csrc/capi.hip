@@ -137,6 +137,7 @@ struct pt_scene {
     ptd::WavefrontConfig wf_cfg;
     int adjoint_integrator = PT_INTEGRATOR_WAVEFRONT;
     bool env_tables_on_device = false;  // the device env tables were built on the GPU (host copy stale)
+    bool emissive_tables_on_device = false;  // likewise for the emissive-triangle tables
     uint64_t generation = 0;  // bumped by every upload; keys the captured graph
 
     ~pt_scene() {
@@ -503,7 +504,8 @@ pt_status pt_scene_get_info(const pt_scene* s, pt_scene_info* info) {
         info->num_materials = static_cast<uint32_t>(s->host.materials.size());
         info->num_textures = static_cast<uint32_t>(s->host.textures.size());
         info->num_lights = static_cast<uint32_t>(s->host.lights.size());
-        info->num_emissive_triangles = static_cast<uint32_t>(s->lights.emissive.size());
+        info->num_emissive_triangles = static_cast<uint32_t>(s->emissive_tables_on_device ? s->emissive.n
+                                                                                          : s->lights.emissive.size());
         info->num_bvh_nodes = s->nodes.p ? static_cast<uint32_t>(s->nodes.n / 2) : 0;
         info->bvh_depth = s->depth;
         info->max_leaf_size = s->max_leaf;
@@ -582,6 +584,23 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
     });
 }
 
+namespace {
+void upload_emissive_tables(pt_scene* s) {
+    const ptamd::LightTables& L = s->lights;
+    std::vector<uint4> em(L.emissive.size());
+    for (size_t i = 0; i < em.size(); ++i) {
+        uint32_t pmf, cdf;
+        std::memcpy(&pmf, &L.emissive[i].pmf, 4);
+        std::memcpy(&cdf, &L.emissive[i].cdf, 4);
+        em[i] = make_uint4(L.emissive[i].tri, pmf, cdf, 0u);
+    }
+    s->emissive.upload(em.data(), em.size(), s->stream);
+    s->tri_pmf.upload(L.tri_pmf.data(), L.tri_pmf.size(), s->stream);
+    s->tri_alias.upload(reinterpret_cast<const float2*>(L.tri_alias.data()), L.tri_alias.size(), s->stream);
+    s->emissive_tables_on_device = false;
+}
+}  // namespace
+
 pt_status pt_scene_upload(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
@@ -644,16 +663,7 @@ pt_status pt_scene_upload(pt_scene* s) {
             s->env_w = s->env_h = 0;
         }
         const ptamd::LightTables& L = s->lights;
-        std::vector<uint4> em(L.emissive.size());
-        for (size_t i = 0; i < em.size(); ++i) {
-            uint32_t pmf, cdf;
-            std::memcpy(&pmf, &L.emissive[i].pmf, 4);
-            std::memcpy(&cdf, &L.emissive[i].cdf, 4);
-            em[i] = make_uint4(L.emissive[i].tri, pmf, cdf, 0u);
-        }
-        s->emissive.upload(em.data(), em.size(), s->stream);
-        s->tri_pmf.upload(L.tri_pmf.data(), L.tri_pmf.size(), s->stream);
-        s->tri_alias.upload(reinterpret_cast<const float2*>(L.tri_alias.data()), L.tri_alias.size(), s->stream);
+        upload_emissive_tables(s);
         s->env_pmf.upload(L.env_pmf.data(), L.env_pmf.size(), s->stream);
         s->env_alias.upload(reinterpret_cast<const float2*>(L.env_alias.data()), L.env_alias.size(), s->stream);
         if (L.env_pmf.empty()) s->env_pmf.reset(), s->env_alias.reset();
@@ -803,16 +813,24 @@ pt_status pt_scene_get_light_tables(const pt_scene* s, float* tri_pmf, uint32_t*
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
         if (!s->accel_built) throw Error(PT_ERR_STATE, "light tables not built");
         const ptamd::LightTables& L = s->lights;
+        const bool dev = s->emissive_tables_on_device;  // built on the GPU: read the device tables
+        const size_t n_alias = dev ? s->tri_alias.n : L.tri_alias.size();
         if (tri_pmf) {
             if (!n_tri_pmf || *n_tri_pmf < L.tri_pmf.size()) throw Error(PT_ERR_INVALID_ARG, "tri_pmf too small");
-            std::memcpy(tri_pmf, L.tri_pmf.data(), L.tri_pmf.size() * 4);
+            if (dev)
+                hip_check(hipMemcpy(tri_pmf, s->tri_pmf.p, L.tri_pmf.size() * 4, hipMemcpyDeviceToHost), "light tables");
+            else
+                std::memcpy(tri_pmf, L.tri_pmf.data(), L.tri_pmf.size() * 4);
         }
         if (n_tri_pmf) *n_tri_pmf = static_cast<uint32_t>(L.tri_pmf.size());
         if (tri_alias) {
-            if (!n_tri_alias || *n_tri_alias < L.tri_alias.size()) throw Error(PT_ERR_INVALID_ARG, "tri_alias too small");
-            std::memcpy(tri_alias, L.tri_alias.data(), L.tri_alias.size() * 8);
+            if (!n_tri_alias || *n_tri_alias < n_alias) throw Error(PT_ERR_INVALID_ARG, "tri_alias too small");
+            if (dev && n_alias)
+                hip_check(hipMemcpy(tri_alias, s->tri_alias.p, n_alias * 8, hipMemcpyDeviceToHost), "light tables");
+            else if (!dev)
+                std::memcpy(tri_alias, L.tri_alias.data(), n_alias * 8);
         }
-        if (n_tri_alias) *n_tri_alias = static_cast<uint32_t>(L.tri_alias.size());
+        if (n_tri_alias) *n_tri_alias = static_cast<uint32_t>(n_alias);
         if (env_pmf || env_alias) {
             if (!n_env || *n_env < L.env_pmf.size()) throw Error(PT_ERR_INVALID_ARG, "env buffers too small");
             if (s->env_tables_on_device) {  // built on the GPU: read the device tables
@@ -1233,6 +1251,62 @@ pt_status pt_scene_build_env_tables_device(pt_scene* s) {
         if (e == hipErrorInvalidValue) throw Error(PT_ERR_INVALID_ARG, err.empty() ? "environment is all zero" : err);
         hip_check(e, "environment tables on the GPU");
         s->env_tables_on_device = true;
+    });
+}
+
+pt_status pt_scene_update_emission(pt_scene* s, const float* emissive, const float* strength) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        auto& M = s->host.materials;
+        for (size_t i = 0; i < M.size(); ++i) {
+            if (emissive) {
+                M[i].emissive.x = emissive[3 * i + 0];
+                M[i].emissive.y = emissive[3 * i + 1];
+                M[i].emissive.z = emissive[3 * i + 2];
+            }
+            if (strength) M[i].emissive_strength = strength[i];
+        }
+        if (s->uploaded && s->materials.p && s->materials.n == M.size())
+            hip_check(hipMemcpy(s->materials.p, M.data(), sizeof(pt_material) * M.size(), hipMemcpyHostToDevice),
+                      "material update");
+    });
+}
+
+
+pt_status pt_scene_rebuild_emissive_tables(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        const HostScene& h = s->host;
+        const uint32_t nf = static_cast<uint32_t>(s->prim_order_host.size());
+        std::vector<pt_face> faces_bvh(nf);
+        for (uint32_t i = 0; i < nf; ++i) faces_bvh[i] = h.faces[s->prim_order_host[i]];
+        ptamd::build_emissive_tables(h.vertices, faces_bvh, h.materials, s->lights);
+        upload_emissive_tables(s);
+        hip_check(hipStreamSynchronize(s->stream), "emissive table update");
+    });
+}
+
+pt_status pt_scene_build_emissive_tables_device(pt_scene* s) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
+        const uint32_t nf = static_cast<uint32_t>(s->faces_bvh.n);
+        if (nf == 0) return;
+        if (s->tri_pmf.n != nf) throw Error(PT_ERR_STATE, "per-face light pmf not uploaded");
+        uint32_t m = 0;
+        uint4* em = nullptr;
+        float2* table = nullptr;
+        std::string err;
+        hipError_t e = ptd::build_emissive_tables_device(s->vpos.p, s->faces_bvh.p, nf, s->materials.p,
+                                                         static_cast<uint32_t>(s->materials.n), s->tri_pmf.p, &m, &em,
+                                                         &table, s->stream, &err);
+        if (e != hipSuccess) {
+            hip_check(e, err.empty() ? "emissive tables on the GPU" : err.c_str());
+        }
+        s->emissive.adopt(em, m);
+        s->tri_alias.adopt(table, m);
+        s->emissive_tables_on_device = true;
     });
 }
 

@@ -6,6 +6,8 @@
 
 #include <string>
 
+#include "pt_amd.h"
+
 namespace ptd {
 
 // Alias table + pmf of n non-negative weights in device memory: pmf[i] =
@@ -22,5 +24,17 @@ hipError_t build_alias_device(const float* weights, uint32_t n, float* pmf, floa
 // sin(theta) of the texel centre) from its device texels.
 hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, float2* alias, hipStream_t stream,
                                    std::string* err);
+
+// The emissive-triangle tables (light_sampler.cpp:57-110) from the device
+// geometry: faces {v0, v1, v2, material} in BVH order over vpos, and the
+// materials.  Weight = area * max(luminance(emissive * strength), 0) for
+// faces whose material emits; the emissive list holds {face, pmf bits, cdf
+// bits, 0} in face order like the host build, tri_pmf (nf floats, zeroed
+// here) each face's selection pmf.  *emissive and *alias (*count entries)
+// are allocated here and owned by the caller; both stay null, and *count 0,
+// when no face emits.  Synchronous on `stream`.
+hipError_t build_emissive_tables_device(const float4* vpos, const uint4* faces, uint32_t nf, const pt_material* mats,
+                                        uint32_t nmat, float* tri_pmf, uint32_t* count, uint4** emissive,
+                                        float2** alias, hipStream_t stream, std::string* err);
 
 }  // namespace ptd

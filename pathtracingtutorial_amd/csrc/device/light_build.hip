@@ -31,6 +31,13 @@ struct Acc {
 };
 __device__ __forceinline__ Acc acc_add(const Acc& a, const Acc& b) { return Acc{a.lights + b.lights, a.def + b.def, a.sur + b.sur}; }
 
+// Emissive faces: how many, and their weight.
+struct Cnt {
+    uint32_t n;
+    double w;
+};
+__device__ __forceinline__ Cnt acc_add(const Cnt& a, const Cnt& b) { return Cnt{a.n + b.n, a.w + b.w}; }
+
 __device__ __forceinline__ Acc item_acc(const double* p, uint32_t i, uint32_t n) {
     if (i >= n) return Acc{0u, 0.0, 0.0};
     const double v = p[i];
@@ -38,17 +45,18 @@ __device__ __forceinline__ Acc item_acc(const double* p, uint32_t i, uint32_t n)
 }
 
 // Inclusive scan of one value per thread over the block (Hillis-Steele in LDS).
-__device__ Acc block_scan(Acc v, Acc* sh) {
+template <class T>
+__device__ T block_scan(T v, T* sh) {
     const int t = static_cast<int>(threadIdx.x);
     sh[t] = v;
     __syncthreads();
     for (int off = 1; off < kBlock; off <<= 1) {
-        Acc add = t >= off ? sh[t - off] : Acc{0u, 0.0, 0.0};
+        T add = t >= off ? sh[t - off] : T{};
         __syncthreads();
         if (t >= off) sh[t] = acc_add(sh[t], add);
         __syncthreads();
     }
-    const Acc r = sh[t];
+    const T r = sh[t];
     __syncthreads();
     return r;
 }
@@ -224,6 +232,99 @@ __global__ __launch_bounds__(kBlock) void k_assign_heavies(const Acc* totals, co
     table[i] = make_float2(prob, __uint_as_float(alias));
 }
 
+// light_sampler.cpp:67-81: area * max(luminance(emissive * strength), 0) for
+// faces whose material emits (material.h:83-86), 0 for the others.
+__global__ __launch_bounds__(kBlock) void k_emissive_weights(const float4* __restrict__ vpos,
+                                                             const uint4* __restrict__ faces, uint32_t nf,
+                                                             const pt_material* __restrict__ mats, uint32_t nmat,
+                                                             float* wts) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nf) return;
+    const uint4 f = faces[i];
+    float w = 0.0f;
+    if (f.w < nmat) {  // a negative material id reads as a large unsigned one
+        const pt_material& m = mats[f.w];
+        if (m.emissive_strength > 0.0f && (m.emissive.x > 0.0f || m.emissive.y > 0.0f || m.emissive.z > 0.0f)) {
+            const float4 a = vpos[f.x], b = vpos[f.y], c = vpos[f.z];
+            const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+            const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+            const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+            const float area = 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);
+            const float s = m.emissive_strength;
+            const float le = 0.2126f * (m.emissive.x * s) + 0.7152f * (m.emissive.y * s) + 0.0722f * (m.emissive.z * s);
+            if (area > 0.0f) w = fmaxf(area * fmaxf(le, 0.0f), 0.0f);
+        }
+    }
+    wts[i] = w;
+}
+
+__device__ __forceinline__ Cnt face_cnt(const float* w, uint32_t i, uint32_t n) {
+    if (i >= n) return Cnt{0u, 0.0};
+    const float v = w[i];
+    return v > 0.0f ? Cnt{1u, static_cast<double>(v)} : Cnt{0u, 0.0};
+}
+
+__global__ __launch_bounds__(kBlock) void k_cnt_tiles(const float* __restrict__ w, uint32_t n, Cnt* part) {
+    __shared__ Cnt sh[kBlock];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPerThread;
+    Cnt v{0u, 0.0};
+    for (int k = 0; k < kPerThread; ++k) v = acc_add(v, face_cnt(w, base + k, n));
+    v = block_scan(v, sh);
+    if (threadIdx.x == kBlock - 1) part[blockIdx.x] = v;
+}
+
+// Exclusive offsets of the tiles (one block); the grand total in part[np].
+__global__ __launch_bounds__(kBlock) void k_cnt_parts(Cnt* part, uint32_t np) {
+    __shared__ Cnt sh[kBlock];
+    Cnt carry{0u, 0.0};
+    for (uint32_t b0 = 0; b0 < np; b0 += kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const Cnt v = i < np ? part[i] : Cnt{0u, 0.0};
+        const Cnt inc = block_scan(v, sh);
+        if (i < np) part[i] = Cnt{carry.n + inc.n - v.n, carry.w + inc.w - v.w};
+        const Cnt last = sh[kBlock - 1];
+        __syncthreads();
+        carry = acc_add(carry, last);
+    }
+    if (threadIdx.x == 0) part[np] = carry;
+}
+
+// Emitting faces in face order: their weights, face indices and cdf values
+// (running weight / total; the last entry is 1, light_sampler.cpp:93-105).
+__global__ __launch_bounds__(kBlock) void k_cnt_compact(const float* __restrict__ w, uint32_t n, const Cnt* part,
+                                                        float* wc, uint32_t* face, float* cdf) {
+    __shared__ Cnt sh[kBlock];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPerThread;
+    Cnt own{0u, 0.0};
+    for (int k = 0; k < kPerThread; ++k) own = acc_add(own, face_cnt(w, base + k, n));
+    const Cnt inc = block_scan(own, sh);
+    const Cnt tot = part[gridDim.x];
+    Cnt run = acc_add(part[blockIdx.x], inc);
+    run.n -= own.n;
+    run.w -= own.w;
+    for (int k = 0; k < kPerThread; ++k) {
+        const uint32_t i = base + k;
+        if (i >= n) break;
+        const Cnt a = face_cnt(w, i, n);
+        if (a.n) {
+            run = acc_add(run, a);
+            wc[run.n - 1] = w[i];
+            face[run.n - 1] = i;
+            cdf[run.n - 1] = run.n == tot.n ? 1.0f : static_cast<float>(run.w / tot.w);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_emissive_refs(const uint32_t* __restrict__ face, const float* __restrict__ pmf,
+                                                          const float* __restrict__ cdf, uint32_t m, uint4* refs,
+                                                          float* tri_pmf) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t f = face[k];
+    refs[k] = make_uint4(f, __float_as_uint(pmf[k]), __float_as_uint(cdf[k]), 0u);
+    tri_pmf[f] = pmf[k];
+}
+
 template <class T>
 hipError_t dmalloc(T*& p, size_t n) {
     return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1));
@@ -289,6 +390,58 @@ hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, 
     if (e == hipSuccess) e = build_alias_device(wts, n, pmf, alias, stream, err);
     (void)hipStreamSynchronize(stream);
     (void)hipFree(wts);
+    return e;
+}
+
+hipError_t build_emissive_tables_device(const float4* vpos, const uint4* faces, uint32_t nf, const pt_material* mats,
+                                        uint32_t nmat, float* tri_pmf, uint32_t* count, uint4** emissive,
+                                        float2** alias, hipStream_t stream, std::string* err) {
+    *count = 0;
+    *emissive = nullptr;
+    *alias = nullptr;
+    if (nf == 0) return hipSuccess;
+    const uint32_t tiles = (nf + kTile - 1) / kTile;
+    const uint32_t blocks = (nf + kBlock - 1) / kBlock;
+    float *w = nullptr, *wc = nullptr, *cdf = nullptr, *pmf = nullptr;
+    uint32_t* face = nullptr;
+    Cnt* part = nullptr;
+    uint4* refs = nullptr;
+    float2* table = nullptr;
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+        return e == hipSuccess;
+    };
+    Cnt tot{0u, 0.0};
+    if (ok(dmalloc(w, nf)) && ok(dmalloc(part, tiles + 1)) && ok(hipMemsetAsync(tri_pmf, 0, sizeof(float) * nf, stream))) {
+        hipLaunchKernelGGL(k_emissive_weights, dim3(blocks), dim3(kBlock), 0, stream, vpos, faces, nf, mats, nmat, w);
+        hipLaunchKernelGGL(k_cnt_tiles, dim3(tiles), dim3(kBlock), 0, stream, w, nf, part);
+        hipLaunchKernelGGL(k_cnt_parts, dim3(1), dim3(kBlock), 0, stream, part, tiles);
+        if (ok(hipGetLastError()) && ok(hipMemcpyAsync(&tot, part + tiles, sizeof(Cnt), hipMemcpyDeviceToHost, stream)))
+            ok(hipStreamSynchronize(stream));
+    }
+    const uint32_t m = tot.n;
+    if (e == hipSuccess && m > 0 && ok(dmalloc(wc, m)) && ok(dmalloc(cdf, m)) && ok(dmalloc(pmf, m)) &&
+        ok(dmalloc(face, m)) && ok(dmalloc(refs, m)) && ok(dmalloc(table, m))) {
+        hipLaunchKernelGGL(k_cnt_compact, dim3(tiles), dim3(kBlock), 0, stream, w, nf, part, wc, face, cdf);
+        if (ok(hipGetLastError()) && ok(build_alias_device(wc, m, pmf, table, stream, err))) {
+            hipLaunchKernelGGL(k_emissive_refs, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, face, pmf,
+                               cdf, m, refs, tri_pmf);
+            if (ok(hipGetLastError())) ok(hipStreamSynchronize(stream));
+        }
+    }
+    (void)hipStreamSynchronize(stream);
+    void* bufs[] = {w, wc, cdf, pmf, face, part};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (e == hipSuccess && m > 0) {
+        *count = m;
+        *emissive = refs;
+        *alias = table;
+    } else {
+        if (refs) (void)hipFree(refs);
+        if (table) (void)hipFree(table);
+    }
     return e;
 }
 

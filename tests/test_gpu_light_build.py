@@ -132,3 +132,88 @@ def test_env_tables_on_device_match_host_and_render_alike():
     z = np.where(var > 0, d / np.sqrt(np.maximum(var, 1e-30)), np.where(d > 1e-5, np.inf, 0.0))
     assert (z > 3.5).mean() <= 0.01
     assert abs(a.mean() - b.mean()) <= 3 * np.sqrt(var.mean() / var.size) + 1e-3 * b.mean()
+
+
+def _emissive_exact(sm, emissive, strength):
+    """area * luminance(emissive * strength) weights (light_sampler.cpp:67-81) of the
+    BVH-ordered faces, normalized in double precision."""
+    v = _capi.HostSceneView()
+    _capi.check(pt.lib().pt_scene_host_view(sm.handle, C.byref(v)))
+    verts = np.ctypeslib.as_array(C.cast(v.vertices, C.POINTER(C.c_float)),
+                                  shape=(v.num_vertices * C.sizeof(_capi.Vertex) // 4,))
+    verts = verts.reshape(v.num_vertices, -1)[:, :3].astype(np.float64)
+    faces = np.ctypeslib.as_array(C.cast(v.faces, C.POINTER(C.c_int32)), shape=(v.num_faces * 4,)).reshape(-1, 4)
+    _, order = sm.bvh(0)
+    f = faces[order]
+    a, b, c = verts[f[:, 0]], verts[f[:, 1]], verts[f[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    e = emissive.astype(np.float64) * strength.astype(np.float64)[:, None]
+    lum = 0.2126 * e[:, 0] + 0.7152 * e[:, 1] + 0.0722 * e[:, 2]
+    emits = (strength > 0) & np.any(emissive > 0, axis=1)
+    w = np.where(emits[f[:, 3]], area * np.maximum(lum[f[:, 3]], 0.0), 0.0)
+    return w / w.sum()
+
+
+def test_emissive_tables_on_device_match_host_and_render_alike():
+    sm = _env_scene()
+    nm = sm.info().num_materials
+    assert nm >= 3
+    rng = np.random.default_rng(11)
+    em = (rng.random((nm, 3)) * 4).astype(np.float32)
+    em[::2] = 0.0  # every other material stays dark
+    st = (rng.random(nm) + 0.5).astype(np.float32)
+    sm.updateEmission(em, st)
+    sm.rebuildEmissiveTables(on_device=False)
+    host = sm.lightTables()
+    n_host = sm.info().num_emissive_triangles
+    assert n_host > 100
+    exact = _emissive_exact(sm, em, st)
+    assert np.allclose(host["tri_pmf"], exact, rtol=1e-3, atol=1e-12)
+    sm.rebuildEmissiveTables(on_device=True)
+    dev = sm.lightTables()
+    assert sm.info().num_emissive_triangles == n_host
+    # the same emissive faces, in face order; the same pmf up to rounding
+    assert np.array_equal(dev["tri_pmf"] > 0, exact > 0)
+    assert np.allclose(dev["tri_pmf"], exact, rtol=1e-4, atol=1e-12)
+    sel = dev["tri_pmf"][dev["tri_pmf"] > 0].astype(np.float64)
+    assert np.allclose(_implied(dev["tri_alias"]), sel, rtol=1e-4, atol=1e-10)
+    a = _mean_image(sm, 12)
+    sm.rebuildEmissiveTables(on_device=False)
+    assert np.array_equal(sm.lightTables()["tri_alias"], host["tri_alias"])
+    b = _mean_image(sm, 12)
+    var = (a.var(0, ddof=1) + b.var(0, ddof=1)) / a.shape[0]
+    d = np.abs(a.mean(0) - b.mean(0))
+    z = np.where(var > 0, d / np.sqrt(np.maximum(var, 1e-30)), np.where(d > 1e-5, np.inf, 0.0))
+    assert (z > 3.5).mean() <= 0.01
+    assert abs(a.mean() - b.mean()) <= 3 * np.sqrt(var.mean() / var.size) + 1e-3 * b.mean()
+
+
+def test_emissive_tables_on_device_follow_emission_changes():
+    sm = pt.SceneManager()
+    sm.generate(0)
+    sm.setSamplePerFrame(4)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    host = sm.lightTables()
+    n0 = sm.info().num_emissive_triangles
+    assert n0 == 2
+    sm.rebuildEmissiveTables(on_device=True)
+    dev = sm.lightTables()
+    assert np.array_equal(dev["tri_pmf"] > 0, host["tri_pmf"] > 0)
+    assert np.allclose(dev["tri_pmf"], host["tri_pmf"], rtol=1e-6)
+    assert np.allclose(_implied(dev["tri_alias"]), [0.5, 0.5], rtol=1e-6)
+    # every emission off: no emissive faces, and the render still runs (no area lights)
+    nm = sm.info().num_materials
+    sm.updateEmission(strength=np.zeros(nm, np.float32))
+    sm.rebuildEmissiveTables(on_device=True)
+    assert sm.info().num_emissive_triangles == 0
+    assert not sm.lightTables()["tri_pmf"].any()
+    r = pt.Renderer(sm, pt.WavefrontIntegrator())
+    r.renderBatch()
+    img = r.downloadCurrentImage(1)
+    assert np.isfinite(img).all()
+    # back on: the two faces return
+    sm.updateEmission(strength=np.ones(nm, np.float32))
+    sm.rebuildEmissiveTables(on_device=True)
+    assert sm.info().num_emissive_triangles == n0
+    assert np.allclose(sm.lightTables()["tri_pmf"], host["tri_pmf"], rtol=1e-6)
