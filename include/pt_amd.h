@@ -172,7 +172,7 @@ typedef struct pt_scene_info {
 typedef struct pt_counters {
     uint64_t closest_rays;   /* camera + extension rays */
     uint64_t shadow_rays;    /* any-hit rays */
-    uint64_t node_visits;    /* 32-B node records tested */
+    uint64_t node_visits;    /* 32-B units of node data fetched (a binary node record; a 4-wide node is 2) */
     uint64_t tri_tests;      /* triangle intersection tests */
     uint64_t paths;
     uint64_t stack_overflows;
@@ -188,7 +188,9 @@ enum { PT_INTEGRATOR_PATH = 0, PT_INTEGRATOR_WAVEFRONT = 1 };
 /* Launch configuration of the wavefront integrator (no reference
  * counterpart: the reference has only the per-pixel megakernel,
  * integrators/path.cu:209-255).  Every field changes speed only, never the
- * image: the estimator and its random numbers are the megakernel's. */
+ * estimate: the estimator and its random numbers are the megakernel's (with
+ * bvh_width 4 a ray whose closest hit is an exact-distance tie between
+ * triangles may take the other one). */
 typedef struct pt_wavefront_config {
     int32_t chunk_spp;        /* paths in flight = chunk_spp x all jobs' pixels, at most 2^26 (default 64) */
     int32_t extend_block;     /* threads per block of the traversal kernels: 64, 128 or 256 */
@@ -202,6 +204,9 @@ typedef struct pt_wavefront_config {
     int32_t grab_extend;      /* queue items a wave takes per atomic in the closest-hit kernel: 64..1024, power of 2 */
     int32_t grab_shade;       /* the same for the shading kernel */
     int32_t grab_shadow;      /* the same for the shadow-ray kernels */
+    int32_t bvh_width;        /* 4: traverse the 4-wide BVH with quantized child boxes (built with the
+                                 binary one; the same closest hits up to exact-distance ties), 2: the
+                                 binary BVH in the reference's traversal order (default 4) */
 } pt_wavefront_config;
 
 /* GPU time (ms) and launch count per stage of the last wavefront batch
@@ -303,6 +308,18 @@ pt_status pt_scene_set_texture_data(pt_scene* scene, uint32_t i, const float* rg
  * face_order[i] = original face index of BVH-ordered face i. */
 pt_status pt_scene_get_bvh(const pt_scene* scene, int layout, float* nodes8, uint32_t* num_nodes,
                            uint32_t* face_order, uint32_t* num_faces);
+/* 4-wide BVH collapsed from a binary one in the reference's node layout
+ * (accel/bvh_common.h:16-24, as layout 0 above): 16 words per node
+ * {P.xyz, S.x}, {S.y, S.z, qlo.x, qlo.y}, {qlo.z, qhi.x, qhi.y, qhi.z},
+ * {child words}; child i's box on axis a is [fma(byte i of qlo.a, S.a, P.a),
+ * fma(byte i of qhi.a, S.a, P.a)] and contains the child's exact box; child
+ * words are a node index, 0x80000000 | first triangle for a leaf, or
+ * 0xFFFFFFFF for an unused slot.  max_stack: traversal stack entries a ray
+ * can need.  Host only (the scene build runs the same collapse; it has no
+ * reference counterpart: the reference traverses the binary tree).  Pass
+ * NULL wide16 to query the count. */
+pt_status pt_bvh_collapse_wide(const float* nodes8, uint32_t num_nodes, float* wide16, uint32_t* num_wide,
+                               uint32_t* max_stack);
 /* Emissive-triangle and environment tables (lights/light_sampler.cpp:57-149).
  * Pass NULL to query counts.  alias entries are {float prob, uint32 alias}. */
 pt_status pt_scene_get_light_tables(const pt_scene* scene,
@@ -360,7 +377,9 @@ typedef struct pt_job {
 pt_status pt_render_jobs(const pt_job* jobs, int n, float* ms_out);
 /* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
 pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
-/* Wavefront configuration of a scene (defaults: 8, 128, 128, 0, 1, 1, 0, 0). */
+/* Wavefront configuration of a scene (defaults: chunk_spp 64, blocks 128 and
+ * 128, waves_per_simd 0, sort 1, graph 1, time_stages 0, shade_waves 0,
+ * stack_lds 16, grabs 128 / 1024 / 64, bvh_width 4). */
 pt_status pt_scene_set_wavefront_config(pt_scene* scene, const pt_wavefront_config* cfg);
 pt_status pt_scene_get_wavefront_config(const pt_scene* scene, pt_wavefront_config* cfg);
 /* Autotuner: times up to max_trials configurations (coordinate descent over
@@ -476,7 +495,8 @@ pt_status pt_probe_sampler(int sampler, uint32_t seed, const int32_t* xy_idx, in
  * floats each, host buffers.  Directions are normalised as Ray's constructor
  * does (core/ray.h:13).  t_out gets the hit distance (FLT_MAX on a miss; any-
  * hit: 1 occluded / 0 not), face_out the ORIGINAL face index or -1,
- * bary_out (may be NULL) 2 floats per ray. */
+ * bary_out (may be NULL) 2 floats per ray.  any_hit bit 0: any-hit; bit 1:
+ * traverse the scene's 4-wide BVH instead of the binary one. */
 /* Disney BSDF (disney_bsdf.cuh:53-122 fetch + clamps without textures,
  * :209-321 eval, :325-410 sample) of n materials on the device: dirs9 holds
  * wo, wi, n per case, u3 the three uniforms a sample consumes (lobe choice,

@@ -423,15 +423,29 @@ def probe_sampler(sampler: int, xyi: np.ndarray, dims: int, seed: int = 0):
     return u, f
 
 
-def trace_rays(scene: SceneManager, rays: np.ndarray, any_hit: bool = False):
-    """Device closest/any hit for (n, 8) float32 rays {o, d, tmax, _}."""
+def trace_rays(scene: SceneManager, rays: np.ndarray, any_hit: bool = False, wide: bool = False):
+    """Device closest/any hit for (n, 8) float32 rays {o, d, tmax, _}; wide
+    traverses the 4-wide BVH instead of the binary one."""
     rays = np.ascontiguousarray(rays, np.float32)
     n = rays.shape[0]
     t = np.zeros(n, np.float32)
     face = np.zeros(n, np.int32)
     bary = np.zeros((n, 2), np.float32)
-    check(lib().pt_trace_rays(scene.handle, fptr(rays), n, 1 if any_hit else 0, fptr(t), i32ptr(face), fptr(bary)))
+    flags = (1 if any_hit else 0) | (2 if wide else 0)
+    check(lib().pt_trace_rays(scene.handle, fptr(rays), n, flags, fptr(t), i32ptr(face), fptr(bary)))
     return t, face, bary
+
+
+def collapse_wide(nodes8: np.ndarray):
+    """4-wide BVH (n, 16) float32 (child words and quantized planes as uint32
+    bits) and its traversal stack bound from a binary BVH in the reference's
+    node layout ((m, 8) float32).  Host only."""
+    nodes8 = np.ascontiguousarray(nodes8, np.float32)
+    nw, ms = C.c_uint32(0), C.c_uint32(0)
+    check(lib().pt_bvh_collapse_wide(fptr(nodes8), nodes8.shape[0], None, C.byref(nw), C.byref(ms)))
+    out = np.zeros((nw.value, 16), np.float32)
+    check(lib().pt_bvh_collapse_wide(fptr(nodes8), nodes8.shape[0], fptr(out), C.byref(nw), C.byref(ms)))
+    return out, int(ms.value)
 
 
 def probe_copy(nbytes: int = 1 << 30, iters: int = 10) -> float:

@@ -155,7 +155,8 @@ class WavefrontConfig(C.Structure):
     _fields_ = [("chunk_spp", C.c_int32), ("extend_block", C.c_int32), ("shade_block", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("sort_by_material", C.c_int32), ("use_graph", C.c_int32),
                 ("time_stages", C.c_int32), ("shade_waves", C.c_int32), ("stack_lds", C.c_int32),
-                ("grab_extend", C.c_int32), ("grab_shade", C.c_int32), ("grab_shadow", C.c_int32)]
+                ("grab_extend", C.c_int32), ("grab_shade", C.c_int32), ("grab_shadow", C.c_int32),
+                ("bvh_width", C.c_int32)]
 
 
 class WavefrontStats(C.Structure):
@@ -205,6 +206,7 @@ SIGNATURES = {
     "pt_scene_set_texture_data": (C.c_int, [_P, C.c_uint32, _FP, C.c_size_t]),
     "pt_scene_get_bvh": (C.c_int, [_P, C.c_int, _FP, _U32P, _U32P, _U32P]),
     "pt_scene_get_light_tables": (C.c_int, [_P, _FP, _U32P, _FP, _U32P, _FP, _FP, _U32P]),
+    "pt_bvh_collapse_wide": (C.c_int, [_FP, C.c_uint32, _FP, _U32P, _U32P]),
     "pt_film_create": (C.c_int, [C.c_int, C.c_int, _PP]),
     "pt_film_destroy": (None, [_P]),
     "pt_film_reset": (C.c_int, [_P, _P]),

@@ -19,6 +19,7 @@
 #include "device/light_build.h"
 #include "device/render.h"
 #include "device/wavefront.h"
+#include "host/bvh_wide.h"
 #include "host/host_scene.h"
 #include "pt_amd.h"
 
@@ -114,6 +115,8 @@ struct pt_scene {
     DBuf<float4> vpos;
     DBuf<uint4> faces_orig, faces_bvh;
     DBuf<float4> tris, nodes;
+    DBuf<float4> wnodes;  // 4-wide BVH (empty when the tree could overflow its traversal stack)
+    uint32_t wide_max_stack = 0;
     DBuf<uint32_t> prim_order;
     std::vector<uint32_t> prim_order_host;
     ptd::BvhHostTopology topo;
@@ -218,6 +221,8 @@ ptd::DevScene make_view(const pt_scene* s) {
     v.env = s->env.p;
     v.lights = s->plights.p;
     v.num_nodes = s->nodes.p ? static_cast<uint32_t>(s->nodes.n / 2) : 0;
+    v.wnodes = s->wnodes.p;
+    v.num_wnodes = s->wnodes.p ? static_cast<uint32_t>(s->wnodes.n / 4) : 0;
     v.num_faces = static_cast<uint32_t>(s->host.faces.size());
     v.num_materials = static_cast<uint32_t>(s->host.materials.size());
     v.num_textures = static_cast<uint32_t>(s->texdesc.n);
@@ -347,6 +352,7 @@ void check_wavefront_config(const pt_wavefront_config& c) {
     auto grab_ok = [](int g) { return g >= 64 && g <= 1024 && (g & (g - 1)) == 0; };
     if (!grab_ok(c.grab_extend) || !grab_ok(c.grab_shade) || !grab_ok(c.grab_shadow))
         throw Error(PT_ERR_INVALID_ARG, "grab sizes must be 64, 128, 256, 512 or 1024");
+    if (c.bvh_width != 2 && c.bvh_width != 4) throw Error(PT_ERR_INVALID_ARG, "bvh_width must be 2 or 4");
 }
 
 ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
@@ -363,6 +369,7 @@ ptd::WavefrontConfig to_device_config(const pt_wavefront_config& c) {
     d.grab_extend = c.grab_extend;
     d.grab_shade = c.grab_shade;
     d.grab_shadow = c.grab_shadow;
+    d.bvh_width = c.bvh_width;
     return d;
 }
 
@@ -380,6 +387,7 @@ pt_wavefront_config to_abi_config(const ptd::WavefrontConfig& d) {
     c.grab_extend = d.grab_extend;
     c.grab_shade = d.grab_shade;
     c.grab_shadow = d.grab_shadow;
+    c.bvh_width = d.bvh_width;
     return c;
 }
 
@@ -545,6 +553,8 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
         s->vpos.upload(pos.data(), nv, s->stream);
         s->faces_orig.upload(fo.data(), nf, s->stream);
         s->nodes.reset();
+        s->wnodes.reset();
+        s->wide_max_stack = 0;
         s->tris.reset();
         s->faces_bvh.reset();
         s->prim_order.reset();
@@ -571,6 +581,17 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
                                             s->tris.p, s->faces_bvh.p, s->stream),
                       "emit triangles");
             s->prim_order_host = s->topo.prim_order;
+            // the 4-wide BVH, collapsed from the binary tree on the host
+            std::vector<float> ref_nodes;
+            dfs_layout(s->topo, ref_nodes);
+            ptamd::WideBvh wb;
+            std::string werr;
+            if (!ptamd::collapse_wide_bvh(ref_nodes.data(), static_cast<uint32_t>(ref_nodes.size() / 8), &wb, &werr))
+                throw Error(PT_ERR_STATE, "wide BVH: " + werr);
+            s->wide_max_stack = wb.max_stack;
+            if (wb.num_nodes > 0 && wb.max_stack <= static_cast<uint32_t>(ptd::kWideStack))
+                s->wnodes.upload(reinterpret_cast<const float4*>(wb.words.data()), static_cast<size_t>(wb.num_nodes) * 4,
+                                 s->stream);
         } else {
             s->prim_order_host.clear();
         }
@@ -804,6 +825,22 @@ pt_status pt_scene_get_bvh(const pt_scene* s, int layout, float* nodes8, uint32_
             std::memcpy(face_order, s->prim_order_host.data(), nf * sizeof(uint32_t));
         }
         if (num_faces) *num_faces = nf;
+    });
+}
+
+pt_status pt_bvh_collapse_wide(const float* nodes8, uint32_t num_nodes, float* wide16, uint32_t* num_wide,
+                               uint32_t* max_stack) {
+    return guarded([&] {
+        if (num_nodes > 0 && !nodes8) throw Error(PT_ERR_INVALID_ARG, "null nodes");
+        ptamd::WideBvh wb;
+        std::string err;
+        if (!ptamd::collapse_wide_bvh(nodes8, num_nodes, &wb, &err)) throw Error(PT_ERR_INVALID_ARG, err);
+        if (wide16) {
+            if (!num_wide || *num_wide < wb.num_nodes) throw Error(PT_ERR_INVALID_ARG, "wide node buffer too small");
+            std::memcpy(wide16, wb.words.data(), wb.words.size() * sizeof(float));
+        }
+        if (num_wide) *num_wide = wb.num_nodes;
+        if (max_stack) *max_stack = wb.max_stack;
     });
 }
 
@@ -1388,6 +1425,7 @@ pt_status pt_scene_tune_wavefront(pt_scene* s, int max_trials, pt_wavefront_conf
                     }
                 }
             };
+            try_axis([](ptd::WavefrontConfig& c, int v) { c.bvh_width = v; }, {4, 2});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_extend = v; }, {64, 128, 256});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_shade = v; }, {128, 256, 512, 1024});
             try_axis([](ptd::WavefrontConfig& c, int v) { c.grab_shadow = v; }, {64, 128});
@@ -1489,7 +1527,10 @@ pt_status pt_trace_rays(pt_scene* s, const float* rays8, int n, int any_hit, flo
         d_t.alloc(n);
         d_face.alloc(n);
         if (bary_out) d_b.alloc(static_cast<size_t>(n) * 2);
-        hip_check(ptd::launch_trace_rays(v, d_r.p, n, any_hit, d_t.p, d_face.p, d_b.p, s->stream), "trace launch");
+        const bool wide = (any_hit & 2) != 0;
+        if (wide && !s->wnodes.p) throw Error(PT_ERR_STATE, "the scene has no 4-wide BVH");
+        hip_check(ptd::launch_trace_rays(v, d_r.p, n, any_hit & 1, wide, d_t.p, d_face.p, d_b.p, s->stream),
+                  "trace launch");
         hip_check(hipStreamSynchronize(s->stream), "trace");
         hip_check(hipMemcpy(t_out, d_t.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost), "download");
         hip_check(hipMemcpy(face_out, d_face.p, 4 * static_cast<size_t>(n), hipMemcpyDeviceToHost), "download");

@@ -6,6 +6,8 @@
 //            so one 64 B load fetches both children of a node.
 //            internal: data0 = index of the left child (right = left + 1)
 //            leaf    : data0 = 0x80000000 | first triangle, data1 = count
+//  wnodes  : the same tree collapsed to 4 children per node with quantized
+//            child boxes, 64 B per node (layout in dev_traverse.h).
 //  tris    : 48 B per triangle in BVH order, three float4
 //            {p0.xyz, face | last<<31}, {p1-p0, 0}, {p2-p0, 0}; bit 31 of the
 //            first word marks the last triangle of a leaf; face is the original
@@ -23,6 +25,7 @@ namespace ptd {
 
 constexpr uint32_t kLeafFlag = 0x80000000u;
 constexpr int kStackSize = 32;  // traversal stack entries; the builder caps depth at this
+constexpr int kWideStack = 64;  // the same for the 4-wide BVH; wider trees that could need more are not used
 
 struct TexDesc {
     uint32_t offset;  // first texel in the texel pool
@@ -34,6 +37,7 @@ struct TexDesc {
 
 struct DevScene {
     const float4* nodes;
+    const float4* wnodes;        // 4-wide BVH, 4 float4 per node (null when not built)
     const float4* tris;
     const uint4* faces;
     const float4* vpos;
@@ -52,7 +56,7 @@ struct DevScene {
     const float4* env;           // env texels, RGBA
     const pt_punctual_light* lights;
     uint32_t num_nodes, num_faces, num_materials, num_textures;
-    uint32_t num_emissive, num_env_texels, num_lights, _pad0;
+    uint32_t num_emissive, num_env_texels, num_lights, num_wnodes;
     int32_t env_w, env_h, env_wrap_s, env_wrap_t;
 };
 

@@ -77,27 +77,28 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
 
 // Per-thread traversal stack: the first LDS entries live in LDS
 // (stk[level * stride]), deeper ones in global memory
-// (spill[(level - LDS) * spill_stride]); kStackSize entries in all.  With
-// LDS == kStackSize there is no spill area.
-template <int LDS>
+// (spill[(level - LDS) * spill_stride]); TOTAL entries in all.  With
+// LDS == TOTAL there is no spill area.
+template <int LDS, int TOTAL = kStackSize>
 struct TraceStack {
+    static constexpr int kTotal = TOTAL;
     uint32_t* stk;
     int stride;
     uint32_t* spill;
     uint32_t spill_stride;
     __device__ __forceinline__ void put(int level, uint32_t v) const {
-        if (LDS >= kStackSize || level < LDS) stk[level * stride] = v;
+        if (LDS >= TOTAL || level < LDS) stk[level * stride] = v;
         else spill[static_cast<size_t>(level - LDS) * spill_stride] = v;
     }
     __device__ __forceinline__ uint32_t get(int level) const {
-        if (LDS >= kStackSize || level < LDS) return stk[level * stride];
+        if (LDS >= TOTAL || level < LDS) return stk[level * stride];
         return spill[static_cast<size_t>(level - LDS) * spill_stride];
     }
 };
 
 // Closest hit.
-template <bool COUNT, int LDS>
-__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS>& sk,
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     Hit h;
     h.t = kInf;
@@ -137,7 +138,7 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < kStackSize) sk.put(top++, far);
+                if (top < TOTAL) sk.put(top++, far);
                 else overflow++;
                 cur = near;
                 continue;
@@ -152,9 +153,10 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
 }
 
 // Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).
-template <bool COUNT, int LDS>
-__device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax, const TraceStack<LDS>& sk,
-                                            TraceCounters& cnt, uint32_t& overflow) {
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax,
+                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            uint32_t& overflow) {
     if (s.num_nodes == 0) return false;
     RayInv r = make_ray_inv(o, d);
     float4 rlo = s.nodes[0], rhi = s.nodes[1];
@@ -183,7 +185,7 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < kStackSize) sk.put(top++, far);
+                if (top < TOTAL) sk.put(top++, far);
                 else overflow++;
                 cur = near;
                 continue;
@@ -195,6 +197,168 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
         cur = sk.get(--top);
     }
     return false;
+}
+
+// ---- 4-wide BVH with quantized child boxes --------------------------------
+// Node layout (64 B): {P.xyz, S.x}, {S.y, S.z, qlo.x, qlo.y},
+// {qlo.z, qhi.x, qhi.y, qhi.z}, {child words}; child i's plane on axis a is
+// fma(byte i of q.a, S.a, P.a), a box that contains the child's exact box.
+// Child words: wide-node index, kLeafFlag | first triangle, or kWideEmpty.
+// Hit children are entered nearest first (entry distance, then slot), the
+// others pushed farthest first.  The triangle tests and the slab arithmetic
+// are the binary traversal's, on boxes that contain the binary ones, so the
+// closest hit is the same triangle up to exact-distance ties.
+
+constexpr uint32_t kWideEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float wide_plane(uint32_t bytes, int i, float s, float p) {
+    return fmaf(static_cast<float>((bytes >> (8 * i)) & 0xFFu), s, p);
+}
+
+// Slab-tests the four children of wide node nd against (r, tmax); returns how
+// many were hit and leaves their words in w[0..n) nearest first.
+__device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, float tmax, uint32_t (&w)[4]) {
+    const float4 a = nd[0], b = nd[1], c = nd[2], cw = nd[3];
+    const uint32_t qlx = __float_as_uint(b.z), qly = __float_as_uint(b.w), qlz = __float_as_uint(c.x);
+    const uint32_t qhx = __float_as_uint(c.y), qhy = __float_as_uint(c.z), qhz = __float_as_uint(c.w);
+    w[0] = __float_as_uint(cw.x);
+    w[1] = __float_as_uint(cw.y);
+    w[2] = __float_as_uint(cw.z);
+    w[3] = __float_as_uint(cw.w);
+    uint32_t k[4];  // sort key: entry distance bits (t >= 0), or ~0 for a miss
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float4 lo = make_float4(wide_plane(qlx, i, a.w, a.x), wide_plane(qly, i, b.x, a.y),
+                                      wide_plane(qlz, i, b.y, a.z), 0.0f);
+        const float4 hi = make_float4(wide_plane(qhx, i, a.w, a.x), wide_plane(qhy, i, b.x, a.y),
+                                      wide_plane(qhz, i, b.y, a.z), 0.0f);
+        const float t = slab(r, lo, hi, tmax);
+        k[i] = (w[i] != kWideEmpty && t >= 0.0f) ? (__float_as_uint(t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+    }
+    // sorting network on (key, word); equal keys keep the lower slot first
+    auto cswap = [&](int i, int j) {
+        const bool sw = k[j] < k[i];
+        const uint32_t ki = k[i], wi = w[i];
+        k[i] = sw ? k[j] : ki;
+        w[i] = sw ? w[j] : wi;
+        k[j] = sw ? ki : k[j];
+        w[j] = sw ? wi : w[j];
+    };
+    cswap(0, 1);
+    cswap(2, 3);
+    cswap(0, 2);
+    cswap(1, 3);
+    cswap(1, 2);
+    return (k[0] != 0xFFFFFFFFu) + (k[1] != 0xFFFFFFFFu) + (k[2] != 0xFFFFFFFFu) + (k[3] != 0xFFFFFFFFu);
+}
+
+// Pushes w[1..n) farthest first; returns w[0].
+template <class Stack>
+__device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], int n, const Stack& sk, int& top,
+                                                 uint32_t& overflow) {
+#pragma unroll
+    for (int i = 3; i >= 1; --i) {
+        if (n > i) {
+            if (top < Stack::kTotal) sk.put(top++, w[i]);
+            else overflow++;
+        }
+    }
+    return w[0];
+}
+
+// Closest hit over the wide BVH (node 0 is the root).
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+                                               TraceCounters& cnt, uint32_t& overflow) {
+    Hit h;
+    h.t = kInf;
+    h.u = h.v = 0.0f;
+    h.tri = 0xFFFFFFFFu;
+    if (s.num_wnodes == 0) return h;
+    RayInv r = make_ray_inv(o, d);
+    uint32_t cur = 0;
+    int top = 0;
+    while (true) {
+        if (cur & kLeafFlag) {
+            uint32_t i = cur & ~kLeafFlag;
+            while (true) {
+                float u, v;
+                uint32_t word;
+                float t = tri_test(o, d, s.tris + 3 * i, u, v, word);
+                if (COUNT) cnt.tris += 1;
+                if (t >= 1e-4f && t < h.t) {
+                    h.t = t;
+                    h.u = u;
+                    h.v = v;
+                    h.tri = i;
+                }
+                if (word & 0x80000000u) break;
+                ++i;
+            }
+        } else {
+            uint32_t w[4];
+            if (COUNT) cnt.nodes += 2;  // 64 B: two 32 B node records
+            const int n = wide_children(s.wnodes + 4 * cur, r, h.t, w);
+            if (n > 0) {
+                cur = wide_descend(w, n, sk, top, overflow);
+                continue;
+            }
+        }
+        if (top == 0) break;
+        cur = sk.get(--top);
+    }
+    return h;
+}
+
+// Any hit in (1e-4, tmax) over the wide BVH.
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float tmax,
+                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            uint32_t& overflow) {
+    if (s.num_wnodes == 0) return false;
+    RayInv r = make_ray_inv(o, d);
+    uint32_t cur = 0;
+    int top = 0;
+    while (true) {
+        if (cur & kLeafFlag) {
+            uint32_t i = cur & ~kLeafFlag;
+            while (true) {
+                float u, v;
+                uint32_t word;
+                float t = tri_test(o, d, s.tris + 3 * i, u, v, word);
+                if (COUNT) cnt.tris += 1;
+                if (t > 1e-4f && t < tmax) return true;
+                if (word & 0x80000000u) break;
+                ++i;
+            }
+        } else {
+            uint32_t w[4];
+            if (COUNT) cnt.nodes += 2;
+            const int n = wide_children(s.wnodes + 4 * cur, r, tmax, w);
+            if (n > 0) {
+                cur = wide_descend(w, n, sk, top, overflow);
+                continue;
+            }
+        }
+        if (top == 0) break;
+        cur = sk.get(--top);
+    }
+    return false;
+}
+
+// Either BVH: the wide one when WIDE.
+template <bool WIDE, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ Hit trace_closest_v(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+                                               TraceCounters& cnt, uint32_t& overflow) {
+    if constexpr (WIDE) return trace_closest_w<COUNT>(s, o, d, sk, cnt, overflow);
+    else return trace_closest_s<COUNT>(s, o, d, sk, cnt, overflow);
+}
+template <bool WIDE, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_v(const DevScene& s, F3 o, F3 d, float tmax,
+                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            uint32_t& overflow) {
+    if constexpr (WIDE) return trace_any_w<COUNT>(s, o, d, tmax, sk, cnt, overflow);
+    else return trace_any_s<COUNT>(s, o, d, tmax, sk, cnt, overflow);
 }
 
 // The all-LDS forms (stk: this thread's first LDS slot, kStackSize entries).

@@ -13,8 +13,9 @@ hipError_t launch_probe_bsdf(const pt_material* d_mats, int n, const float* d_di
                              const float* d_u, float* d_out, hipStream_t stream);
 hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, int n, int dims, uint32_t* d_u,
                                 float* d_f, hipStream_t stream);
-hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, float* d_t, int32_t* d_face,
-                             float* d_bary, hipStream_t stream);
+// wide: traverse the 4-wide BVH (s.wnodes) instead of the binary one.
+hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, bool wide, float* d_t,
+                             int32_t* d_face, float* d_bary, hipStream_t stream);
 // Prepares the triangle records of the traversal layout (dev_scene.h) and the
 // BVH-ordered faces from the original faces and the build's slot order.
 hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, const uint32_t* d_prim_order,

@@ -120,10 +120,13 @@ __global__ void k_probe_sampler(int type, uint32_t seed, const int32_t* __restri
     }
 }
 
-__global__ __launch_bounds__(256) void k_trace_rays(DevScene s, const float* __restrict__ rays8, int n, int any,
-                                                    float* __restrict__ t_out, int32_t* __restrict__ face_out,
-                                                    float* __restrict__ bary_out) {
-    __shared__ uint32_t stack[kStackSize * 256];
+// One ray per thread; WIDE traverses the 4-wide BVH (its whole stack in LDS).
+template <bool WIDE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_rays(DevScene s, const float* __restrict__ rays8, int n, int any,
+                                                      float* __restrict__ t_out, int32_t* __restrict__ face_out,
+                                                      float* __restrict__ bary_out) {
+    constexpr int kTot = WIDE ? kWideStack : kStackSize;
+    __shared__ uint32_t stack[kTot * BLOCK];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays8 + 8 * static_cast<size_t>(i);
@@ -131,13 +134,14 @@ __global__ __launch_bounds__(256) void k_trace_rays(DevScene s, const float* __r
     F3 d = normalize_rn(f3(r[3], r[4], r[5]));  // Ray's constructor (core/ray.h:13)
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
+    const TraceStack<kTot, kTot> sk{stack + threadIdx.x, BLOCK, nullptr, 0};
     if (any) {
-        bool occ = trace_any<false>(s, o, d, r[6], stack + threadIdx.x, 256, cnt, ovf);
+        bool occ = trace_any_v<WIDE, false>(s, o, d, r[6], sk, cnt, ovf);
         t_out[i] = occ ? 1.0f : 0.0f;
         face_out[i] = -1;
         return;
     }
-    Hit h = trace_closest<false>(s, o, d, stack + threadIdx.x, 256, cnt, ovf);
+    Hit h = trace_closest_v<WIDE, false>(s, o, d, sk, cnt, ovf);
     t_out[i] = h.t;
     if (h.tri == 0xFFFFFFFFu) {
         face_out[i] = -1;
@@ -195,10 +199,14 @@ hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, i
     return hipGetLastError();
 }
 
-hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, float* d_t, int32_t* d_face,
-                             float* d_bary, hipStream_t stream) {
-    hipLaunchKernelGGL(k_trace_rays, dim3((n + 255) / 256), dim3(256), 0, stream, s, d_rays8, n, any, d_t, d_face,
-                       d_bary);
+hipError_t launch_trace_rays(const DevScene& s, const float* d_rays8, int n, int any, bool wide, float* d_t,
+                             int32_t* d_face, float* d_bary, hipStream_t stream) {
+    if (wide)
+        hipLaunchKernelGGL((k_trace_rays<true, 128>), dim3((n + 127) / 128), dim3(128), 0, stream, s, d_rays8, n, any,
+                           d_t, d_face, d_bary);
+    else
+        hipLaunchKernelGGL((k_trace_rays<false, 256>), dim3((n + 255) / 256), dim3(256), 0, stream, s, d_rays8, n, any,
+                           d_t, d_face, d_bary);
     return hipGetLastError();
 }
 

@@ -28,6 +28,9 @@ struct WavefrontConfig {
     int grab_extend = 128;
     int grab_shade = 1024;
     int grab_shadow = 64;
+    // 4: traverse the 4-wide BVH with quantized child boxes when every job's
+    // scene has one; 2: the binary BVH (the reference's traversal order)
+    int bvh_width = 4;
 };
 
 // Stages timed when WavefrontConfig::time_stages is set.

@@ -336,15 +336,22 @@ __global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
     for (uint32_t k = C_CUR_EXT + t; k < C_WORDS; k += blockDim.x) ctrl[k] = 0;
 }
 
-template <int LDS, int BLOCK>
-__device__ __forceinline__ TraceStack<LDS> thread_stack(const WfState& w, uint32_t* lds) {
-    return TraceStack<LDS>{lds + threadIdx.x, BLOCK, w.spill + blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK};
+// Traversal stack entries of the binary or the 4-wide BVH.
+template <bool WIDE>
+constexpr int stack_total() {
+    return WIDE ? kWideStack : kStackSize;
 }
 
-template <bool COUNT, int BLOCK, int LDS>
+template <int LDS, int BLOCK, int TOTAL = kStackSize>
+__device__ __forceinline__ TraceStack<LDS, TOTAL> thread_stack(const WfState& w, uint32_t* lds) {
+    return TraceStack<LDS, TOTAL>{lds + threadIdx.x, BLOCK, w.spill + blockIdx.x * BLOCK + threadIdx.x,
+                                  gridDim.x * BLOCK};
+}
+
+template <bool COUNT, int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
-    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
+    const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
@@ -359,7 +366,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
             uint32_t p = q[i];
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             const float4 o = w.path[p].o, d = w.path[p].d;
-            Hit h = trace_closest_s<COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
+            Hit h = trace_closest_v<WIDE, COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
             w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
             // the sort's key, stored in queue order so the sort reads it contiguously
             if (w.key_bins)
@@ -574,12 +581,12 @@ __device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, flo
 
 // Any-hit over a segmented shadow queue (area or environment): each path has
 // at most one entry, so the contribution is added without atomics.
-template <bool COUNT, int BLOCK, int LDS>
+template <bool COUNT, int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, int cur_slot,
                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                      const float4* __restrict__ qc, DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
-    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
+    const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + count_slot;
@@ -594,7 +601,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
             const uint32_t p = __float_as_uint(d.w);
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any_s<COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, qc[i], false);
+            if (!trace_any_v<WIDE, COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, qc[i], false);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -602,10 +609,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
 
 // Any-hit over the flat punctual-light queue (several entries per path:
 // contributions are added with float atomics).
-template <bool COUNT, int BLOCK, int LDS>
+template <bool COUNT, int BLOCK, int LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
-    const TraceStack<LDS> sk = thread_stack<LDS, BLOCK>(w, stack);
+    const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
     QueueWalk qw(w.ctrl, C_CUR_SP, nullptr, n, w.grab_shadow);
     uint32_t i, end;
@@ -618,7 +625,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCoun
             const uint32_t p = __float_as_uint(d.w);
             const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any_s<COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, w.sp_c[i], true);
+            if (!trace_any_v<WIDE, COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, w.sp_c[i], true);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -1006,7 +1013,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
 
 // Any-hit over a segmented shadow queue of the replay: an unoccluded ray
 // adds its contribution and applies its sensitivities.
-template <bool COUNT, int BLOCK, bool ENV>
+template <bool COUNT, int BLOCK, bool ENV, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot, int cur_slot,
                                                       const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                       const float4* __restrict__ qc, const float4* __restrict__ qx,
@@ -1014,7 +1021,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot,
     __shared__ uint32_t stack[kStackSize * BLOCK];
     __shared__ float macc[kWfLdsMaterials * kDP];
     clear_materials(macc);
-    const TraceStack<kStackSize> sk = thread_stack<kStackSize, BLOCK>(w, stack);
+    const auto sk = thread_stack<kStackSize, BLOCK, stack_total<WIDE>()>(w, stack);
     const AdjointGrads G = *w.grads;
     const DevScene& s0 = w.jobs[0].scene;
     unsigned long long c0;
@@ -1035,7 +1042,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot,
             float4 o = qo[i], d = qd[i];
             const uint32_t p = __float_as_uint(d.w);
             rays++;
-            if (!trace_any_s<COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
+            if (!trace_any_v<WIDE, COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
                 add_unoccluded(w, p, qc[i], false);
                 const float4 x0 = qx[kPayload * static_cast<size_t>(i)];
                 const float4 x1 = qx[kPayload * static_cast<size_t>(i) + 1];
@@ -1059,12 +1066,12 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot,
     flush_materials(macc, G, s0.num_materials);
 }
 
-template <bool COUNT, int BLOCK>
+template <bool COUNT, int BLOCK, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_wfa_shadow_punctual(WfState w, DevCounters* ctr) {
     __shared__ uint32_t stack[kStackSize * BLOCK];
     __shared__ float macc[kWfLdsMaterials * kDP];
     clear_materials(macc);
-    const TraceStack<kStackSize> sk = thread_stack<kStackSize, BLOCK>(w, stack);
+    const auto sk = thread_stack<kStackSize, BLOCK, stack_total<WIDE>()>(w, stack);
     const AdjointGrads G = *w.grads;
     const DevScene& s0 = w.jobs[0].scene;
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
@@ -1082,7 +1089,7 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow_punctual(WfState w, DevCou
             float4 o = w.sp_o[i], d = w.sp_d[i];
             const uint32_t p = __float_as_uint(d.w);
             rays++;
-            if (!trace_any_s<COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
+            if (!trace_any_v<WIDE, COUNT>(s0, f3(o), f3(d), o.w, sk, cnt, ovf)) {
                 add_unoccluded(w, p, w.sp_c[i], true);
                 const float4 x0 = w.sp_x[kPayload * static_cast<size_t>(i)];
                 const float4 x1 = w.sp_x[kPayload * static_cast<size_t>(i) + 1];
@@ -1336,6 +1343,7 @@ struct ChunkShape {
     bool sort, lights;
     bool area, env;  // some job samples emissive triangles / the environment (else that shadow queue stays empty)
     bool adjoint;    // primal pass, then the replay of the path-replay adjoint
+    bool wide;       // traverse the 4-wide BVH (every job's scene has one)
 };
 
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
@@ -1345,35 +1353,64 @@ struct Grids {
     int ashade, ashadow;  // the adjoint's replay kernels
 };
 
-// The traversal kernels of one stack variant (LDS = stack entries in LDS).
-template <bool COUNT, int EB, int LDS>
+// The traversal kernels of one stack variant (LDS = stack entries in LDS)
+// over the binary or the 4-wide BVH.
+template <bool COUNT, int EB, int LDS, bool WIDE>
 void launch_traversal_extend(const Grids& g, hipStream_t stream, const WfState& w, int in, DevCounters* ctr) {
-    hipLaunchKernelGGL((k_wf_extend<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
+    hipLaunchKernelGGL((k_wf_extend<COUNT, EB, LDS, WIDE>), dim3(g.extend), dim3(EB), 0, stream, w, in, ctr);
 }
-template <bool COUNT, int EB, int LDS>
+template <bool COUNT, int EB, int LDS, bool WIDE>
 void launch_traversal_shadow(const Grids& g, hipStream_t stream, const WfState& w, const ChunkShape& cs,
                              DevCounters* ctr) {
     if (cs.area)
-        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT, C_CUR_SA,
-                           w.sa_o, w.sa_d, w.sa_c, ctr);
+        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS, WIDE>), dim3(g.extend), dim3(EB), 0, stream, w, C_SACNT,
+                           C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, ctr);
     if (cs.env)
-        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT, C_CUR_SE,
-                           w.se_o, w.se_d, w.se_c, ctr);
+        hipLaunchKernelGGL((k_wf_shadow<COUNT, EB, LDS, WIDE>), dim3(g.extend), dim3(EB), 0, stream, w, C_SECNT,
+                           C_CUR_SE, w.se_o, w.se_d, w.se_c, ctr);
     if (cs.lights)
-        hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB, LDS>), dim3(g.extend), dim3(EB), 0, stream, w, ctr);
+        hipLaunchKernelGGL((k_wf_shadow_punctual<COUNT, EB, LDS, WIDE>), dim3(g.extend), dim3(EB), 0, stream, w,
+                           ctr);
 }
 
-template <bool COUNT, int EB>
+template <bool COUNT, int EB, bool WIDE>
 void launch_replay_shadow(const Grids& g, hipStream_t stream, const WfState& w, const ChunkShape& cs,
                           DevCounters* ctr) {
     if (cs.area)
-        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, false>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SACNT,
+        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, false, WIDE>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SACNT,
                            C_CUR_SA, w.sa_o, w.sa_d, w.sa_c, w.sa_x, ctr);
     if (cs.env)
-        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, true>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SECNT,
+        hipLaunchKernelGGL((k_wfa_shadow<COUNT, EB, true, WIDE>), dim3(g.ashadow), dim3(EB), 0, stream, w, C_SECNT,
                            C_CUR_SE, w.se_o, w.se_d, w.se_c, w.se_x, ctr);
     if (cs.lights)
-        hipLaunchKernelGGL((k_wfa_shadow_punctual<COUNT, EB>), dim3(g.ashadow), dim3(EB), 0, stream, w, ctr);
+        hipLaunchKernelGGL((k_wfa_shadow_punctual<COUNT, EB, WIDE>), dim3(g.ashadow), dim3(EB), 0, stream, w, ctr);
+}
+
+template <bool COUNT, int EB>
+void launch_extend(const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g, hipStream_t stream,
+                   const WfState& w, int in, DevCounters* ctr) {
+    if (cs.wide) {
+        if (cfg.stack_lds == 16) launch_traversal_extend<COUNT, EB, 16, true>(g, stream, w, in, ctr);
+        else launch_traversal_extend<COUNT, EB, kStackSize, true>(g, stream, w, in, ctr);
+    } else {
+        if (cfg.stack_lds == 16) launch_traversal_extend<COUNT, EB, 16, false>(g, stream, w, in, ctr);
+        else launch_traversal_extend<COUNT, EB, kStackSize, false>(g, stream, w, in, ctr);
+    }
+}
+
+template <bool COUNT, int EB>
+void launch_shadow(const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g, hipStream_t stream,
+                   const WfState& w, bool replay, DevCounters* ctr) {
+    if (replay) {
+        if (cs.wide) launch_replay_shadow<COUNT, EB, true>(g, stream, w, cs, ctr);
+        else launch_replay_shadow<COUNT, EB, false>(g, stream, w, cs, ctr);
+    } else if (cs.wide) {
+        if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16, true>(g, stream, w, cs, ctr);
+        else launch_traversal_shadow<COUNT, EB, kStackSize, true>(g, stream, w, cs, ctr);
+    } else {
+        if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16, false>(g, stream, w, cs, ctr);
+        else launch_traversal_shadow<COUNT, EB, kStackSize, false>(g, stream, w, cs, ctr);
+    }
 }
 
 // Camera rays and max_depth bounces of the chunk's paths; `replay` runs the
@@ -1399,8 +1436,7 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_EXTEND);
-            if (cfg.stack_lds == 16) launch_traversal_extend<COUNT, EB, 16>(g, stream, w, in, ctr);
-            else launch_traversal_extend<COUNT, EB, kStackSize>(g, stream, w, in, ctr);
+            launch_extend<COUNT, EB>(cfg, cs, g, stream, w, in, ctr);
         }
         if (cs.sort) {
             Timed t(tm, WF_SORT);
@@ -1424,9 +1460,7 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         {
             Timed t(tm, WF_SHADOW);
-            if (replay) launch_replay_shadow<COUNT, EB>(g, stream, w, cs, ctr);
-            else if (cfg.stack_lds == 16) launch_traversal_shadow<COUNT, EB, 16>(g, stream, w, cs, ctr);
-            else launch_traversal_shadow<COUNT, EB, kStackSize>(g, stream, w, cs, ctr);
+            launch_shadow<COUNT, EB>(cfg, cs, g, stream, w, replay, ctr);
         }
     }
 }
@@ -1454,25 +1488,34 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
     }
 }
 
-template <bool COUNT, int EB, int SB>
-Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg) {
-    Grids g;
+template <bool COUNT, int EB, bool WIDE>
+int traversal_grid(const Wavefront& wf, const WavefrontConfig& cfg) {
     // extend and shadow kernels share the grid: take the smaller occupancy
-    const void* ext = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, 16>)
-                                          : reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, kStackSize>);
-    const void* shd = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, 16>)
-                                          : reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, kStackSize>);
-    g.extend = std::min(grid_for(ext, EB, wf.num_cus, cfg.waves_per_simd),
-                        grid_for(shd, EB, wf.num_cus, cfg.waves_per_simd));
+    const void* ext = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, 16, WIDE>)
+                                          : reinterpret_cast<const void*>(&k_wf_extend<COUNT, EB, kStackSize, WIDE>);
+    const void* shd = cfg.stack_lds == 16 ? reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, 16, WIDE>)
+                                          : reinterpret_cast<const void*>(&k_wf_shadow<COUNT, EB, kStackSize, WIDE>);
+    return std::min(grid_for(ext, EB, wf.num_cus, cfg.waves_per_simd),
+                    grid_for(shd, EB, wf.num_cus, cfg.waves_per_simd));
+}
+
+template <bool COUNT, int EB, int SB>
+Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg, bool wide) {
+    Grids g;
+    g.extend = wide ? traversal_grid<COUNT, EB, true>(wf, cfg) : traversal_grid<COUNT, EB, false>(wf, cfg);
     const void* shade = cfg.shade_waves == 4   ? reinterpret_cast<const void*>(&k_wf_shade<SB, 4>)
                         : cfg.shade_waves == 3 ? reinterpret_cast<const void*>(&k_wf_shade<SB, 3>)
                                                : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
     g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
     g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
-    g.ashadow = std::min(
-        grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true>), EB, wf.num_cus, cfg.waves_per_simd),
-        grid_for(reinterpret_cast<const void*>(&k_wfa_shadow_punctual<COUNT, EB>), EB, wf.num_cus,
-                 cfg.waves_per_simd));
+    g.ashadow = wide ? std::min(grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true, true>), EB,
+                                         wf.num_cus, cfg.waves_per_simd),
+                                grid_for(reinterpret_cast<const void*>(&k_wfa_shadow_punctual<COUNT, EB, true>), EB,
+                                         wf.num_cus, cfg.waves_per_simd))
+                     : std::min(grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true, false>), EB,
+                                         wf.num_cus, cfg.waves_per_simd),
+                                grid_for(reinterpret_cast<const void*>(&k_wfa_shadow_punctual<COUNT, EB, false>), EB,
+                                         wf.num_cus, cfg.waves_per_simd));
     return g;
 }
 
@@ -1532,10 +1575,11 @@ template <bool COUNT>
 struct GridQuery {
     const Wavefront& wf;
     const WavefrontConfig& cfg;
+    bool wide;
     Grids out;
     template <int EB, int SB>
     void operator()() {
-        out = grids_for<COUNT, EB, SB>(wf, cfg);
+        out = grids_for<COUNT, EB, SB>(wf, cfg, wide);
     }
 };
 
@@ -1569,7 +1613,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     unsigned long long paths = 0, pixels = 0;
     uint32_t max_lights = 0, max_mat = 0;
     int max_depth = 0;
-    bool any_area = false, any_env = false;
+    bool any_area = false, any_env = false, all_wide = true;
     for (size_t j = 0; j < jobs_in.size(); ++j) {
         const WavefrontJob& in = jobs_in[j];
         JobDesc& d = jobs[j];
@@ -1601,6 +1645,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
                               in.scene.env_w > 0 && in.scene.env_h > 0);
         max_mat = std::max(max_mat, in.scene.num_materials);
         max_depth = std::max(max_depth, in.params.max_depth);
+        all_wide = all_wide && in.scene.wnodes != nullptr && in.scene.num_wnodes > 0;
     }
     if (pixels > 0xFFFFFFFFull) {
         if (err) *err = "wavefront batch has more than 2^32 pixels";
@@ -1639,6 +1684,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.area = any_area;
     cs.env = any_env;
     cs.adjoint = G != nullptr;
+    cs.wide = cfg.bvh_width == 4 && all_wide;
     wf->st.key_bins = cs.sort ? cs.nbins : 0u;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
@@ -1657,7 +1703,8 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
-                        static_cast<uint64_t>(cs.lights + 2 * cs.area + 4 * cs.env), static_cast<uint64_t>(cs.nbins),
+                        static_cast<uint64_t>(cs.lights + 2 * cs.area + 4 * cs.env + 8 * cs.wide),
+                        static_cast<uint64_t>(cs.nbins),
                         static_cast<uint64_t>(cfg.stack_lds + (cs.adjoint ? 1000 : 0)) |
                             (static_cast<uint64_t>(cfg.grab_extend) << 16) | (static_cast<uint64_t>(cfg.grab_shade) << 32) |
                             (static_cast<uint64_t>(cfg.grab_shadow) << 48)};
@@ -1666,17 +1713,21 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     StageTimer* tm = cfg.time_stages ? &timer : nullptr;
     Grids g;
     if (count) {
-        GridQuery<true> q{*wf, cfg, {}};
+        GridQuery<true> q{*wf, cfg, cs.wide, {}};
         with_blocks(cfg, q);
         g = q.out;
     } else {
-        GridQuery<false> q{*wf, cfg, {}};
+        GridQuery<false> q{*wf, cfg, cs.wide, {}};
         with_blocks(cfg, q);
         g = q.out;
     }
     // traversal stack entries kept in global memory: one column per thread of the traversal grid
-    if (cfg.stack_lds < kStackSize &&
-        !wf->ensure_spill(static_cast<size_t>(kStackSize - cfg.stack_lds) * g.extend * cfg.extend_block)) {
+    // (the replay's shadow kernels keep kStackSize entries in LDS)
+    const int total = cs.wide ? kWideStack : kStackSize;
+    const size_t spill = std::max(static_cast<size_t>(total - cfg.stack_lds) * g.extend * cfg.extend_block,
+                                  cs.adjoint ? static_cast<size_t>(total - kStackSize) * g.ashadow * cfg.extend_block
+                                             : size_t(0));
+    if (spill > 0 && !wf->ensure_spill(spill)) {
         if (err) *err = "wavefront: out of device memory for the traversal stacks";
         return hipErrorOutOfMemory;
     }

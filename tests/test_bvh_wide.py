@@ -1,0 +1,187 @@
+"""The 4-wide BVH (pt_bvh_collapse_wide), host only.
+
+It is collapsed from the reference-identical binary BVH (the oracle's build,
+pinned node for node against the reference's) and must keep every property
+the traversal relies on:
+  * every binary leaf appears exactly once as a leaf child, so every triangle
+    is reachable exactly once;
+  * each child's decoded box fma(q, S, P) contains every triangle under it
+    (the ray-box test on it never rejects what the exact box accepts);
+  * the boxes are tight to one quantization step, S minimal;
+  * the stack bound equals the one recomputed here and fits the traversal
+    stack;
+  * for every golden ray that hits, every box on the path from the root to
+    the leaf of its closest triangle passes the reference's slab test
+    (bvh_traversal.cuh:29-41, float32) with tmax 1e-6 (relative) past that
+    hit, so the wide traversal cannot cull it (its current best is never
+    below the closest hit; a best within that margin is an exact-distance tie
+    up to rounding: a hit on a flat box's plane enters at t to the last few
+    bits)."""
+import numpy as np
+import pytest
+
+import oracle
+import pathtracingtutorial_amd as pt
+from conftest import golden_npz
+from test_oracle_golden import _scene_from_case, case_arrays
+
+LEAF = 0x80000000
+EMPTY = 0xFFFFFFFF
+CASES = ["row1", "row5", "row100", "soup200", "soup3000", "cornell"]
+
+
+def _decode(node):
+    u = node.view(np.uint32)
+    P = node[0:3]
+    S = np.array([node[3], node[4], node[5]], np.float32)
+    ql = np.array([u[6], u[7], u[8]], np.uint32)
+    qh = np.array([u[9], u[10], u[11]], np.uint32)
+    words = u[12:16]
+    lo = np.zeros((4, 3), np.float32)
+    hi = np.zeros((4, 3), np.float32)
+    for i in range(4):
+        qi_lo = ((ql >> np.uint32(8 * i)) & np.uint32(0xFF)).astype(np.float32)
+        qi_hi = ((qh >> np.uint32(8 * i)) & np.uint32(0xFF)).astype(np.float32)
+        lo[i] = P + qi_lo * S  # q * S is exact: one rounding, as fma
+        hi[i] = P + qi_hi * S
+    return lo, hi, words, S
+
+
+def _tri_bounds(sc_nodes, order, positions, faces):
+    f = faces[order]
+    tri = positions[f[:, :3]]  # (n, 3, 3)
+    return tri.min(axis=1), tri.max(axis=1)
+
+
+def _case(golden, name):
+    sc, e = _scene_from_case(golden, name)
+    nodes, order = sc.bvh()
+    vb, fb, _ = case_arrays(golden, name)
+    v = np.frombuffer(vb, np.float32).reshape(-1, 16)
+    f = np.frombuffer(fb, np.uint32).reshape(-1, 4)
+    return nodes, order, v[:, :3].astype(np.float32), f
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_wide_bvh_invariants(golden, name):
+    nodes, order, pos, faces = _case(golden, name)
+    wide, max_stack = pt.collapse_wide(nodes)
+    u8 = nodes.view(np.uint32)
+    leaves = sorted((int(u8[k, 3] & 0x7FFFFFFF), int(u8[k, 7])) for k in range(nodes.shape[0]) if u8[k, 3] & LEAF)
+    leaf_count = dict(leaves)
+    tlo, thi = _tri_bounds(nodes, order, pos, faces)
+    seen = []
+    parent = {0: None}
+
+    def visit(w, pushed):
+        """Returns the exact union box of the triangles under wide node w; checks its children."""
+        lo, hi, words, S = _decode(wide[w])
+        n = int(np.sum(words != EMPTY))
+        assert 1 <= n <= 4 and np.all(words[:n] != EMPTY) and np.all(words[n:] == EMPTY)
+        need = pushed + n - 1
+        deepest = need
+        box_lo, box_hi = np.full(3, np.inf, np.float32), np.full(3, -np.inf, np.float32)
+        for i in range(n):
+            word = int(words[i])
+            if word & LEAF:
+                first = word & 0x7FFFFFFF
+                cnt = leaf_count[first]
+                seen.append((first, cnt))
+                clo, chi = tlo[first:first + cnt].min(axis=0), thi[first:first + cnt].max(axis=0)
+            else:
+                assert word not in parent and 0 < word < wide.shape[0]
+                parent[word] = w
+                clo, chi, sub = visit(word, need)
+                deepest = max(deepest, sub)
+            # conservative: the decoded box holds the child's triangles
+            assert np.all(lo[i] <= clo) and np.all(hi[i] >= chi), (w, i)
+            # tight to one quantization step (flat axes decode exactly)
+            assert np.all(lo[i] >= clo - S * 1.0001 - np.abs(clo) * 1e-6), (w, i)
+            assert np.all(hi[i] <= chi + S * 1.0001 + np.abs(chi) * 1e-6), (w, i)
+            box_lo, box_hi = np.minimum(box_lo, clo), np.maximum(box_hi, chi)
+        # S minimal: half of it could not span the node
+        ext = (box_hi - box_lo).astype(np.float64)
+        assert np.all((ext == 0) | (255 * S.astype(np.float64) / 2 < ext * (1 + 1e-6) + 1e-30))
+        return box_lo, box_hi, deepest
+
+    _, _, deepest = visit(0, 0)
+    assert max_stack == deepest and max_stack <= 64
+    assert len(parent) == wide.shape[0]  # every wide node reached once
+    assert sorted(seen) == leaves     # every binary leaf exactly once
+
+
+def _slab(o, inv, lo, hi, tmax):
+    t_min, t_max = np.float32(0.0), np.float32(tmax)
+    for a in range(3):
+        t1 = np.float32((lo[a] - o[a]) * inv[a])
+        t2 = np.float32((hi[a] - o[a]) * inv[a])
+        t_min = np.fmin(np.fmax(t1, t_min), np.fmax(t2, t_min))
+        t_max = np.fmax(np.fmin(t1, t_max), np.fmin(t2, t_max))
+    return t_min <= t_max
+
+
+@pytest.mark.parametrize("name", ["row100", "soup200", "soup3000", "cornell"])
+def test_wide_bvh_path_to_closest_hit_passes(golden, name):
+    nodes, order, _, _ = _case(golden, name)
+    wide, _ = pt.collapse_wide(nodes)
+    slot_of = np.argsort(order)  # original face -> BVH slot
+    # leaf word of each slot, and the path of wide nodes/children to it
+    path_to = {}
+
+    def walk(w, path):
+        _, _, words, _ = _decode(wide[w])
+        for i, word in enumerate(words):
+            word = int(word)
+            if word == EMPTY:
+                continue
+            if word & LEAF:
+                path_to[word & 0x7FFFFFFF] = path + [(w, i)]
+            else:
+                walk(word, path + [(w, i)])
+
+    walk(0, [])
+    starts = np.array(sorted(path_to))
+    tr = golden_npz(f"trace_{name}.npz")
+    rays, t_ref, f_ref = tr["rays"], tr["t"], tr["face"]
+    checked = 0
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        for r in np.nonzero(f_ref >= 0)[0][:400]:
+            o = rays[r, :3].astype(np.float32)
+            d = rays[r, 3:6].astype(np.float32)
+            d = (d / np.float32(np.sqrt(np.float32(d @ d)))).astype(np.float32)
+            inv = (np.float32(1.0) / d).astype(np.float32)
+            slot = slot_of[f_ref[r]]
+            first = int(starts[np.searchsorted(starts, slot, side="right") - 1])
+            for w, i in path_to[first]:
+                lo, hi, _, _ = _decode(wide[w])
+                assert _slab(o, inv, lo[i], hi[i], np.float32(t_ref[r] * (1 + 1e-6))), (r, w, i)
+            checked += 1
+    assert checked > 0
+
+
+def test_wide_bvh_single_leaf_root_and_empty():
+    # a one-triangle tree: the root is a leaf, the wide root has it as its only child
+    nodes = np.zeros((1, 8), np.float32)
+    nodes[0, 0:3] = [0, 0, 0]
+    nodes[0, 4:7] = [1, 1, 0]
+    nodes.view(np.uint32)[0, 3] = LEAF | 0
+    nodes.view(np.uint32)[0, 7] = 1
+    wide, ms = pt.collapse_wide(nodes)
+    assert wide.shape == (1, 16) and ms == 0
+    lo, hi, words, _ = _decode(wide[0])
+    assert int(words[0]) == LEAF and np.all(words[1:] == EMPTY)
+    assert np.all(lo[0] <= [0, 0, 0]) and np.all(hi[0] >= [1, 1, 0])
+    wide, ms = pt.collapse_wide(np.zeros((0, 8), np.float32))
+    assert wide.shape == (0, 16) and ms == 0
+
+
+def test_wide_bvh_rejects_a_malformed_tree():
+    nodes = np.zeros((3, 8), np.float32)
+    u = nodes.view(np.uint32)
+    u[0, 3], u[0, 7] = 1, 7  # right child out of range
+    with pytest.raises(pt.PtError):
+        pt.collapse_wide(nodes)
+    u[0, 7] = 1  # both children the same node: not a tree
+    u[1, 3], u[1, 7] = LEAF, 1
+    with pytest.raises(pt.PtError):
+        pt.collapse_wide(nodes)

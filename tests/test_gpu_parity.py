@@ -168,6 +168,44 @@ def test_gpu_closest_and_any_hit_vs_reference(golden, name):
     assert np.array_equal(occ, tr["occluded"])
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_wide_bvh_closest_and_any_hit_vs_reference(golden, name):
+    # the 4-wide BVH: the same closest triangles up to exact-distance ties, the same occlusion
+    sm = _case_scene(golden, name)
+    sm.buildAccel(24, 4, 0)
+    tr = golden_npz(f"trace_{name}.npz")
+    t, face, _ = pt.trace_rays(sm, tr["rays"], wide=True)
+    _check_hits(t, face, tr["t"], tr["face"])
+    occ, _, _ = pt.trace_rays(sm, tr["rays_any"], any_hit=True, wide=True)
+    assert np.array_equal(occ, tr["occluded"])
+
+
+def test_gpu_wide_bvh_sphere2_and_random_rays_vs_binary():
+    m = golden_npz("trace_sphere2.npz")
+    pos, faces = m["positions"], m["faces"]
+    verts = np.zeros((pos.shape[0], 16), np.float32)
+    verts[:, 0:3] = pos
+    f = np.zeros((faces.shape[0], 4), np.uint32)
+    f[:, :3] = faces
+    sm = scene_from_bytes(verts.tobytes(), f.tobytes(), pos.shape[0], faces.shape[0])
+    sm.buildAccel(24, 4, 0)
+    t, face, _ = pt.trace_rays(sm, m["rays"], wide=True)
+    _check_hits(t, face, m["t"], m["face"])
+    # random rays from inside and outside: wide and binary agree (ties aside), any-hit exactly
+    rng = np.random.default_rng(5)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, :3] = rng.uniform(-2.5, 2.5, (20000, 3))
+    d = rng.normal(size=(20000, 3))
+    rays[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 6] = rng.uniform(0.05, 4.0, 20000)
+    tb, fb, _ = pt.trace_rays(sm, rays)
+    tw, fw, _ = pt.trace_rays(sm, rays, wide=True)
+    _check_hits(tw, fw, tb, fb)
+    ob, _, _ = pt.trace_rays(sm, rays, any_hit=True)
+    ow, _, _ = pt.trace_rays(sm, rays, any_hit=True, wide=True)
+    assert np.array_equal(ob, ow)
+
+
 def test_gpu_bvh_config_settings_invariants():
     # 32 bins, <= 4 triangles per leaf: every face in exactly one leaf, children inside parents
     sm = pt.SceneManager()

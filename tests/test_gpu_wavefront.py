@@ -149,6 +149,7 @@ def test_wavefront_pcg_cornell_configs_bit_identical():
 
 def test_wavefront_counters_match_megakernel():
     sm = _cornell(w=64, h=64, spf=2)
+    sm.setWavefrontConfig(bvh_width=2)  # the megakernel's tree
     sm.enableCounters(True)
     sm.resetCounters()
     _render(sm, pt.PathIntegrator(), 1)
@@ -161,6 +162,28 @@ def test_wavefront_counters_match_megakernel():
     for k in ("closest_rays", "shadow_rays", "node_visits", "tri_tests"):
         assert abs(a[k] - b[k]) <= 0.01 * a[k] + 10, (k, a[k], b[k])
     assert b["stack_overflows"] == 0
+
+
+@pytest.mark.parametrize("scene", ["cornell", "env"])
+def test_wavefront_wide_bvh_same_paths_fewer_node_bytes(scene):
+    # The 4-wide BVH gives the binary tree's closest hits (up to exact-distance
+    # ties), so the images agree pixel by pixel with the same random numbers,
+    # every ray is traced alike, and the node data fetched per ray drops.
+    sm = _cornell(w=64, h=64, spf=4) if scene == "cornell" else _env_scene(w=72, h=40, spf=8)
+    sm.enableCounters(True)
+    out = {}
+    for width in (2, 4):
+        sm.setWavefrontConfig(bvh_width=width)
+        sm.uploadToDevice()
+        sm.resetCounters()
+        out[width] = _render(sm, pt.WavefrontIntegrator(), 2)[0], sm.counters()
+    (a, ca), (b, cb) = out[2], out[4]
+    bad = _same_paths(b, a, tol_frac=0.002)
+    print(f"{scene}: {bad:.4%} of channels differ between the binary and the 4-wide BVH")
+    for k in ("closest_rays", "shadow_rays", "paths"):
+        assert abs(ca[k] - cb[k]) <= 0.002 * ca[k] + 2, (k, ca[k], cb[k])
+    assert cb["node_visits"] < 0.9 * ca["node_visits"], (ca["node_visits"], cb["node_visits"])
+    assert cb["stack_overflows"] == 0
 
 
 def test_wavefront_depth_limits():
@@ -233,9 +256,12 @@ def test_wavefront_bad_config_rejected():
         sm.setWavefrontConfig(stack_lds=8)
     with pytest.raises(pt.PtError):
         sm.setWavefrontConfig(grab_shade=96)
+    with pytest.raises(pt.PtError):
+        sm.setWavefrontConfig(bvh_width=8)
 
 
-def test_wavefront_split_stack_deep_bvh_bit_identical():
+@pytest.mark.parametrize("width", [2, 4])
+def test_wavefront_split_stack_deep_bvh_bit_identical(width):
     # One triangle per leaf over 2M triangles: a BVH deeper than the 16 LDS
     # stack entries, so traversal stacks spill into device memory.
     sm = pt.SceneManager()
@@ -250,7 +276,7 @@ def test_wavefront_split_stack_deep_bvh_bit_identical():
     sm.enableCounters(True)
     out = {}
     for lds in (32, 16):
-        sm.setWavefrontConfig(stack_lds=lds)
+        sm.setWavefrontConfig(stack_lds=lds, bvh_width=width)
         sm.uploadToDevice()
         sm.resetCounters()
         out[lds] = _render(sm, pt.WavefrontIntegrator(), 1)[0], sm.counters()

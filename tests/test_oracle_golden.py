@@ -79,6 +79,13 @@ def test_struct_layout_matches_reference(golden):
 
 # -------------------------------------------------------------------- BVH
 def _scene_from_case(golden, name):
+    vb, fb, e = case_arrays(golden, name)
+    mat = bytes.fromhex(golden["default_material_hex"])
+    return oracle.OracleScene(vb, e["nv"], fb, e["nf"], mat, 1), e
+
+
+def case_arrays(golden, name):
+    """Vertex and face bytes of a golden BVH case, and its golden entry."""
     if name.startswith("soup"):
         m = golden_npz(f"mesh_{name}.npz")
         vb, fb = m["vertices"].tobytes(), m["faces"].tobytes()
@@ -101,9 +108,7 @@ def _scene_from_case(golden, name):
                 verts[3 * i + j].normal = _capi.Vec3p.of((0, 0, 1))
             faces[i] = _capi.Face(3 * i, 3 * i + 1, 3 * i + 2, 0)
         vb, fb = bytes(verts), bytes(faces)
-    e = golden["bvh"][name]
-    mat = bytes.fromhex(golden["default_material_hex"])
-    return oracle.OracleScene(vb, e["nv"], fb, e["nf"], mat, 1), e
+    return vb, fb, golden["bvh"][name]
 
 
 @pytest.mark.parametrize("name", ["row1", "row5", "row100", "soup200", "soup3000", "cornell"])
