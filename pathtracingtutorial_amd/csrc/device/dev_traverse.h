@@ -37,6 +37,22 @@ __device__ __forceinline__ RayInv make_ray_inv(F3 o, F3 d) {
     return r;
 }
 
+// Scene arrays live in device memory; loading them through a global-address-
+// space pointer gives global loads (a generic pointer read from the job table
+// compiles to flat loads, which also count against the LDS counter that the
+// traversal stack's LDS accesses wait on).
+// (The load goes through a plain vector type: float4's copy constructor
+// takes a generic reference and would undo the address space.)
+typedef float V4F __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld(const float4* p) {
+    const V4F v = *(const __attribute__((address_space(1))) V4F*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ uint32_t gld_u32(const uint32_t* p) {
+    return *(const __attribute__((address_space(1))) uint32_t*)p;
+}
+
 // bvh_traversal.cuh:29-41 (t_min starts at 0, t_max at the current distance);
 // returns the entry distance, or a negative value on a miss.
 __device__ __forceinline__ float slab(const RayInv& r, float4 lo, float4 hi, float tmax) {
@@ -59,7 +75,7 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
     // no FMA contraction: every product and sum rounds as in the reference's
     // float arithmetic, so t matches it to the last bit
 #pragma clang fp contract(off)
-    float4 a = tri[0], b = tri[1], c = tri[2];
+    float4 a = gld(tri), b = gld(tri + 1), c = gld(tri + 2);
     word = __float_as_uint(a.w);
     F3 e1 = f3(b.x, b.y, b.z), e2 = f3(c.x, c.y, c.z);
     F3 pvec = cross_rn(d, e2);
@@ -86,13 +102,16 @@ struct TraceStack {
     int stride;
     uint32_t* spill;
     uint32_t spill_stride;
+    // (the device-memory part is addressed as global memory: a generic
+    // pointer would let the compiler merge both cases into one flat access)
     __device__ __forceinline__ void put(int level, uint32_t v) const {
         if (LDS >= TOTAL || level < LDS) stk[level * stride] = v;
-        else spill[static_cast<size_t>(level - LDS) * spill_stride] = v;
+        else *(__attribute__((address_space(1))) uint32_t*)(spill + static_cast<size_t>(level - LDS) * spill_stride) = v;
     }
     __device__ __forceinline__ uint32_t get(int level) const {
         if (LDS >= TOTAL || level < LDS) return stk[level * stride];
-        return spill[static_cast<size_t>(level - LDS) * spill_stride];
+        return *(const __attribute__((address_space(1))) uint32_t*)(spill +
+                                                                     static_cast<size_t>(level - LDS) * spill_stride);
     }
 };
 
@@ -106,7 +125,7 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
     h.tri = 0xFFFFFFFFu;
     if (s.num_nodes == 0) return h;
     RayInv r = make_ray_inv(o, d);
-    float4 rlo = s.nodes[0], rhi = s.nodes[1];
+    float4 rlo = gld(s.nodes), rhi = gld(s.nodes + 1);
     if (COUNT) cnt.nodes += 1;
     if (slab(r, rlo, rhi, h.t) < 0.0f) return h;
     uint32_t cur = __float_as_uint(rlo.w);
@@ -131,7 +150,7 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
             }
         } else {
             const float4* c = s.nodes + 2 * cur;
-            float4 alo = c[0], ahi = c[1], blo = c[2], bhi = c[3];
+            float4 alo = gld(c), ahi = gld(c + 1), blo = gld(c + 2), bhi = gld(c + 3);
             if (COUNT) cnt.nodes += 2;
             float ta = slab(r, alo, ahi, h.t);
             float tb = slab(r, blo, bhi, h.t);
@@ -159,7 +178,7 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
                                             uint32_t& overflow) {
     if (s.num_nodes == 0) return false;
     RayInv r = make_ray_inv(o, d);
-    float4 rlo = s.nodes[0], rhi = s.nodes[1];
+    float4 rlo = gld(s.nodes), rhi = gld(s.nodes + 1);
     if (COUNT) cnt.nodes += 1;
     if (slab(r, rlo, rhi, tmax) < 0.0f) return false;
     uint32_t cur = __float_as_uint(rlo.w);
@@ -178,7 +197,7 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
             }
         } else {
             const float4* c = s.nodes + 2 * cur;
-            float4 alo = c[0], ahi = c[1], blo = c[2], bhi = c[3];
+            float4 alo = gld(c), ahi = gld(c + 1), blo = gld(c + 2), bhi = gld(c + 3);
             if (COUNT) cnt.nodes += 2;
             float ta = slab(r, alo, ahi, tmax);
             float tb = slab(r, blo, bhi, tmax);
@@ -218,7 +237,7 @@ __device__ __forceinline__ float wide_plane(uint32_t bytes, int i, float s, floa
 // Slab-tests the four children of wide node nd against (r, tmax); returns how
 // many were hit and leaves their words in w[0..n) nearest first.
 __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, float tmax, uint32_t (&w)[4]) {
-    const float4 a = nd[0], b = nd[1], c = nd[2], cw = nd[3];
+    const float4 a = gld(nd), b = gld(nd + 1), c = gld(nd + 2), cw = gld(nd + 3);
     const uint32_t qlx = __float_as_uint(b.z), qly = __float_as_uint(b.w), qlz = __float_as_uint(c.x);
     const uint32_t qhx = __float_as_uint(c.y), qhy = __float_as_uint(c.z), qhz = __float_as_uint(c.w);
     w[0] = __float_as_uint(cw.x);

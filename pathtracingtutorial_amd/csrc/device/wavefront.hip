@@ -349,7 +349,8 @@ __device__ __forceinline__ TraceStack<LDS, TOTAL> thread_stack(const WfState& w,
 }
 
 template <bool COUNT, int BLOCK, int LDS, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, DevCounters* ctr) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_wf_extend(WfState w, int in_slot,
+                                                                                                DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
     const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     unsigned long long c0;
@@ -361,16 +362,27 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfState w, int in_slot, Dev
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
+    // One job: its scene is the same for every lane, so the traversal's base
+    // addresses stay in scalar registers (a uniform branch picks that copy).
+    const bool one_job = w.njobs == 1;
     while (qw.next(i, end)) {
         if (i < end) {
-            uint32_t p = q[i];
-            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
+            const uint32_t p = q[i];
             const float4 o = w.path[p].o, d = w.path[p].d;
-            Hit h = trace_closest_v<WIDE, COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
+            Hit h;
+            uint32_t mat = 0;
+            if (one_job) {
+                const DevScene& s = w.jobs[0].scene;
+                h = trace_closest_v<WIDE, COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
+                if (w.key_bins && h.tri != 0xFFFFFFFFu) mat = gld_u32(&s.faces[h.tri].w);
+            } else {
+                const DevScene& s = w.jobs[path_job(w, p)].scene;
+                h = trace_closest_v<WIDE, COUNT>(s, f3(o), f3(d), sk, cnt, ovf);
+                if (w.key_bins && h.tri != 0xFFFFFFFFu) mat = gld_u32(&s.faces[h.tri].w);
+            }
             w.hit[p] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
             // the sort's key, stored in queue order so the sort reads it contiguously
-            if (w.key_bins)
-                w.qkey[i] = h.tri == 0xFFFFFFFFu ? w.key_bins - 1 : min(s.faces[h.tri].w, w.key_bins - 2);
+            if (w.key_bins) w.qkey[i] = h.tri == 0xFFFFFFFFu ? w.key_bins - 1 : min(mat, w.key_bins - 2);
             rays++;
         }
     }
@@ -595,13 +607,16 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
+    const bool one_job = w.njobs == 1;  // uniform scene: scalar base addresses
     while (qw.next(i, end)) {
         if (i < end) {
             float4 o = qo[i], d = qd[i];
             const uint32_t p = __float_as_uint(d.w);
-            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any_v<WIDE, COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, qc[i], false);
+            const bool occ = one_job ? trace_any_v<WIDE, COUNT>(w.jobs[0].scene, f3(o), f3(d), o.w, sk, cnt, ovf)
+                                     : trace_any_v<WIDE, COUNT>(w.jobs[path_job(w, p)].scene, f3(o), f3(d), o.w, sk,
+                                                                cnt, ovf);
+            if (!occ) add_unoccluded(w, p, qc[i], false);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
@@ -619,13 +634,16 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCoun
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
     unsigned long long rays = 0;
+    const bool one_job = w.njobs == 1;
     while (qw.next(i, end)) {
         if (i < end) {
             float4 o = w.sp_o[i], d = w.sp_d[i];
             const uint32_t p = __float_as_uint(d.w);
-            const DevScene& s = w.jobs[w.njobs == 1 ? 0u : path_job(w, p)].scene;
             rays++;
-            if (!trace_any_v<WIDE, COUNT>(s, f3(o), f3(d), o.w, sk, cnt, ovf)) add_unoccluded(w, p, w.sp_c[i], true);
+            const bool occ = one_job ? trace_any_v<WIDE, COUNT>(w.jobs[0].scene, f3(o), f3(d), o.w, sk, cnt, ovf)
+                                     : trace_any_v<WIDE, COUNT>(w.jobs[path_job(w, p)].scene, f3(o), f3(d), o.w, sk,
+                                                                cnt, ovf);
+            if (!occ) add_unoccluded(w, p, w.sp_c[i], true);
         }
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
