@@ -69,6 +69,33 @@ __device__ __forceinline__ float slab(const RayInv& r, float4 lo, float4 hi, flo
     return t_min <= t_max ? t_min : -1.0f;
 }
 
+// The same test in fewer instructions.  When every component of inv is
+// finite, no product above is NaN, and then fmin(fmax(t1, m), fmax(t2, m)) =
+// max(min(t1, t2), m) and fmax(fmin(t1, M), fmin(t2, M)) = min(max(t1, t2), M)
+// exactly (min and max do not round; the per-axis updates commute), so entry
+// and exit are the reference's to the bit.  A zero (or subnormal) direction
+// component gives inf and possibly 0 * inf = NaN, which the reference's form
+// turns into "no constraint on this axis": such rays use slab().
+__device__ __forceinline__ float slab_fast(const RayInv& r, float4 lo, float4 hi, float tmax) {
+    const float x1 = (lo.x - r.o.x) * r.inv.x, x2 = (hi.x - r.o.x) * r.inv.x;
+    const float y1 = (lo.y - r.o.y) * r.inv.y, y2 = (hi.y - r.o.y) * r.inv.y;
+    const float z1 = (lo.z - r.o.z) * r.inv.z, z2 = (hi.z - r.o.z) * r.inv.z;
+    const float tn = fmaxf(fmaxf(fminf(x1, x2), fminf(y1, y2)), fmaxf(fminf(z1, z2), 0.0f));
+    const float tf = fminf(fminf(fmaxf(x1, x2), fmaxf(y1, y2)), fminf(fmaxf(z1, z2), tmax));
+    return tn <= tf ? tn : -1.0f;
+}
+
+template <bool FAST>
+__device__ __forceinline__ float slab_v(const RayInv& r, float4 lo, float4 hi, float tmax) {
+    if constexpr (FAST) return slab_fast(r, lo, hi, tmax);
+    else return slab(r, lo, hi, tmax);
+}
+
+// inv = 1 / d has no infinite (or NaN) component: slab_fast applies.
+__device__ __forceinline__ bool finite_inv(const RayInv& r) {
+    return fabsf(r.inv.x) <= kInf && fabsf(r.inv.y) <= kInf && fabsf(r.inv.z) <= kInf;
+}
+
 // Moller-Trumbore (triangle.h:13-51 / :53-76).  Returns t, or -1 on a miss;
 // u, v are the barycentrics of p1 and p2.
 __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& u, float& v, uint32_t& word) {
@@ -116,18 +143,17 @@ struct TraceStack {
 };
 
 // Closest hit.
-template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+template <bool FAST, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ Hit trace_closest_s_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, const TraceStack<LDS, TOTAL>& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     Hit h;
     h.t = kInf;
     h.u = h.v = 0.0f;
     h.tri = 0xFFFFFFFFu;
     if (s.num_nodes == 0) return h;
-    RayInv r = make_ray_inv(o, d);
     float4 rlo = gld(s.nodes), rhi = gld(s.nodes + 1);
     if (COUNT) cnt.nodes += 1;
-    if (slab(r, rlo, rhi, h.t) < 0.0f) return h;
+    if (slab_v<FAST>(r, rlo, rhi, h.t) < 0.0f) return h;
     uint32_t cur = __float_as_uint(rlo.w);
     int top = 0;
     while (true) {
@@ -152,8 +178,8 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
             const float4* c = s.nodes + 2 * cur;
             float4 alo = gld(c), ahi = gld(c + 1), blo = gld(c + 2), bhi = gld(c + 3);
             if (COUNT) cnt.nodes += 2;
-            float ta = slab(r, alo, ahi, h.t);
-            float tb = slab(r, blo, bhi, h.t);
+            float ta = slab_v<FAST>(r, alo, ahi, h.t);
+            float tb = slab_v<FAST>(r, blo, bhi, h.t);
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
@@ -171,16 +197,23 @@ __device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, co
     return h;
 }
 
-// Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).
 template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax,
+__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+                                               TraceCounters& cnt, uint32_t& overflow) {
+    const RayInv r = make_ray_inv(o, d);
+    if (finite_inv(r)) return trace_closest_s_impl<true, COUNT>(s, o, d, r, sk, cnt, overflow);
+    return trace_closest_s_impl<false, COUNT>(s, o, d, r, sk, cnt, overflow);
+}
+
+// Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).
+template <bool FAST, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_s_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, float tmax,
                                             const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
                                             uint32_t& overflow) {
     if (s.num_nodes == 0) return false;
-    RayInv r = make_ray_inv(o, d);
     float4 rlo = gld(s.nodes), rhi = gld(s.nodes + 1);
     if (COUNT) cnt.nodes += 1;
-    if (slab(r, rlo, rhi, tmax) < 0.0f) return false;
+    if (slab_v<FAST>(r, rlo, rhi, tmax) < 0.0f) return false;
     uint32_t cur = __float_as_uint(rlo.w);
     int top = 0;
     while (true) {
@@ -199,8 +232,8 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
             const float4* c = s.nodes + 2 * cur;
             float4 alo = gld(c), ahi = gld(c + 1), blo = gld(c + 2), bhi = gld(c + 3);
             if (COUNT) cnt.nodes += 2;
-            float ta = slab(r, alo, ahi, tmax);
-            float tb = slab(r, blo, bhi, tmax);
+            float ta = slab_v<FAST>(r, alo, ahi, tmax);
+            float tb = slab_v<FAST>(r, blo, bhi, tmax);
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
@@ -216,6 +249,15 @@ __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float
         cur = sk.get(--top);
     }
     return false;
+}
+
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax,
+                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            uint32_t& overflow) {
+    const RayInv r = make_ray_inv(o, d);
+    if (finite_inv(r)) return trace_any_s_impl<true, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
+    return trace_any_s_impl<false, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
 }
 
 // ---- 4-wide BVH with quantized child boxes --------------------------------
@@ -236,6 +278,7 @@ __device__ __forceinline__ float wide_plane(uint32_t bytes, int i, float s, floa
 
 // Slab-tests the four children of wide node nd against (r, tmax); returns how
 // many were hit and leaves their words in w[0..n) nearest first.
+template <bool FAST>
 __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, float tmax, uint32_t (&w)[4]) {
     const float4 a = gld(nd), b = gld(nd + 1), c = gld(nd + 2), cw = gld(nd + 3);
     const uint32_t qlx = __float_as_uint(b.z), qly = __float_as_uint(b.w), qlz = __float_as_uint(c.x);
@@ -251,7 +294,7 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
                                       wide_plane(qlz, i, b.y, a.z), 0.0f);
         const float4 hi = make_float4(wide_plane(qhx, i, a.w, a.x), wide_plane(qhy, i, b.x, a.y),
                                       wide_plane(qhz, i, b.y, a.z), 0.0f);
-        const float t = slab(r, lo, hi, tmax);
+        const float t = slab_v<FAST>(r, lo, hi, tmax);
         k[i] = (w[i] != kWideEmpty && t >= 0.0f) ? (__float_as_uint(t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
     }
     // sorting network on (key, word); equal keys keep the lower slot first
@@ -286,15 +329,14 @@ __device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], int n, 
 }
 
 // Closest hit over the wide BVH (node 0 is the root).
-template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+template <bool FAST, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ Hit trace_closest_w_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, const TraceStack<LDS, TOTAL>& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     Hit h;
     h.t = kInf;
     h.u = h.v = 0.0f;
     h.tri = 0xFFFFFFFFu;
     if (s.num_wnodes == 0) return h;
-    RayInv r = make_ray_inv(o, d);
     uint32_t cur = 0;
     int top = 0;
     while (true) {
@@ -317,7 +359,7 @@ __device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, co
         } else {
             uint32_t w[4];
             if (COUNT) cnt.nodes += 2;  // 64 B: two 32 B node records
-            const int n = wide_children(s.wnodes + 4 * cur, r, h.t, w);
+            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, h.t, w);
             if (n > 0) {
                 cur = wide_descend(w, n, sk, top, overflow);
                 continue;
@@ -329,13 +371,20 @@ __device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, co
     return h;
 }
 
-// Any hit in (1e-4, tmax) over the wide BVH.
 template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float tmax,
+__device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+                                               TraceCounters& cnt, uint32_t& overflow) {
+    const RayInv r = make_ray_inv(o, d);
+    if (finite_inv(r)) return trace_closest_w_impl<true, COUNT>(s, o, d, r, sk, cnt, overflow);
+    return trace_closest_w_impl<false, COUNT>(s, o, d, r, sk, cnt, overflow);
+}
+
+// Any hit in (1e-4, tmax) over the wide BVH.
+template <bool FAST, bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_w_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, float tmax,
                                             const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
                                             uint32_t& overflow) {
     if (s.num_wnodes == 0) return false;
-    RayInv r = make_ray_inv(o, d);
     uint32_t cur = 0;
     int top = 0;
     while (true) {
@@ -353,7 +402,7 @@ __device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float
         } else {
             uint32_t w[4];
             if (COUNT) cnt.nodes += 2;
-            const int n = wide_children(s.wnodes + 4 * cur, r, tmax, w);
+            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, tmax, w);
             if (n > 0) {
                 cur = wide_descend(w, n, sk, top, overflow);
                 continue;
@@ -363,6 +412,15 @@ __device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float
         cur = sk.get(--top);
     }
     return false;
+}
+
+template <bool COUNT, int LDS, int TOTAL>
+__device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float tmax,
+                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            uint32_t& overflow) {
+    const RayInv r = make_ray_inv(o, d);
+    if (finite_inv(r)) return trace_any_w_impl<true, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
+    return trace_any_w_impl<false, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
 }
 
 // Either BVH: the wide one when WIDE.

@@ -594,7 +594,8 @@ __device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, flo
 // Any-hit over a segmented shadow queue (area or environment): each path has
 // at most one entry, so the contribution is added without atomics.
 template <bool COUNT, int BLOCK, int LDS, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, int cur_slot,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_wf_shadow(WfState w, int count_slot,
+                                                                                                int cur_slot,
                                                      const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                      const float4* __restrict__ qc, DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
@@ -625,7 +626,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfState w, int count_slot, 
 // Any-hit over the flat punctual-light queue (several entries per path:
 // contributions are added with float atomics).
 template <bool COUNT, int BLOCK, int LDS, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow_punctual(WfState w, DevCounters* ctr) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_wf_shadow_punctual(
+    WfState w, DevCounters* ctr) {
     __shared__ uint32_t stack[LDS * BLOCK];
     const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
