@@ -288,14 +288,37 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
     w[2] = __float_as_uint(cw.z);
     w[3] = __float_as_uint(cw.w);
     uint32_t k[4];  // sort key: entry distance bits (t >= 0), or ~0 for a miss
+    if constexpr (FAST) {
+        // slab_fast with each axis's near plane picked once per node: with
+        // inv >= 0 the low plane's distance is the smaller one (a decoded low
+        // plane never exceeds the high one, and the rounding of - o and * inv
+        // keeps the order), so min/max per axis become a choice of bytes
+        const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+        const uint32_t nx = px ? qlx : qhx, fx = px ? qhx : qlx;
+        const uint32_t ny = py ? qly : qhy, fy = py ? qhy : qly;
+        const uint32_t nz = pz ? qlz : qhz, fz = pz ? qhz : qlz;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const float4 lo = make_float4(wide_plane(qlx, i, a.w, a.x), wide_plane(qly, i, b.x, a.y),
-                                      wide_plane(qlz, i, b.y, a.z), 0.0f);
-        const float4 hi = make_float4(wide_plane(qhx, i, a.w, a.x), wide_plane(qhy, i, b.x, a.y),
-                                      wide_plane(qhz, i, b.y, a.z), 0.0f);
-        const float t = slab_v<FAST>(r, lo, hi, tmax);
-        k[i] = (w[i] != kWideEmpty && t >= 0.0f) ? (__float_as_uint(t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+        for (int i = 0; i < 4; ++i) {
+            const float tnx = (wide_plane(nx, i, a.w, a.x) - r.o.x) * r.inv.x;
+            const float tfx = (wide_plane(fx, i, a.w, a.x) - r.o.x) * r.inv.x;
+            const float tny = (wide_plane(ny, i, b.x, a.y) - r.o.y) * r.inv.y;
+            const float tfy = (wide_plane(fy, i, b.x, a.y) - r.o.y) * r.inv.y;
+            const float tnz = (wide_plane(nz, i, b.y, a.z) - r.o.z) * r.inv.z;
+            const float tfz = (wide_plane(fz, i, b.y, a.z) - r.o.z) * r.inv.z;
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+            k[i] = (w[i] != kWideEmpty && tn <= tf) ? (__float_as_uint(tn) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float4 lo = make_float4(wide_plane(qlx, i, a.w, a.x), wide_plane(qly, i, b.x, a.y),
+                                          wide_plane(qlz, i, b.y, a.z), 0.0f);
+            const float4 hi = make_float4(wide_plane(qhx, i, a.w, a.x), wide_plane(qhy, i, b.x, a.y),
+                                          wide_plane(qhz, i, b.y, a.z), 0.0f);
+            const float t = slab(r, lo, hi, tmax);
+            k[i] = (w[i] != kWideEmpty && t >= 0.0f) ? (__float_as_uint(t) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+        }
     }
     // sorting network on (key, word); equal keys keep the lower slot first
     auto cswap = [&](int i, int j) {
@@ -314,16 +337,29 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
     return (k[0] != 0xFFFFFFFFu) + (k[1] != 0xFFFFFFFFu) + (k[2] != 0xFFFFFFFFu) + (k[3] != 0xFFFFFFFFu);
 }
 
-// Pushes w[1..n) farthest first; returns w[0].
-template <class Stack>
-__device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], int n, const Stack& sk, int& top,
-                                                 uint32_t& overflow) {
+// Pushes w[1..n) farthest first; returns w[0].  The wide build keeps only
+// trees whose stack need fits TOTAL entries, so no push can overflow.  While
+// three entries fit below the LDS part's end, the three slots above top are
+// written unconditionally (slots at and above top hold nothing live) and top
+// advances by n - 1: one LDS address for every case.
+template <int LDS, int TOTAL>
+__device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], int n, const TraceStack<LDS, TOTAL>& sk,
+                                                 int& top) {
+    const int m = n - 1;
+    if (top + 3 <= LDS) {
+        // (values, not elements picked by index: an indexed pick would put w in scratch)
+        const uint32_t w1 = w[1], w2 = w[2], w3 = w[3];
+        const uint32_t v1 = m >= 3 ? w2 : w1;
+        const uint32_t v0 = m >= 3 ? w3 : (m == 2 ? w2 : w1);
+        uint32_t* p = sk.stk + top * sk.stride;
+        p[0] = v0;
+        p[sk.stride] = v1;
+        p[2 * sk.stride] = w1;
+        top += m;
+    } else {
 #pragma unroll
-    for (int i = 3; i >= 1; --i) {
-        if (n > i) {
-            if (top < Stack::kTotal) sk.put(top++, w[i]);
-            else overflow++;
-        }
+        for (int i = 3; i >= 1; --i)
+            if (n > i) sk.put(top++, w[i]);
     }
     return w[0];
 }
@@ -361,7 +397,7 @@ __device__ __forceinline__ Hit trace_closest_w_impl(const DevScene& s, F3 o, F3 
             if (COUNT) cnt.nodes += 2;  // 64 B: two 32 B node records
             const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, h.t, w);
             if (n > 0) {
-                cur = wide_descend(w, n, sk, top, overflow);
+                cur = wide_descend(w, n, sk, top);
                 continue;
             }
         }
@@ -404,7 +440,7 @@ __device__ __forceinline__ bool trace_any_w_impl(const DevScene& s, F3 o, F3 d, 
             if (COUNT) cnt.nodes += 2;
             const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, tmax, w);
             if (n > 0) {
-                cur = wide_descend(w, n, sk, top, overflow);
+                cur = wide_descend(w, n, sk, top);
                 continue;
             }
         }
