@@ -11,7 +11,8 @@ Prints one JSON line:
              "plain", rendered from the same random numbers: the same estimate)
   roofline   the dominant traversal kernel's algorithmic bytes per launch over
              its average HIP-event duration, against the 8 TB/s HBM3E peak:
-             k_wf_extend (32 B per node record tested + 36 B per triangle test
+             k_wf_extend (32 B per 32-byte unit of node data fetched: a binary
+             node record is one, a 4-wide node two; + 36 B per triangle test
              + 52 B of ray/hit/queue I/O per ray); for --integrator path,
              k_render_plain (48 B per ray of I/O)
   cpu_baseline  the oracle (a C++ restatement of the reference integrator)
