@@ -714,6 +714,7 @@ constexpr int kPend = 5;     // float4 per path: rho[3][5], material | kNoVertex
 constexpr int kPayload = 3;  // float4 per shadow entry: gv[5], material, tu, tv, env weight rgb
 constexpr uint32_t kNoVertex = 0xFFFFFFFFu;
 constexpr uint32_t kWfLdsMaterials = 64;  // material sensitivities summed in LDS up to this many materials
+constexpr int kReplayStackLds = 16;        // replay shadow kernels: traversal stack entries in LDS (rest spilled)
 
 __device__ __forceinline__ float wave_sum_f(float v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1034,14 +1035,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
 // Any-hit over a segmented shadow queue of the replay: an unoccluded ray
 // adds its contribution and applies its sensitivities.
 template <bool COUNT, int BLOCK, bool ENV, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot, int cur_slot,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_wfa_shadow(WfState w, int count_slot, int cur_slot,
                                                       const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                       const float4* __restrict__ qc, const float4* __restrict__ qx,
                                                       DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * BLOCK];
+    __shared__ uint32_t stack[kReplayStackLds * BLOCK];
     __shared__ float macc[kWfLdsMaterials * kDP];
     clear_materials(macc);
-    const auto sk = thread_stack<kStackSize, BLOCK, stack_total<WIDE>()>(w, stack);
+    const auto sk = thread_stack<kReplayStackLds, BLOCK, stack_total<WIDE>()>(w, stack);
     const AdjointGrads G = *w.grads;
     const DevScene& s0 = w.jobs[0].scene;
     unsigned long long c0;
@@ -1087,11 +1088,11 @@ __global__ __launch_bounds__(BLOCK) void k_wfa_shadow(WfState w, int count_slot,
 }
 
 template <bool COUNT, int BLOCK, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_wfa_shadow_punctual(WfState w, DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * BLOCK];
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_wfa_shadow_punctual(WfState w, DevCounters* ctr) {
+    __shared__ uint32_t stack[kReplayStackLds * BLOCK];
     __shared__ float macc[kWfLdsMaterials * kDP];
     clear_materials(macc);
-    const auto sk = thread_stack<kStackSize, BLOCK, stack_total<WIDE>()>(w, stack);
+    const auto sk = thread_stack<kReplayStackLds, BLOCK, stack_total<WIDE>()>(w, stack);
     const AdjointGrads G = *w.grads;
     const DevScene& s0 = w.jobs[0].scene;
     const uint32_t n = min(w.ctrl[C_SP], w.sp_capacity);
@@ -1742,10 +1743,11 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
         g = q.out;
     }
     // traversal stack entries kept in global memory: one column per thread of the traversal grid
-    // (the replay's shadow kernels keep kStackSize entries in LDS)
+    // (the replay's shadow kernels keep kReplayStackLds entries in LDS)
     const int total = cs.wide ? kWideStack : kStackSize;
     const size_t spill = std::max(static_cast<size_t>(total - cfg.stack_lds) * g.extend * cfg.extend_block,
-                                  cs.adjoint ? static_cast<size_t>(total - kStackSize) * g.ashadow * cfg.extend_block
+                                  cs.adjoint ? static_cast<size_t>(total - kReplayStackLds) * g.ashadow *
+                                                   cfg.extend_block
                                              : size_t(0));
     if (spill > 0 && !wf->ensure_spill(spill)) {
         if (err) *err = "wavefront: out of device memory for the traversal stacks";
