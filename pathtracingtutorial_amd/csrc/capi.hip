@@ -584,12 +584,14 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
             // the 4-wide BVH, collapsed from the binary tree on the host
             std::vector<float> ref_nodes;
             dfs_layout(s->topo, ref_nodes);
+            // (a tree it cannot take, such as one with non-finite bounds, keeps
+            // the binary traversal only)
             ptamd::WideBvh wb;
             std::string werr;
-            if (!ptamd::collapse_wide_bvh(ref_nodes.data(), static_cast<uint32_t>(ref_nodes.size() / 8), &wb, &werr))
-                throw Error(PT_ERR_STATE, "wide BVH: " + werr);
+            const bool wide_ok =
+                ptamd::collapse_wide_bvh(ref_nodes.data(), static_cast<uint32_t>(ref_nodes.size() / 8), &wb, &werr);
             s->wide_max_stack = wb.max_stack;
-            if (wb.num_nodes > 0 && wb.max_stack <= static_cast<uint32_t>(ptd::kWideStack))
+            if (wide_ok && wb.num_nodes > 0 && wb.max_stack <= static_cast<uint32_t>(ptd::kWideStack))
                 s->wnodes.upload(reinterpret_cast<const float4*>(wb.words.data()), static_cast<size_t>(wb.num_nodes) * 4,
                                  s->stream);
         } else {

@@ -115,7 +115,11 @@ bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, st
         for (int i = 0; i < n; ++i) {
             if (++visits[set[i]] > 1) return fail("BVH node reached twice (not a tree)");
             kids[i] = read_node(nodes8, set[i]);
-            for (int a = 0; a < 3; ++a) P[a] = std::fmin(P[a], kids[i].lo[a]), Q[a] = std::fmax(Q[a], kids[i].hi[a]);
+            for (int a = 0; a < 3; ++a) {
+                if (!std::isfinite(kids[i].lo[a]) || !std::isfinite(kids[i].hi[a]))
+                    return fail("non-finite BVH bounds");
+                P[a] = std::fmin(P[a], kids[i].lo[a]), Q[a] = std::fmax(Q[a], kids[i].hi[a]);
+            }
         }
         for (int a = 0; a < 3; ++a) S[a] = axis_scale(P[a], Q[a]);
         uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0}, words[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};

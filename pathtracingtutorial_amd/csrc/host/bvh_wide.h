@@ -38,7 +38,7 @@ struct WideBvh {
 // 0).  Each wide node starts from its binary node's two children and opens
 // the child with the largest surface area until it has four children or
 // only leaves remain.  Nodes are numbered breadth first.  Returns false with
-// *err on a malformed tree.
+// *err on a malformed tree or non-finite bounds.
 bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, std::string* err);
 
 }  // namespace ptamd

@@ -185,3 +185,12 @@ def test_wide_bvh_rejects_a_malformed_tree():
     u[1, 3], u[1, 7] = LEAF, 1
     with pytest.raises(pt.PtError):
         pt.collapse_wide(nodes)
+
+
+def test_wide_bvh_rejects_non_finite_bounds():
+    nodes = np.zeros((1, 8), np.float32)
+    nodes[0, 4:7] = [1, 1, np.inf]
+    nodes.view(np.uint32)[0, 3] = LEAF
+    nodes.view(np.uint32)[0, 7] = 1
+    with pytest.raises(pt.PtError):
+        pt.collapse_wide(nodes)
