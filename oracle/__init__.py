@@ -50,6 +50,8 @@ _SIGS = {
     "oracle_dir_to_uv": (None, [_FP, _FP]),
     "oracle_uv_to_dir": (None, [_FP, _FP]),
     "oracle_env_pdf": (C.c_float, [_P, _FP]),
+    "oracle_emissive_pdf": (C.c_double, [_P, C.c_int, C.c_double, _DP]),
+    "oracle_sample_area": (C.c_int, [_P, _DP, _DP, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _DP, _DP, _DP]),
     "oracle_tex2d": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _FP]),
     "oracle_render": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                              C.c_int, C.c_int, C.c_int, _FP, _U64P]),
@@ -177,6 +179,18 @@ class OracleScene:
     def env_pdf(self, d) -> float:
         d = np.ascontiguousarray(d, np.float32)
         return float(lib().oracle_env_pdf(self.h, _f(d)))
+
+    def emissive_pdf(self, prim: int, dist: float, d) -> float:
+        d = np.ascontiguousarray(d, np.float64)
+        return float(lib().oracle_emissive_pdf(self.h, int(prim), float(dist), _d(d)))
+
+    def sample_area(self, p, n, x: int, y: int, frame: int, sample: int):
+        """(wi, L, pdf) of sampleAreaLight at p with normal n, or None."""
+        p = np.ascontiguousarray(p, np.float64)
+        n = np.ascontiguousarray(n, np.float64)
+        wi, L, pdf = np.zeros(3), np.zeros(3), C.c_double(0.0)
+        ok = lib().oracle_sample_area(self.h, _d(p), _d(n), x, y, frame, sample, _d(wi), _d(L), C.byref(pdf))
+        return (wi, L, pdf.value) if ok else None
 
     def render(self, camera, spf: int, max_depth: int, sampler: int, seed: int = 0, frame: int = 0,
                rect=None, threads: int = 0, fp64: bool = False):

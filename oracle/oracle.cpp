@@ -1691,6 +1691,24 @@ void oracle_uv_to_dir(const float uv[2], float d[3]) {
     d[0] = r.x; d[1] = r.y; d[2] = r.z;
 }
 float oracle_env_pdf(const oracle_scene* s, const float d[3]) { return envPdf(s, V3<float>(d[0], d[1], d[2])); }
+
+double oracle_emissive_pdf(const oracle_scene* s, int prim, double dist, const double d[3]) {
+    return emissivePdf<double>(s, prim, dist, V3<double>(d[0], d[1], d[2]));
+}
+
+int oracle_sample_area(const oracle_scene* s, const double p[3], const double n[3], int x, int y, uint32_t frame,
+                       uint32_t sample, double wi[3], double L[3], double* pdf) {
+    Sampler rng;
+    rng.initPCG(x, y, frame, sample, 0u);
+    Ctx c;
+    LightSample<double> ls =
+        sampleAreaLight<double>(s, V3<double>(p[0], p[1], p[2]), V3<double>(n[0], n[1], n[2]), rng, c);
+    if (!ls.valid) return 0;
+    wi[0] = ls.wi.x, wi[1] = ls.wi.y, wi[2] = ls.wi.z;
+    L[0] = ls.L.x, L[1] = ls.L.y, L[2] = ls.L.z;
+    *pdf = ls.pdf;
+    return 1;
+}
 void oracle_tex2d(const float* rgba, int w, int h, int ws, int wt, float u, float v, float out[4]) {
     tex2D(rgba, w, h, wrapMode(ws), wrapMode(wt), u, v, out);
 }
