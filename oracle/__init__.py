@@ -51,7 +51,8 @@ _SIGS = {
     "oracle_uv_to_dir": (None, [_FP, _FP]),
     "oracle_env_pdf": (C.c_float, [_P, _FP]),
     "oracle_emissive_pdf": (C.c_double, [_P, C.c_int, C.c_double, _DP]),
-    "oracle_sample_area": (C.c_int, [_P, _DP, _DP, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _DP, _DP, _DP]),
+    "oracle_sample_light": (C.c_int, [_P, C.c_int, _DP, _DP, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _DP, _DP,
+                                      _DP]),
     "oracle_tex2d": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _FP]),
     "oracle_render": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int,
                              C.c_int, C.c_int, C.c_int, _FP, _U64P]),
@@ -184,12 +185,13 @@ class OracleScene:
         d = np.ascontiguousarray(d, np.float64)
         return float(lib().oracle_emissive_pdf(self.h, int(prim), float(dist), _d(d)))
 
-    def sample_area(self, p, n, x: int, y: int, frame: int, sample: int):
-        """(wi, L, pdf) of sampleAreaLight at p with normal n, or None."""
+    def sample_light(self, env: bool, p, n, x: int, y: int, frame: int, sample: int):
+        """(wi, L, pdf) of sampleEnvLight (env) or sampleAreaLight at p with normal n, or None."""
         p = np.ascontiguousarray(p, np.float64)
         n = np.ascontiguousarray(n, np.float64)
         wi, L, pdf = np.zeros(3), np.zeros(3), C.c_double(0.0)
-        ok = lib().oracle_sample_area(self.h, _d(p), _d(n), x, y, frame, sample, _d(wi), _d(L), C.byref(pdf))
+        ok = lib().oracle_sample_light(self.h, 1 if env else 0, _d(p), _d(n), x, y, frame, sample, _d(wi), _d(L),
+                                       C.byref(pdf))
         return (wi, L, pdf.value) if ok else None
 
     def render(self, camera, spf: int, max_depth: int, sampler: int, seed: int = 0, frame: int = 0,

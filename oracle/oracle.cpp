@@ -1696,13 +1696,13 @@ double oracle_emissive_pdf(const oracle_scene* s, int prim, double dist, const d
     return emissivePdf<double>(s, prim, dist, V3<double>(d[0], d[1], d[2]));
 }
 
-int oracle_sample_area(const oracle_scene* s, const double p[3], const double n[3], int x, int y, uint32_t frame,
-                       uint32_t sample, double wi[3], double L[3], double* pdf) {
+int oracle_sample_light(const oracle_scene* s, int env, const double p[3], const double n[3], int x, int y,
+                        uint32_t frame, uint32_t sample, double wi[3], double L[3], double* pdf) {
     Sampler rng;
     rng.initPCG(x, y, frame, sample, 0u);
     Ctx c;
-    LightSample<double> ls =
-        sampleAreaLight<double>(s, V3<double>(p[0], p[1], p[2]), V3<double>(n[0], n[1], n[2]), rng, c);
+    const V3<double> P(p[0], p[1], p[2]), N(n[0], n[1], n[2]);
+    LightSample<double> ls = env ? sampleEnvLight<double>(s, P, N, rng, c) : sampleAreaLight<double>(s, P, N, rng, c);
     if (!ls.valid) return 0;
     wi[0] = ls.wi.x, wi[1] = ls.wi.y, wi[2] = ls.wi.z;
     L[0] = ls.L.x, L[1] = ls.L.y, L[2] = ls.L.z;

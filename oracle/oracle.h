@@ -83,11 +83,12 @@ float oracle_env_pdf(const oracle_scene* s, const float d[3]);
 /* emissiveTrianglePdf (light_pdf.cuh:29-54), FP64: solid-angle pdf of hitting
  * primitive prim at distance dist along direction d. */
 double oracle_emissive_pdf(const oracle_scene* s, int prim, double dist, const double d[3]);
-/* sampleAreaLight (direct_lighting.cuh:75-121), FP64, at point p with normal
- * n, drawing from the PCG sequence of (x, y, frame, sample); returns 1 and
- * the direction, radiance and solid-angle pdf of a valid sample, else 0. */
-int oracle_sample_area(const oracle_scene* s, const double p[3], const double n[3], int x, int y, uint32_t frame,
-                       uint32_t sample, double wi[3], double L[3], double* pdf);
+/* sampleAreaLight (env = 0, direct_lighting.cuh:75-121) or sampleEnvLight
+ * (env = 1, :123-154), FP64, at point p with normal n, drawing from the PCG
+ * sequence of (x, y, frame, sample); returns 1 and the direction, radiance
+ * and solid-angle pdf of a valid sample, else 0. */
+int oracle_sample_light(const oracle_scene* s, int env, const double p[3], const double n[3], int x, int y,
+                        uint32_t frame, uint32_t sample, double wi[3], double L[3], double* pdf);
 /* Bilinear texture fetch with CUDA normalized-coordinate semantics. */
 void oracle_tex2d(const float* rgba, int w, int h, int wrap_s, int wrap_t, float u, float v, float out[4]);
 

@@ -7,6 +7,11 @@ and light_pdf.cuh need the CUDA runtime).
   point, which for a uniformly emitting polygon has a closed form (Lambert:
   E = L/2 * sum over edges of theta_i * n . unit(v_i x v_(i+1))).  A wrong
   area, distance or cosine factor in the pdf changes the mean.
+* sampleEnvLight (direct_lighting.cuh:123-154): the same estimator under a
+  constant environment is pi L, and under an environment brighter above the
+  horizon (normal up) it is pi L_up (up to the bilinear blend of the rows at
+  the horizon); L is the texel value times 2 pi, the reference's radiance
+  scale (sampleEnvironmentRadiance, lights/envmap.h:43-45).
 * emissiveTrianglePdf (light_pdf.cuh:29-54) is a density over directions: for
   a single emitter, its integral over the sphere (directions that hit the
   triangle) is the triangle's selection probability, 1.  Estimated with
@@ -53,7 +58,7 @@ def test_area_light_estimator_is_the_irradiance(x):
     x, n = np.array(x), np.array([0.0, 1.0, 0.0])
     vals = []
     for k in range(20000):
-        s = sc.sample_area(x, n, 3, 5, 0, k)
+        s = sc.sample_light(False, x, n, 3, 5, 0, k)
         if s is None:
             vals.append(np.zeros(3))
             continue
@@ -89,3 +94,41 @@ def test_emissive_pdf_integrates_to_the_selection_probability():
     est = 4.0 * np.pi * pdf.sum() / N
     se = 4.0 * np.pi * np.sqrt(((np.concatenate([pdf, np.zeros(N - hit.size)]) - pdf.sum() / N) ** 2).sum() / (N - 1) / N)
     assert abs(est - 1.0) <= 4 * se and abs(est - 1.0) <= 0.03, (est, se)
+
+
+def _env_scene(env):
+    # one far-away, non-emitting triangle: the scene needs geometry, the estimator only the environment
+    verts = (_capi.Vertex * 3)()
+    for i, p in enumerate(TRI + np.array([0.0, -50.0, 0.0])):
+        verts[i].position = _capi.Vec3p.of(p)
+        verts[i].normal = _capi.Vec3p.of((0.0, 1.0, 0.0))
+    faces = (_capi.Face * 1)()
+    faces[0] = _capi.Face(0, 1, 2, 0)
+    m = _capi.Material()
+    m.base_color = _capi.Vec3p.of((0.5, 0.5, 0.5))
+    m.roughness = 0.5
+    m.ior = 1.5
+    return oracle.OracleScene(bytes(verts), 3, bytes(faces), 1, bytes(m), 1, env=env)
+
+
+@pytest.mark.parametrize("case", ["constant", "upper"])
+def test_env_light_estimator_is_the_irradiance(case):
+    h, w = 64, 128
+    env = np.zeros((h, w, 4), np.float32)
+    Lup, Ldown = np.array([0.8, 0.6, 0.3]), np.array([0.1, 0.2, 0.4])
+    env[:, :, :3] = Lup
+    if case == "upper":
+        # rows in the lower half (v > 0.5: directions below the horizon) are dimmer
+        env[h // 2:, :, :3] = Ldown
+    env[..., 3] = 1.0
+    sc = _env_scene(env)
+    x, n = np.array([0.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+    vals = []
+    for k in range(20000):
+        s = sc.sample_light(True, x, n, 7, 2, 1, k)
+        vals.append(np.zeros(3) if s is None else s[1] * max(0.0, n @ s[0]) / s[2])
+    vals = np.array(vals)
+    est, se = vals.mean(0), vals.std(0, ddof=1) / np.sqrt(len(vals))
+    exact = np.pi * (2.0 * np.pi * Lup)
+    tol = 0.01 if case == "constant" else 0.03  # the horizon rows blend the two halves bilinearly
+    assert np.all(np.abs(est - exact) <= 4 * se + tol * exact), (est, exact, se)
