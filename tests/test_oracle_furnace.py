@@ -88,3 +88,59 @@ def test_plane_under_constant_environment(case):
     # only the light-sampling half at maxDepth 1: strictly below the full estimate
     half = np.array([sc.render(cam, 64, 1, 0, 0, frame=f)[0].reshape(-1, 3).mean(0) for f in range(4)]).mean(0)
     assert np.all(half < est - 4 * se)
+
+
+def _lights():
+    pt_l = (_capi.PunctualLight * 2)()
+    pt_l[0].type, pt_l[0].range = 0, 3.402823466e38  # point, no range limit
+    pt_l[0].position = _capi.Vec3p.of((0.3, 2.0, -0.2))
+    pt_l[0].color = _capi.Vec3p.of((1.0, 0.8, 0.6))
+    pt_l[0].intensity = 3.0
+    pt_l[1].type = 1  # directional
+    pt_l[1].direction = _capi.Vec3p.of(tuple(np.array([0.4, -1.0, 0.3]) / np.linalg.norm([0.4, -1.0, 0.3])))
+    pt_l[1].color = _capi.Vec3p.of((0.5, 0.6, 1.0))
+    pt_l[1].intensity = 0.7
+    return pt_l
+
+
+def punctual_closed_form(mat, wo):
+    """evaluatePunctualLights (direct_lighting.cuh:158-200) at the origin of the plane, normal +y:
+    point light color * I / (d^2 + 1e-7) * f(wo, wi) plus directional color * I * f(wo, -dir)."""
+    n = np.array([0.0, 1.0, 0.0])
+    L = _lights()
+    p = np.array([L[0].position.x, L[0].position.y, L[0].position.z])
+    d2 = p @ p
+    f0, _ = oracle.bsdf_eval(mat, wo, p / np.sqrt(d2), n, True, fp64=True)
+    c0 = np.array([L[0].color.x, L[0].color.y, L[0].color.z]) * L[0].intensity / (d2 + 1e-7)
+    wd = -np.array([L[1].direction.x, L[1].direction.y, L[1].direction.z])
+    f1, _ = oracle.bsdf_eval(mat, wo, wd, n, True, fp64=True)
+    c1 = np.array([L[1].color.x, L[1].color.y, L[1].color.z]) * L[1].intensity
+    return c0 * f0 + c1 * f1
+
+
+@pytest.mark.parametrize("case", ["diffuse", "glossy-metal"])
+def test_plane_under_punctual_lights(case):
+    # no environment: every path that leaves the plane escapes dark, and punctual lights are
+    # evaluated without sampling, so each pixel is the closed form (up to the pixel's footprint)
+    kw = dict(metallic=0.0, roughness=0.8) if case == "diffuse" else dict(metallic=1.0, roughness=0.4)
+    mat = _material(**kw)
+    verts = (_capi.Vertex * 4)()
+    for i, p in enumerate([(-1000, 0, -1000), (1000, 0, -1000), (1000, 0, 1000), (-1000, 0, 1000)]):
+        verts[i].position = _capi.Vec3p.of(p)
+        verts[i].normal = _capi.Vec3p.of((0, 1, 0))
+    faces = (_capi.Face * 2)()
+    faces[0] = _capi.Face(0, 2, 1, 0)
+    faces[1] = _capi.Face(0, 3, 2, 0)
+    sc = oracle.OracleScene(bytes(verts), 4, bytes(faces), 2, bytes(mat), 1, lights=bytes(_lights()), nl=2)
+    cam = _capi.Camera()
+    cam.position = _capi.Vec3p.of((0.0, 1.0, 1.2))
+    cam.look_at = _capi.Vec3p.of((0.0, 0.0, 0.0))
+    cam.up = _capi.Vec3p.of((0.0, 1.0, 0.0))
+    cam.fov, cam.focal_distance = 0.2, 1.0
+    cam.width = cam.height = 4
+    wo = np.array([0.0, 1.0, 1.2]) / np.linalg.norm([0.0, 1.0, 1.2])
+    exact = punctual_closed_form(mat, wo)
+    img = sc.render(cam, 4, 3, 0, 0, frame=0, fp64=True)[0].reshape(-1, 3)
+    # pixels straddle the origin symmetrically: their mean is the closed form to second order
+    assert np.allclose(img.mean(0), exact, rtol=5e-4), (img.mean(0), exact)
+    assert np.allclose(img, exact, rtol=2e-2)
