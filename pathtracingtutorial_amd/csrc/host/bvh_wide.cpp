@@ -61,6 +61,96 @@ uint32_t quant_hi(float v, float s, float p) {
     return q;
 }
 
+
+// Which binary nodes become wide nodes.  A wide node costs one fetch and
+// four box tests whenever a ray enters its box, so the expected traversal
+// cost of a collapse is proportional to the summed surface area of the
+// binary nodes chosen as wide nodes (surface area heuristic); every binary
+// leaf ends up in exactly one wide node whatever the choice, so the leaves'
+// cost is fixed.  best[k] = the least cost of covering a subtree by at most
+// k children of its parent wide node (k = 1: the subtree's root is one child,
+// a leaf or a wide node of its own), computed bottom-up:
+//   wide(n)   = A(n) + min_j best_l[j] + best_r[4 - j]   (n opened as a wide node)
+//   best_n[1] = leaf ? 0 : wide(n)
+//   best_n[k] = min(best_n[1], min_j best_l[j] + best_r[k - j])
+// split[k] records the j of the minimum (0: n stays one child).  This is the
+// dynamic program of Ylitie, Karras and Laine, "Efficient incoherent ray
+// traversal on GPUs through compressed wide BVHs" (HPG 2017), restricted to
+// binary leaves kept as they are.
+struct CutPlan {
+    std::vector<double> best;     // 5 per node (index 1..4)
+    std::vector<uint8_t> split;   // 5 per node
+    std::vector<uint8_t> split4;  // j of wide(n)
+};
+
+bool plan_cuts(const float* nodes8, uint32_t num_nodes, CutPlan* plan, std::string* err) {
+    plan->best.assign(5 * static_cast<size_t>(num_nodes), 0.0);
+    plan->split.assign(5 * static_cast<size_t>(num_nodes), 0);
+    plan->split4.assign(num_nodes, 1);
+    // breadth-first order from the root: children after parents
+    std::vector<uint32_t> order;
+    std::vector<uint8_t> seen(num_nodes, 0);
+    order.push_back(0);
+    seen[0] = 1;
+    for (size_t i = 0; i < order.size(); ++i) {
+        const BinNode b = read_node(nodes8, order[i]);
+        if (b.leaf()) continue;
+        for (uint32_t c : {b.d0, b.d1}) {
+            if (c >= num_nodes) {
+                if (err) *err = "BVH child index out of range";
+                return false;
+            }
+            if (seen[c]) {
+                if (err) *err = "BVH node reached twice (not a tree)";
+                return false;
+            }
+            seen[c] = 1;
+            order.push_back(c);
+        }
+    }
+    for (size_t i = order.size(); i-- > 0;) {
+        const uint32_t k = order[i];
+        const BinNode b = read_node(nodes8, k);
+        double* bn = &plan->best[5 * static_cast<size_t>(k)];
+        uint8_t* sp = &plan->split[5 * static_cast<size_t>(k)];
+        if (b.leaf()) {
+            for (int m = 1; m <= 4; ++m) bn[m] = 0.0, sp[m] = 0;
+            continue;
+        }
+        const double* bl = &plan->best[5 * static_cast<size_t>(b.d0)];
+        const double* br = &plan->best[5 * static_cast<size_t>(b.d1)];
+        auto pair_min = [&](int m, int* arg) {
+            double v = INFINITY;
+            for (int j = 1; j < m; ++j)
+                if (bl[j] + br[m - j] < v) v = bl[j] + br[m - j], *arg = j;
+            return v;
+        };
+        int j4 = 1;
+        bn[1] = static_cast<double>(area(b)) + pair_min(4, &j4);
+        sp[1] = 0;
+        plan->split4[k] = static_cast<uint8_t>(j4);
+        for (int m = 2; m <= 4; ++m) {
+            int j = 1;
+            const double v = pair_min(m, &j);
+            if (v < bn[1]) bn[m] = v, sp[m] = static_cast<uint8_t>(j);
+            else bn[m] = bn[1], sp[m] = 0;
+        }
+    }
+    return true;
+}
+
+// Appends the children that subtree n contributes with a budget of m slots.
+void emit_cut(const CutPlan& plan, const float* nodes8, uint32_t n, int m, uint32_t* set, int* count) {
+    const int j = plan.split[5 * static_cast<size_t>(n) + m];
+    if (j == 0) {
+        set[(*count)++] = n;
+        return;
+    }
+    const BinNode b = read_node(nodes8, n);
+    emit_cut(plan, nodes8, b.d0, j, set, count);
+    emit_cut(plan, nodes8, b.d1, m - j, set, count);
+}
+
 }  // namespace
 
 bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, std::string* err) {
@@ -76,6 +166,11 @@ bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, st
         uint32_t stack;  // stack entries its ancestors can leave pushed
         uint32_t level;
     };
+    CutPlan plan;
+    if (!plan_cuts(nodes8, num_nodes, &plan, err)) {
+        *out = WideBvh();
+        return false;
+    }
     std::deque<Item> todo;
     todo.push_back({0u, 0u, 1u});
     out->num_nodes = 1;
@@ -89,25 +184,9 @@ bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, st
         int n = 0;
         if (b.leaf()) {
             set[n++] = it.bin;  // a leaf root: one leaf child
-        } else {
-            if (b.d0 >= num_nodes || b.d1 >= num_nodes) return fail("BVH child index out of range");
-            set[n++] = b.d0;
-            set[n++] = b.d1;
-        }
-        while (n < 4) {  // open the internal child with the largest surface area
-            int best = -1;
-            float best_a = -1.0f;
-            for (int i = 0; i < n; ++i) {
-                const BinNode c = read_node(nodes8, set[i]);
-                if (!c.leaf() && area(c) > best_a) best = i, best_a = area(c);
-            }
-            if (best < 0) break;
-            const BinNode c = read_node(nodes8, set[best]);
-            if (c.d0 >= num_nodes || c.d1 >= num_nodes) return fail("BVH child index out of range");
-            for (int i = n; i > best + 1; --i) set[i] = set[i - 1];  // keep the spatial order
-            set[best] = c.d0;
-            set[best + 1] = c.d1;
-            ++n;
+        } else {  // the cut the cost table chose: up to four descendants, in spatial order
+            emit_cut(plan, nodes8, b.d0, plan.split4[it.bin], set, &n);
+            emit_cut(plan, nodes8, b.d1, 4 - plan.split4[it.bin], set, &n);
         }
         BinNode kids[4];
         float P[3], Q[3], S[3];

@@ -35,10 +35,12 @@ struct WideBvh {
 // Collapses a binary BVH given in the reference's node layout
 // (accel/bvh_common.h:16-24: bmin[3], data0, bmax[3], data1; internal nodes
 // hold both child indices, leaves kLeafFlag | first and the count; root at
-// 0).  Each wide node starts from its binary node's two children and opens
-// the child with the largest surface area until it has four children or
-// only leaves remain.  Nodes are numbered breadth first.  Returns false with
-// *err on a malformed tree or non-finite bounds.
+// 0).  Which binary nodes become wide nodes is chosen by a dynamic program
+// that minimises the summed surface area of the wide nodes (the surface area
+// heuristic; binary leaves are kept as they are); each wide node's children
+// are the up to four descendants of its binary node that the program picked,
+// in the binary tree's left-to-right order.  Nodes are numbered breadth
+// first.  Returns false with *err on a malformed tree or non-finite bounds.
 bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, std::string* err);
 
 }  // namespace ptamd

@@ -194,3 +194,121 @@ def test_wide_bvh_rejects_non_finite_bounds():
     nodes.view(np.uint32)[0, 7] = 1
     with pytest.raises(pt.PtError):
         pt.collapse_wide(nodes)
+
+
+def _area(lo, hi):
+    d = (np.asarray(hi, np.float64) - np.asarray(lo, np.float64))
+    return 2.0 * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2])
+
+
+def _wide_cost(wide, leaf_box):
+    """surface-area cost of a collapse: the summed area of every wide node's (exact) box."""
+    total = 0.0
+
+    def box(w):
+        nonlocal total
+        _, _, words, _ = _decode(wide[w])
+        lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+        for word in (int(x) for x in words if x != EMPTY):
+            clo, chi = leaf_box[word & 0x7FFFFFFF] if word & LEAF else box(word)
+            lo, hi = np.minimum(lo, clo), np.maximum(hi, chi)
+        total += _area(lo, hi)
+        return lo, hi
+
+    box(0)
+    return total
+
+
+def _random_tree(rng, leaves):
+    """a random binary tree in the reference's node layout; parents are the unions of their children."""
+    nodes = []
+
+    def make(lo_leaf, hi_leaf):
+        k = len(nodes)
+        nodes.append(None)
+        if hi_leaf - lo_leaf == 1:
+            c = rng.random(3) * 10.0
+            e = rng.random(3) * rng.choice([0.1, 1.0, 4.0])
+            nodes[k] = (c - e, c + e, LEAF | lo_leaf, 1)
+        else:
+            m = int(rng.integers(lo_leaf + 1, hi_leaf))
+            a, b = make(lo_leaf, m), make(m, hi_leaf)
+            nodes[k] = (np.minimum(nodes[a][0], nodes[b][0]), np.maximum(nodes[a][1], nodes[b][1]), a, b)
+        return k
+
+    make(0, leaves)
+    arr = np.zeros((len(nodes), 8), np.float32)
+    for k, (lo, hi, d0, d1) in enumerate(nodes):
+        arr[k, 0:3], arr[k, 4:7] = lo, hi
+        arr.view(np.uint32)[k, 3], arr.view(np.uint32)[k, 7] = d0, d1
+    return arr
+
+
+def _internal(nodes):
+    u = nodes.view(np.uint32)
+    return [k for k in range(nodes.shape[0]) if not u[k, 3] & LEAF]
+
+
+def _cut_size(nodes, chosen, k):
+    """children of wide node k when exactly the binary nodes in `chosen` are wide nodes."""
+    u = nodes.view(np.uint32)
+    size = 0
+    for c in (int(u[k, 3]), int(u[k, 7])):
+        size += 1 if (u[c, 3] & LEAF or c in chosen) else _cut_size(nodes, chosen, c)
+    return size
+
+
+def _brute_force_cost(nodes):
+    """least summed area over every set of binary nodes that is a valid 4-wide collapse (root included)."""
+    inner = [k for k in _internal(nodes) if k != 0]
+    best = np.inf
+    for mask in range(1 << len(inner)):
+        chosen = {0} | {inner[i] for i in range(len(inner)) if mask >> i & 1}
+        if all(_cut_size(nodes, chosen, k) <= 4 for k in chosen):
+            best = min(best, sum(_area(nodes[k, 0:3], nodes[k, 4:7]) for k in chosen))
+    return best
+
+
+def _leaf_boxes(nodes):
+    u = nodes.view(np.uint32)
+    return {int(u[k, 3] & 0x7FFFFFFF): (nodes[k, 0:3].astype(np.float64), nodes[k, 4:7].astype(np.float64))
+            for k in range(nodes.shape[0]) if u[k, 3] & LEAF}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_wide_bvh_collapse_is_the_least_surface_area_one(seed):
+    # brute force over every valid collapse of a random 12-leaf tree (11 internal nodes, 1024 sets)
+    rng = np.random.default_rng(seed)
+    nodes = _random_tree(rng, 12)
+    wide, _ = pt.collapse_wide(nodes)
+    got = _wide_cost(wide, _leaf_boxes(nodes))
+    assert got == pytest.approx(_brute_force_cost(nodes), rel=1e-6)
+
+
+def _greedy_cost(nodes):
+    """the largest-area-first collapse: open the internal child of largest area until four children."""
+    u = nodes.view(np.uint32)
+    total, todo = 0.0, [0]
+    while todo:
+        k = todo.pop()
+        total += _area(nodes[k, 0:3], nodes[k, 4:7])
+        kids = [int(u[k, 3]), int(u[k, 7])]
+        while len(kids) < 4:
+            inner = [c for c in kids if not u[c, 3] & LEAF]
+            if not inner:
+                break
+            c = max(inner, key=lambda c: _area(nodes[c, 0:3], nodes[c, 4:7]))
+            i = kids.index(c)
+            kids[i:i + 1] = [int(u[c, 3]), int(u[c, 7])]
+        todo += [c for c in kids if not u[c, 3] & LEAF]
+    return total
+
+
+@pytest.mark.parametrize("name", ["row100", "soup200", "soup3000", "cornell"])
+def test_wide_bvh_collapse_costs_no_more_than_greedy(golden, name):
+    nodes, *_ = _case(golden, name)
+    if nodes.view(np.uint32)[0, 3] & LEAF:
+        pytest.skip("single-leaf tree")
+    wide, _ = pt.collapse_wide(nodes)
+    got = _wide_cost(wide, _leaf_boxes(nodes))
+    assert got <= _greedy_cost(nodes) * (1 + 1e-6)
