@@ -517,6 +517,35 @@ struct DeferNee {
     }
 };
 
+// One queue item of the shading kernel with its path record and hit as
+// loaded from memory (item i of the wave's grab, valid while i < end).
+struct ShadeItem {
+    bool has = false;
+    int seg = 0;
+    uint32_t i = 0, end = 0, p = 0;
+    float4 o, d, tp, lo, hv;
+    uint4 rng;
+    __device__ __forceinline__ void fetch(const WfState& w) {
+        const PathRec& R = w.path[p];
+        o = R.o;
+        d = R.d;
+        tp = R.tp;
+        rng = R.rng;
+        lo = w.lo[p];
+        hv = w.hit[p];
+    }
+    __device__ __forceinline__ void unpack(PathState& st) const {
+        st.o = f3(o);
+        st.d = f3(d);
+        st.last_pdf = d.w;
+        st.tp = f3(tp);
+        const uint32_t bits = __float_as_uint(tp.w);
+        st.depth = static_cast<int>(bits & 0xFFFFu);
+        st.last_delta = (bits >> 31) != 0;
+        st.lo = f3(lo);
+    }
+};
+
 // WAVES > 0 asks the compiler for at least that many waves per SIMD (fewer
 // registers, possibly spills): the shading kernel's occupancy knob.
 template <int BLOCK, int WAVES>
@@ -530,13 +559,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
     QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
-    uint32_t i, end;
-    while (qw.next(i, end)) {
-        const int seg = qw.seg;
-        const bool active = i < end;
-        uint32_t p = 0;
+    // Three items per lane in flight: the one being shaded (A), the next one
+    // whose path record and hit are being loaded (B) and the one after it
+    // whose queue entry is being loaded (C).  Shading is a chain of
+    // dependent gathers at two waves per SIMD; loading B and C during A
+    // takes the queue entry -> path record steps off the chain.
+    ShadeItem b, c;
+    b.has = qw.next(b.i, b.end);
+    b.seg = qw.seg;
+    if (b.has && b.i < b.end) b.p = q[b.i];
+    c.has = b.has && qw.next(c.i, c.end);
+    c.seg = qw.seg;
+    if (c.has && c.i < c.end) c.p = q[c.i];
+    if (b.has && b.i < b.end) b.fetch(w);
+    while (b.has) {
+        const ShadeItem a = b;
+        b = c;
+        if (b.has && b.i < b.end) b.fetch(w);
+        c.has = c.has && qw.next(c.i, c.end);
+        c.seg = qw.seg;
+        if (c.has && c.i < c.end) c.p = q[c.i];
+        const int seg = a.seg;
+        const bool active = a.i < a.end;
+        const uint32_t p = a.p;
         PathState st;
-        uint4 r = make_uint4(0, 0, 0, 0);
+        uint4 r = a.rng;
         bool cont = false;
         DeferNee nee;
         nee.base = seg_base(n, seg);
@@ -554,17 +601,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         nee.sp_d = w.sp_d;
         nee.sp_c = w.sp_c;
         if (active) {
-            p = q[i];
             nee.path = p;
-            load_path(w, p, st, r);
+            a.unpack(st);
             const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
             const RenderParams& rp = job.params;
-            float4 hv = w.hit[p];
             Hit h;
-            h.t = hv.x;
-            h.u = hv.y;
-            h.v = hv.z;
-            h.tri = __float_as_uint(hv.w);
+            h.t = a.hv.x;
+            h.u = a.hv.y;
+            h.v = a.hv.z;
+            h.tri = __float_as_uint(a.hv.w);
             if (st.depth < rp.max_depth) {  // a job of smaller depth than the batch's stops here
                 Sampler rng = sampler_of(rp.sampler, r);
                 cont = bounce(job.scene, rp, h, st, rng, nee);
