@@ -17,7 +17,10 @@ FP64 (no reference values exist: disney_bsdf.cuh needs CUDA):
 * pdf: the visible-normal density D_v(h) = G1(v) max(0, v.h) D(h) / cos_v
   (Heitz 2018) over 4 v.h for the specular lobe, cos / pi for diffuse and
   sheen, D cos_h / (4 v.h) for clearcoat, mixed with the reference's lobe
-  selection weights (disney_bsdf.cuh:184-205).
+  selection weights (disney_bsdf.cuh:184-205);
+* transmission: Walter et al. 2007's rough-dielectric BTDF with the exact
+  unpolarised Fresnel reflectance, and its refracted density (see the test
+  for the two ways the reference departs from it).
 
 eval returns f times cos_l.  A wrong alpha mapping, a dropped 1/4 or pi, a
 Smith term of the wrong form or a Fresnel on the wrong angle fails these.
@@ -131,3 +134,51 @@ def test_eval_at_normal_incidence():
     mat = _material(metallic=0.0, roughness=0.6, specular_factor=0.0)
     f, _ = oracle.bsdf_eval(mat, n, n, n, True, fp64=True)
     assert np.allclose(f, np.array([0.8, 0.6, 0.4]) / np.pi, rtol=1e-5)  # Lambert: base / pi
+
+
+def _fresnel_dielectric(ci, eta):
+    """unpolarised Fresnel reflectance, relative index eta = n_incident / n_transmitted."""
+    s2t = eta * eta * (1.0 - ci * ci)
+    if s2t >= 1.0:
+        return 1.0
+    ct = np.sqrt(1.0 - s2t)
+    r_perp = (eta * ci - ct) / (eta * ci + ct)
+    r_par = (ci - eta * ct) / (ci + eta * ct)
+    return 0.5 * (r_perp ** 2 + r_par ** 2)
+
+
+@pytest.mark.parametrize("roughness", [0.5, 0.8])
+def test_transmission_is_walters_rough_dielectric_btdf(roughness):
+    # Walter et al. 2007, "Microfacet models for refraction through rough surfaces", eq. 21
+    # with h_t = -(eta wo + wi) / |.|:  f_t = |wo.h||wi.h| / (|wo.n||wi.n|) (1 - F) G D / (eta wo.h + wi.h)^2,
+    # and the refracted density D_v(h) |wi.h| / (eta wo.h + wi.h)^2 (eq. 17's Jacobian).  Two properties of
+    # the reference stand beside them: transmitted f carries no cosine (the reflection branch multiplies
+    # by cos_l, disney_bsdf.cuh:319) and the pdf has one more eta^2 (disney_bsdf.cuh:247-250)
+    mat = _material(metallic=0.2, roughness=roughness, spec_trans=0.9, ior=1.5)
+    base = np.array([0.8, 0.6, 0.4])
+    n = np.array([0.0, 0.0, 1.0])
+    eta = 1.0 / 1.5
+    a = roughness * roughness
+    w = np.array([(1 - 0.2) * (1 - 0.9), 1.0, 0.0, (1 - 0.2) * 0.9, 0.0])
+    w_t = w[3] / w.sum()
+    rng = np.random.default_rng(5)
+    checked = 0
+    while checked < 60:
+        wo = _upper(rng, n)
+        wi = -_upper(rng, n)
+        h = -(eta * wo + wi)
+        h /= np.linalg.norm(h)
+        den = eta * (wo @ h) + wi @ h
+        if abs(den) < 0.1:
+            continue
+        cv, cl = abs(wo @ n), abs(wi @ n)
+        D = _ggx_d(abs(h @ n), a)
+        G = 1.0 / (1.0 + _smith_lambda(cl, a) + _smith_lambda(cv, a))
+        F = _fresnel_dielectric(abs(wo @ h), eta)
+        ft = base * (1 - 0.2) * 0.9 * abs(wo @ h) * abs(wi @ h) / (cv * cl) * (1 - F) * G * D / den ** 2
+        g1 = 1.0 / (1.0 + _smith_lambda(cv, a))
+        pdf = w_t * g1 * abs(wo @ h) * D / cv * abs(wi @ h) / den ** 2 * eta ** 2
+        f, p = oracle.bsdf_eval(mat, wo, wi, n, True, fp64=True)
+        assert np.allclose(f, ft, rtol=2e-5), (wo, wi, f, ft)
+        assert p == pytest.approx(pdf, rel=2e-5), (wo, wi, p, pdf)
+        checked += 1
