@@ -64,6 +64,7 @@ __device__ __forceinline__ void add_env_grad(const DevScene& s, const AdjointGra
 
 template <bool COUNT>
 struct AdjointNee {
+    static constexpr bool kDrawAhead = false;
     TraceCtx& c;
     const AdjPath& a;
     const AdjointGrads& G;

@@ -111,10 +111,17 @@ __device__ __forceinline__ bool occluded(const DevScene& s, const ShadowRay& r, 
 
 // sampleAreaLight (direct_lighting.cuh:75-121) up to, not including, the
 // visibility test, which is its last rejection; sh gets the shadow ray.
-__device__ __forceinline__ LightSample sample_area_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
-    LightSample ls;
-    ls.valid = false;
-    if (s.num_emissive == 0 || s.emissive == nullptr) return ls;
+// The half of sampleAreaLight (direct_lighting.cuh:75-121) that does not
+// depend on the shading point: the emitter picked by the alias table, the
+// point and normal on it, its area, pmf and radiance.  It consumes the four
+// dimensions the reference draws (a1, a2, t1, t2) before any early exit.
+struct AreaGeom {
+    F3 lp, ln, radiance;
+    float area, pmf;
+    uint32_t key;  // sampler position the sample started at
+};
+__device__ __forceinline__ void area_geom(const DevScene& s, Sampler& rng, AreaGeom& g) {
+    g.key = rng.key();
     float a1 = rng.next();
     float a2 = rng.next();
     uint4 ref = s.emissive[sample_alias(s.tri_alias, s.num_emissive, a1, a2)];
@@ -126,30 +133,48 @@ __device__ __forceinline__ LightSample sample_area_nv(const DevScene& s, F3 p, F
     // light_sampler.h:58-71
     float su0 = sqrtf(t1);
     float b0 = 1.0f - su0, b1 = t2 * su0, b2 = 1.0f - b0 - b1;
-    F3 lp = p0 * b0 + p1 * b1 + p2 * b2;
+    g.lp = p0 * b0 + p1 * b1 + p2 * b2;
     F3 ln = normalize(f3(s.vnrm[face.x]) * b0 + f3(s.vnrm[face.y]) * b1 + f3(s.vnrm[face.z]) * b2);
     F3 e1 = p1 - p0, e2 = p2 - p0;
     if (length(ln) == 0.0f) ln = normalize(cross(e1, e2));
-    F3 tl = lp - p;
+    g.ln = ln;
+    g.area = 0.5f * length(cross(e1, e2));
+    g.pmf = __uint_as_float(ref.y);
+    const pt_material& lm = s.materials[face.w];
+    g.radiance = f3(lm.emissive.x, lm.emissive.y, lm.emissive.z) * lm.emissive_strength;
+}
+// The rest of sampleAreaLight, at shading point p with normal n.
+__device__ __forceinline__ LightSample area_finish(F3 p, F3 n, const AreaGeom& g, ShadowRay& sh) {
+    LightSample ls;
+    ls.valid = false;
+    F3 tl = g.lp - p;
     float d2 = dot(tl, tl);
     if (d2 <= 1e-8f) return ls;
     float dist = sqrtf(d2);
     F3 wi = tl / dist;
     float cs = fmaxf(0.0f, dot(n, wi));
     if (cs <= 0.0f) return ls;
-    float area = 0.5f * length(cross(e1, e2));
-    if (area <= 0.0f) return ls;
-    float cl = fabsf(dot(ln, -wi));
+    if (g.area <= 0.0f) return ls;
+    float cl = fabsf(dot(g.ln, -wi));
     if (cl <= 1e-5f) return ls;
-    float pdf = (__uint_as_float(ref.y) / area) * d2 / cl;
+    float pdf = (g.pmf / g.area) * d2 / cl;
     if (pdf <= 0.0f) return ls;
     sh = make_shadow(p, n, wi, dist);
-    const pt_material& lm = s.materials[face.w];
     ls.wi = wi;
-    ls.radiance = f3(lm.emissive.x, lm.emissive.y, lm.emissive.z) * lm.emissive_strength;
+    ls.radiance = g.radiance;
     ls.pdf = pdf;
     ls.valid = true;
     return ls;
+}
+__device__ __forceinline__ LightSample sample_area_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
+    if (s.num_emissive == 0 || s.emissive == nullptr) {
+        LightSample ls;
+        ls.valid = false;
+        return ls;
+    }
+    AreaGeom g;
+    area_geom(s, rng, g);
+    return area_finish(p, n, g, sh);
 }
 
 template <bool COUNT>
@@ -162,9 +187,16 @@ __device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n
 
 // sampleEnvironmentLight (direct_lighting.cuh:123-154) without the final
 // visibility test; sh gets the shadow ray.
-__device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
-    LightSample ls;
-    ls.valid = false;
+// The half of sampleEnvLight (direct_lighting.cuh:123-154) that does not
+// depend on the shading point: the texel picked by the alias table, the
+// jittered direction in it, its pdf and radiance (four dimensions).
+struct EnvGeom {
+    F3 wi, radiance;
+    float pdf;
+    uint32_t key;  // sampler position the sample started at
+};
+__device__ __forceinline__ void env_geom(const DevScene& s, Sampler& rng, EnvGeom& g) {
+    g.key = rng.key();
     float a1 = rng.next();
     float a2 = rng.next();
     uint32_t texel = sample_alias(s.env_alias, s.num_env_texels, a1, a2);
@@ -174,19 +206,30 @@ __device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3
     float j2 = rng.next();
     float u = (static_cast<float>(x) + j1) / static_cast<float>(s.env_w);
     float v = (static_cast<float>(y) + j2) / static_cast<float>(s.env_h);
-    F3 wi = uv_to_dir(u, v);
-    float cs = fmaxf(0.0f, dot(n, wi));
-    if (cs <= 0.0f) return ls;
+    g.wi = uv_to_dir(u, v);
     float wu, wv;  // the equirect coordinates of wi, shared by the pdf and the radiance lookup
-    dir_to_uv(wi, wu, wv);
-    float pdf = env_pdf_uv(s, wi, wu, wv);
-    if (pdf <= 0.0f) return ls;
-    sh = make_shadow(p, n, wi, kInf);
-    ls.wi = wi;
-    ls.radiance = env_radiance_uv(s, wu, wv);
-    ls.pdf = pdf;
+    dir_to_uv(g.wi, wu, wv);
+    g.pdf = env_pdf_uv(s, g.wi, wu, wv);
+    g.radiance = env_radiance_uv(s, wu, wv);
+}
+// The rest of sampleEnvLight, at shading point p with normal n.
+__device__ __forceinline__ LightSample env_finish(F3 p, F3 n, const EnvGeom& g, ShadowRay& sh) {
+    LightSample ls;
+    ls.valid = false;
+    float cs = fmaxf(0.0f, dot(n, g.wi));
+    if (cs <= 0.0f) return ls;
+    if (g.pdf <= 0.0f) return ls;
+    sh = make_shadow(p, n, g.wi, kInf);
+    ls.wi = g.wi;
+    ls.radiance = g.radiance;
+    ls.pdf = g.pdf;
     ls.valid = true;
     return ls;
+}
+__device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3 n, Sampler& rng, ShadowRay& sh) {
+    EnvGeom g;
+    env_geom(s, rng, g);
+    return env_finish(p, n, g, sh);
 }
 
 template <bool COUNT>
@@ -381,6 +424,7 @@ struct ShadeCtx {
 // changes (the adjoint uses them; here they are empty).
 template <bool COUNT>
 struct InlineNee {
+    static constexpr bool kDrawAhead = true;
     TraceCtx& c;
     __device__ __forceinline__ explicit InlineNee(TraceCtx& ctx) : c(ctx) {}
     // The area term is added before the environment term, one at a time,
@@ -418,6 +462,28 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
             st.lo += st.tp * env * mis;
         }
         return false;
+    }
+    const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
+    const bool has_env =
+        s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 && s.env_h > 0;
+    // Light samples drawn ahead from a copy of the sampler: their gathers
+    // (alias table -> emitter or texel) overlap the surface's.  One is used
+    // only where the path's own sampler reaches the same position, so the
+    // result and the sampler's sequence are those of drawing it in place.
+    AreaGeom pre_a;
+    EnvGeom pre_e;
+    pre_a.key = pre_e.key = 0xFFFFFFFFu;
+    bool ahead_a = false, ahead_e = false;
+    if constexpr (Nee::kDrawAhead) {
+        Sampler ahead = rng;
+        if (has_area) {
+            area_geom(s, ahead, pre_a);
+            ahead_a = true;
+        }
+        if (has_env) {
+            env_geom(s, ahead, pre_e);
+            ahead_e = true;
+        }
     }
     Surface sf = make_surface(s, st.o, st.d, h);
     const pt_material& mat = s.materials[sf.mat];
@@ -469,23 +535,30 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
     x.tv = tv;
     x.entering = entering;
     if (!delta_surface) {  // :161-166
-        const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;
-        const bool has_env =
-            s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 && s.env_h > 0;
         ShadowRay sa, se;
         bool va = false, ve = false;
         F3 ca = f3(0.0f), ce = f3(0.0f);
         LightSample la, le;
         la.valid = le.valid = false;
         if (has_area) {
-            la = sample_area_nv(s, sf.p, n, rng, sa);
+            if (ahead_a && rng.key() == pre_a.key) {
+                rng.skip(4);
+                la = area_finish(sf.p, n, pre_a, sa);
+            } else {
+                la = sample_area_nv(s, sf.p, n, rng, sa);
+            }
             if (la.valid) {
                 ca = st.tp * eval_direct(la, me, wo, n);
                 va = !is_zero(ca);
             }
         }
         if (has_env) {
-            le = sample_env_nv(s, sf.p, n, rng, se);
+            if (ahead_e && rng.key() == pre_e.key) {
+                rng.skip(4);
+                le = env_finish(sf.p, n, pre_e, se);
+            } else {
+                le = sample_env_nv(s, sf.p, n, rng, se);
+            }
             if (le.valid) {
                 ce = st.tp * eval_direct(le, me, wo, n);
                 ve = !is_zero(ce);

@@ -93,6 +93,18 @@ struct Sampler {
         return (word >> 22u) ^ word;
     }
 
+    // Position in the sequence: the next dimension (Sobol) or the state (PCG).
+    __device__ __forceinline__ uint32_t key() const { return type == 1 ? dim : state; }
+
+    // Advances by k values without computing them.
+    __device__ __forceinline__ void skip(int k) {
+        if (type == 1) {
+            dim += static_cast<uint32_t>(k);
+        } else {
+            for (int i = 0; i < k; ++i) state = state * 747796405u + 2891336453u;
+        }
+    }
+
     __device__ __forceinline__ float next() {
         uint32_t u = next_u32();
         if (type == 1) return fminf(static_cast<float>(u) * kInv2Pow32, kOneMinusEps);

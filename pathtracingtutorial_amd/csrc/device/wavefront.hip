@@ -471,6 +471,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot,
 // that reach this point; each path adds at most one entry per queue) to the
 // segment the path's wave is working on; punctual rays with one atomic each.
 struct DeferNee {
+    static constexpr bool kDrawAhead = true;
     uint32_t path;
     uint32_t base;  // first slot of the segment in the shadow queues
     uint32_t *sa_count, *se_count;
@@ -886,6 +887,7 @@ __global__ void k_wfa_seed(WfState w) {
 // Visibility policy of the replay: DeferNee's queues, each entry with its
 // sensitivities; the continuation's rho goes to the path's pending record.
 struct AdjDeferNee {
+    static constexpr bool kDrawAhead = false;
     uint32_t path;
     uint32_t base;
     uint32_t *sa_count, *se_count;
