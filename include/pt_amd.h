@@ -314,8 +314,10 @@ pt_status pt_scene_get_bvh(const pt_scene* scene, int layout, float* nodes8, uin
  * {child words}; child i's box on axis a is [fma(byte i of qlo.a, S.a, P.a),
  * fma(byte i of qhi.a, S.a, P.a)] and contains the child's exact box; child
  * words are a node index, 0x80000000 | first triangle for a leaf, or
- * 0xFFFFFFFF for an unused slot.  max_stack: traversal stack entries a ray
- * can need.  Host only (the scene build runs the same collapse; it has no
+ * 0xFFFFFFFF for an unused slot.  The binary nodes that become wide nodes
+ * are the choice of least summed surface area (binary leaves are kept), so
+ * the result is deterministic for a given input.  max_stack: traversal
+ * stack entries a ray can need.  Host only (the scene build runs the same collapse; it has no
  * reference counterpart: the reference traverses the binary tree).  Pass
  * NULL wide16 to query the count. */
 pt_status pt_bvh_collapse_wide(const float* nodes8, uint32_t num_nodes, float* wide16, uint32_t* num_wide,
