@@ -468,8 +468,12 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
         s.env != nullptr && s.env_alias != nullptr && s.num_env_texels > 0 && s.env_w > 0 && s.env_h > 0;
     // Light samples drawn ahead from a copy of the sampler: their gathers
     // (alias table -> emitter or texel) overlap the surface's.  One is used
-    // only where the path's own sampler reaches the same position, so the
-    // result and the sampler's sequence are those of drawing it in place.
+    // only where the path's own sampler reaches the same position (a blended
+    // surface draws a number first), so it takes the same numbers through
+    // the same code as drawing it in place, and the sampler's sequence is
+    // unchanged.  (Kernels that must agree to the last bit, such as a primal
+    // and its replay, use the same setting: the compiler's multiply-add
+    // fusion follows the code's shape.)
     AreaGeom pre_a;
     EnvGeom pre_e;
     pre_a.key = pre_e.key = 0xFFFFFFFFu;
