@@ -471,9 +471,11 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
     // only where the path's own sampler reaches the same position (a blended
     // surface draws a number first), so it takes the same numbers through
     // the same code as drawing it in place, and the sampler's sequence is
-    // unchanged.  (Kernels that must agree to the last bit, such as a primal
-    // and its replay, use the same setting: the compiler's multiply-add
-    // fusion follows the code's shape.)
+    // unchanged.  Agreement to the last bit between two kernels still depends
+    // on the compiler's multiply-add fusion, which follows the code's shape:
+    // the megakernel and the wavefront shading kernel both draw ahead (their
+    // images are identical); the adjoint replays draw in place, which keeps
+    // them equal to the forward pass they replay.
     AreaGeom pre_a;
     EnvGeom pre_e;
     pre_a.key = pre_e.key = 0xFFFFFFFFu;
