@@ -246,6 +246,24 @@ __device__ __forceinline__ void store_path(const WfState& w, uint32_t p, const P
     R.rng = r;
 }
 
+// store_path for a bounce that started from st.lo = 0: the radiance it added
+// (an emitter hit or an escape, no more than once per bounce) goes onto the
+// path's sum, and a bounce that added none leaves the sum untouched.  Each
+// path is in a queue once, and the shadow kernels that also add to the sum
+// run in other launches.
+__device__ __forceinline__ void store_path_added(const WfState& w, uint32_t p, const PathState& st, uint4 r) {
+    PathRec& R = w.path[p];
+    R.o = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
+    R.d = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
+    uint32_t bits = static_cast<uint32_t>(st.depth) | (st.last_delta ? 0x80000000u : 0u);
+    R.tp = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
+    R.rng = r;
+    if (!is_zero(st.lo)) {
+        const float4 l = w.lo[p];
+        w.lo[p] = make_float4(l.x + st.lo.x, l.y + st.lo.y, l.z + st.lo.z, 0.0f);
+    }
+}
+
 __device__ __forceinline__ Sampler sampler_of(int type, uint4 r) {
     Sampler s;
     s.type = static_cast<uint32_t>(type);
@@ -524,7 +542,7 @@ struct ShadeItem {
     bool has = false;
     int seg = 0;
     uint32_t i = 0, end = 0, p = 0;
-    float4 o, d, tp, lo, hv;
+    float4 o, d, tp, hv;
     uint4 rng;
     __device__ __forceinline__ void fetch(const WfState& w) {
         const PathRec& R = w.path[p];
@@ -532,7 +550,6 @@ struct ShadeItem {
         d = R.d;
         tp = R.tp;
         rng = R.rng;
-        lo = w.lo[p];
         hv = w.hit[p];
     }
     __device__ __forceinline__ void unpack(PathState& st) const {
@@ -543,7 +560,7 @@ struct ShadeItem {
         const uint32_t bits = __float_as_uint(tp.w);
         st.depth = static_cast<int>(bits & 0xFFFFu);
         st.last_delta = (bits >> 31) != 0;
-        st.lo = f3(lo);
+        st.lo = f3(0.0f);  // what this bounce adds; the path's sum stays in memory
     }
 };
 
@@ -617,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
                 r = sampler_state(rng, r);
                 if (cont) st.depth += 1;
                 cont = cont && st.depth < rp.max_depth;
-                store_path(w, p, st, r);
+                store_path_added(w, p, st, r);
             }
         }
         // the continuation goes to the segment its item came from
