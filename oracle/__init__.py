@@ -59,6 +59,10 @@ _SIGS = {
     "oracle_microfacet": (None, [_FP, _FP, _FP, _FP, _FP]),
     "oracle_adjoint_fd": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _FP, C.c_int, _I32P,
                                  _I32P, _I32P, _I32P, C.c_double, C.c_int, _DP, _DP]),
+    "oracle_adjoint": (None, [_P, _P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_uint32, _FP, C.c_int, _DP, _DP, _DP,
+                              _DP]),
+    "oracle_fetch_material": (None, [_P, _P, C.c_float, C.c_float, _FP]),
+    "oracle_surface": (C.c_int, [_P, _FP, _FP, _FP]),
 }
 
 _lib = None
@@ -132,6 +136,9 @@ class OracleScene:
         self._keep += [vb, fb, mb, lb]
         self.h = L.oracle_scene_create(vb, nv, fb, nf, mb, nm, envp, ew, eh, lb, nl, bins, leaf_threshold)
         self.nf = nf
+        self._nm = nm
+        self._tex_shapes = []
+        self._env_shape = (eh, ew, 4) if ew > 0 else None
 
     @classmethod
     def from_view(cls, view, bins: int = 24, leaf_threshold: int = 4) -> "OracleScene":
@@ -149,6 +156,7 @@ class OracleScene:
 
     def add_texture(self, rgba: np.ndarray, wrap_s: int = 10497, wrap_t: int = 10497, linear: int = 0) -> int:
         rgba = np.ascontiguousarray(rgba, np.float32)
+        self._tex_shapes.append((rgba.shape[0], rgba.shape[1]))
         return lib().oracle_scene_add_texture(self.h, _f(rgba), rgba.shape[1], rgba.shape[0], wrap_s, wrap_t, linear)
 
     def bvh(self):
@@ -219,6 +227,51 @@ class OracleScene:
                                 _f(weights), len(p), _i32(k), _i32(a), _i32(b), _i32(c), h, threads,
                                 grad.ctypes.data_as(_DP), C.byref(value))
         return grad, float(value.value)
+
+    def adjoint(self, camera, spf: int, max_depth: int, sampler: int, seed: int, frame: int, weights: np.ndarray,
+                threads: int = 0):
+        """Reverse-mode FP64 gradient of sum(weights * batch image) with respect to every
+        parameter (the paths and detached estimator of adjoint_fd): dict of base_color (M, 3),
+        roughness (M,), metallic (M,), textures (list of (H, W, 4) arrays), env ((He, We, 4)
+        or None) and value."""
+        weights = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        assert weights.size == camera.width * camera.height * 3
+        nm = self._nm
+        gm = np.zeros((nm, 5), np.float64)
+        ntex = sum(h * w * 4 for (h, w) in self._tex_shapes)
+        gt = np.zeros(max(1, ntex), np.float64)
+        ge = np.zeros(self._env_shape, np.float64) if self._env_shape else None
+        value = C.c_double(0.0)
+        lib().oracle_adjoint(self.h, C.byref(camera), spf, max_depth, sampler, seed & 0xFFFFFFFF, frame,
+                             _f(weights), threads, gm.ctypes.data_as(_DP), gt.ctypes.data_as(_DP),
+                             ge.ctypes.data_as(_DP) if ge is not None else None, C.byref(value))
+        tex, off = [], 0
+        for (h, w) in self._tex_shapes:
+            tex.append(gt[off:off + h * w * 4].reshape(h, w, 4))
+            off += h * w * 4
+        return {"base_color": gm[:, :3], "roughness": gm[:, 3], "metallic": gm[:, 4], "textures": tex, "env": ge,
+                "value": float(value.value)}
+
+    def fetch_material(self, mat, u: float, v: float) -> dict:
+        """fetchMaterial of `mat` with this scene's textures at (u, v), in float."""
+        out = np.zeros(20, np.float32)
+        lib().oracle_fetch_material(self.h, C.byref(mat), float(u), float(v), _f(out))
+        names = ["base", "metallic", "roughness", "spec_trans", "ior", "clearcoat", "clearcoat_roughness", "sheen",
+                 "sheen_tint", "subsurface", "anisotropic", "sheen_color", "emissive", "specular_factor"]
+        vals = [out[0:3], out[3], out[4], out[5], out[6], out[7], out[8], out[9], out[10], out[11], out[12],
+                out[13:16], out[16:19], out[19]]
+        return dict(zip(names, vals))
+
+    def surface(self, o, d):
+        """The integrator's shading point at the closest hit of (o, d): dict of t, P, sn (interpolated
+        normal), gn (on the ray's side), N (after the normal map), entering, face; None on a miss."""
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.zeros(16, np.float32)
+        if not lib().oracle_surface(self.h, _f(o), _f(d), _f(out)):
+            return None
+        return {"t": float(out[1]), "P": out[2:5].copy(), "sn": out[5:8].copy(), "gn": out[8:11].copy(),
+                "N": out[11:14].copy(), "entering": bool(out[14]), "face": int(out[15])}
 
     def __del__(self):
         try:

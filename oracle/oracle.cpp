@@ -15,12 +15,23 @@
 #include <atomic>
 #include <cfloat>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <thread>
 #include <vector>
 
 namespace orc {
+
+// The math functions the integrator's templates call: std's for float and
+// double; the reverse-mode type of the host adjoint (below) overloads them.
+template <class R> inline R msqrt(R x) { return std::sqrt(x); }
+template <class R> inline R mpow(R a, R b) { return std::pow(a, b); }
+template <class R> inline R mfabs(R x) { return std::fabs(x); }
+
+// A parameter read (material factor, texel): its value; the host adjoint's
+// reverse-mode type turns reads of registered parameter arrays into leaves.
+template <class R> inline R paramRead(const float* p) { return R(*p); }
 
 // ------------------------------------------------------------- math/vecmath.h
 template <class R>
@@ -46,7 +57,7 @@ template <class R> R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.
 template <class R> V3<R> cross(const V3<R>& a, const V3<R>& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-template <class R> R length(const V3<R>& a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <class R> R length(const V3<R>& a) { return msqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 template <class R> V3<R> normalize(const V3<R>& a) {  // vecmath.h:96
     R l = length(a);
     return l > 0 ? a / l : V3<R>();
@@ -70,7 +81,7 @@ struct Frame {
     V3<R> t, b, n;
     explicit Frame(const V3<R>& nn) : n(nn) {
         V3<R> helper(1, 0, 0);
-        if (std::fabs(nn.x) > R(0.999)) helper = V3<R>(0, 0, 1);
+        if (mfabs(nn.x) > R(0.999)) helper = V3<R>(0, 0, 1);
         t = normalize(cross(nn, helper));
         b = normalize(cross(nn, t));
     }
@@ -80,9 +91,9 @@ struct Frame {
 
 // ------------------------------------------------------------- math/sampling.h
 template <class R> V3<R> sampleCosineHemisphere(R u1, R u2) {  // :18-25
-    R r = std::sqrt(u1);
+    R r = msqrt(u1);
     R phi = K<R>::TwoPi * u2;
-    return V3<R>(r * std::cos(phi), r * std::sin(phi), std::sqrt(fmx(R(0), R(1) - u1)));
+    return V3<R>(r * std::cos(phi), r * std::sin(phi), msqrt(fmx(R(0), R(1) - u1)));
 }
 template <class R> R powerHeuristic(R a, R b) {  // :34-39
     R a2 = a * a, b2 = b * b, s = a2 + b2;
@@ -515,7 +526,8 @@ void tex2DT(const float* rgba, int w, int h, int ws, int wt, float u, float v, T
     const float* t11 = rgba + 4 * (static_cast<size_t>(yb) * w + xb);
     const float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
     for (int c = 0; c < 4; ++c)
-        out[c] = T(w00) * T(t00[c]) + T(w10) * T(t10[c]) + T(w01) * T(t01[c]) + T(w11) * T(t11[c]);
+        out[c] = T(w00) * paramRead<T>(t00 + c) + T(w10) * paramRead<T>(t10 + c) + T(w01) * paramRead<T>(t01 + c) +
+                 T(w11) * paramRead<T>(t11 + c);
 }
 void tex2D(const float* rgba, int w, int h, int ws, int wt, float u, float v, float out[4]) {
     tex2DT<float>(rgba, w, h, ws, wt, u, v, out);
@@ -534,12 +546,12 @@ template <class R> R D_GGX_Aniso(R noh, R hx, R hy, R ax, R ay) {  // :17-22
 }
 template <class R> R G1_Smith(R nov, R alpha) {  // :25-28
     R a2 = alpha * alpha;
-    return R(2) * nov / (nov + std::sqrt(a2 + (R(1) - a2) * nov * nov) + R(1e-7));
+    return R(2) * nov / (nov + msqrt(a2 + (R(1) - a2) * nov * nov) + R(1e-7));
 }
 template <class R> R G2_Smith(R nol, R nov, R alpha) {  // :31-36
     R a2 = alpha * alpha;
-    R gv = nol * std::sqrt(a2 + (R(1) - a2) * nov * nov);
-    R gl = nov * std::sqrt(a2 + (R(1) - a2) * nol * nol);
+    R gv = nol * msqrt(a2 + (R(1) - a2) * nov * nov);
+    R gl = nov * msqrt(a2 + (R(1) - a2) * nol * nol);
     return R(2) * nol * nov / (gv + gl + R(1e-7));
 }
 template <class R> R F_SchlickS(R c, R f0) {  // :39-43
@@ -556,8 +568,8 @@ template <class R> V3<R> F_SchlickV(R c, const V3<R>& f0) {  // :46-51
 template <class R> R F_Dielectric(R ci0, R eta) {  // :54-62
     R s2t = eta * eta * (R(1) - ci0 * ci0);
     if (s2t >= R(1)) return R(1);
-    R ct = std::sqrt(fmx(R(0), R(1) - s2t));
-    R ci = std::fabs(ci0);
+    R ct = msqrt(fmx(R(0), R(1) - s2t));
+    R ci = mfabs(ci0);
     R rs = (ci - eta * ct) / (ci + eta * ct + R(1e-7));
     R rp = (eta * ci - ct) / (eta * ci + ct + R(1e-7));
     return R(0.5) * (rs * rs + rp * rp);
@@ -567,17 +579,17 @@ template <class R> V3<R> sampleVNDF(const V3<R>& wo, R ax, R ay, R u1, R u2) {  
     if (wh.z < 0) wh = -wh;
     V3<R> T1 = (wh.z < R(0.9999)) ? normalize(cross(V3<R>(0, 0, 1), wh)) : V3<R>(1, 0, 0);
     V3<R> T2 = cross(wh, T1);
-    R r = std::sqrt(u1), phi = K<R>::TwoPi * u2;
+    R r = msqrt(u1), phi = K<R>::TwoPi * u2;
     R t1 = r * std::cos(phi), t2 = r * std::sin(phi);
     R s = R(0.5) * (R(1) + wh.z);
-    t2 = (R(1) - s) * std::sqrt(fmx(R(0), R(1) - t1 * t1)) + s * t2;
-    V3<R> nh = t1 * T1 + t2 * T2 + std::sqrt(fmx(R(0), R(1) - t1 * t1 - t2 * t2)) * wh;
+    t2 = (R(1) - s) * msqrt(fmx(R(0), R(1) - t1 * t1)) + s * t2;
+    V3<R> nh = t1 * T1 + t2 * T2 + msqrt(fmx(R(0), R(1) - t1 * t1 - t2 * t2)) * wh;
     return normalize(V3<R>(ax * nh.x, ay * nh.y, fmx(R(0), nh.z)));
 }
 template <class R> R pdfVNDF(const V3<R>& wo, const V3<R>& h, R ax, R ay) {  // :95-102
     R nov = fmx(wo.z, R(1e-5)), noh = fmx(h.z, R(0)), voh = fmx(dot(wo, h), R(0));
     R d = D_GGX_Aniso(noh, h.x, h.y, ax, ay);
-    R g1 = G1_Smith(nov, std::sqrt(ax * ay));
+    R g1 = G1_Smith(nov, msqrt(ax * ay));
     return d * g1 * voh / (nov + R(1e-7));
 }
 template <class R> bool refractDir(const V3<R>& wi, const V3<R>& n, R eta, V3<R>& wt) {  // :105-113
@@ -585,7 +597,7 @@ template <class R> bool refractDir(const V3<R>& wi, const V3<R>& n, R eta, V3<R>
     R s2i = fmx(R(0), R(1) - ci * ci);
     R s2t = eta * eta * s2i;
     if (s2t >= R(1)) return false;
-    R ct = std::sqrt(R(1) - s2t);
+    R ct = msqrt(R(1) - s2t);
     wt = eta * (-wi) + (eta * ci - ct) * n;
     return true;
 }
@@ -602,14 +614,14 @@ struct MatEval {  // :20-36
 
 template <class R> V3<R> powColor(const R c[4], bool linear) {
     if (linear) return V3<R>(c[0], c[1], c[2]);
-    return V3<R>(std::pow(c[0], R(2.2)), std::pow(c[1], R(2.2)), std::pow(c[2], R(2.2)));
+    return V3<R>(mpow(c[0], R(2.2)), mpow(c[1], R(2.2)), mpow(c[2], R(2.2)));
 }
 
 template <class R>
 MatEval<R> fetchMaterial(const oracle_scene* s, const pt_material& m, float u, float v) {  // :53-122
     MatEval<R> e;
-    e.base = v3of<R>(m.base_color);
-    e.metallic = R(m.metallic); e.roughness = R(m.roughness); e.specTrans = R(m.spec_trans); e.ior = R(m.ior);
+    e.base = V3<R>(paramRead<R>(&m.base_color.x), paramRead<R>(&m.base_color.y), paramRead<R>(&m.base_color.z));
+    e.metallic = paramRead<R>(&m.metallic); e.roughness = paramRead<R>(&m.roughness); e.specTrans = R(m.spec_trans); e.ior = R(m.ior);
     e.clearcoat = R(m.clearcoat); e.clearcoatRoughness = R(m.clearcoat_roughness);
     e.sheen = R(m.sheen); e.sheenTint = R(m.sheen_tint); e.subsurface = R(m.subsurface);
     e.anisotropic = R(m.anisotropic); e.specularTint = R(m.specular_tint); e.specularFactor = R(m.specular_factor);
@@ -673,7 +685,7 @@ V3<R> normalMap(const oracle_scene* s, const pt_material& m, const V3<R>& N, con
 }
 
 template <class R> void computeAlpha(R rough, R aniso, R& ax, R& ay) {  // :126-131
-    R aspect = std::sqrt(R(1) - R(0.9) * aniso);
+    R aspect = msqrt(R(1) - R(0.9) * aniso);
     R a = rough * rough;
     ax = fmx(a / aspect, R(0.001));
     ay = fmx(a * aspect, R(0.001));
@@ -681,13 +693,13 @@ template <class R> void computeAlpha(R rough, R aniso, R& ax, R& ay) {  // :126-
 template <class R> R lum(const V3<R>& c) { return R(0.2126) * c.x + R(0.7152) * c.y + R(0.0722) * c.z; }
 template <class R> V3<R> evalDisneyDiffuse(const MatEval<R>& m, R nol, R nov, R hol) {  // :135-159
     R fd90 = R(0.5) + R(2) * hol * hol * m.roughness;
-    R fl = R(1) + (fd90 - R(1)) * std::pow(R(1) - nol, R(5));
-    R fv = R(1) + (fd90 - R(1)) * std::pow(R(1) - nov, R(5));
+    R fl = R(1) + (fd90 - R(1)) * mpow(R(1) - nol, R(5));
+    R fv = R(1) + (fd90 - R(1)) * mpow(R(1) - nov, R(5));
     V3<R> d = m.base * (K<R>::InvPi * fl * fv);
     if (m.subsurface > R(0)) {
         R f90 = hol * hol * m.roughness;
-        R sl = R(1) + (f90 - R(1)) * std::pow(R(1) - nol, R(5));
-        R sv = R(1) + (f90 - R(1)) * std::pow(R(1) - nov, R(5));
+        R sl = R(1) + (f90 - R(1)) * mpow(R(1) - nol, R(5));
+        R sv = R(1) + (f90 - R(1)) * mpow(R(1) - nov, R(5));
         R ss = R(1.25) * (sl * sv * (R(1) / (nol + nov + R(1e-5)) - R(0.5)) + R(0.5));
         V3<R> sd = m.base * (K<R>::InvPi * ss);
         d = (R(1) - m.subsurface) * d + m.subsurface * sd;
@@ -700,7 +712,7 @@ template <class R> V3<R> evalSheen(const MatEval<R>& m, R hol) {  // :163-171
     V3<R> tint = l > R(0) ? m.base / l : V3<R>(1);
     V3<R> bs = (R(1) - m.sheenTint) * V3<R>(1) + m.sheenTint * tint;
     V3<R> fc = bs * m.sheenColor;
-    R fh = std::pow(R(1) - hol, R(5));
+    R fh = mpow(R(1) - hol, R(5));
     return m.sheen * fc * fh;
 }
 template <class R> V3<R> computeF0(const MatEval<R>& m) {  // :175-183
@@ -734,26 +746,26 @@ BsdfEval<R> evalDisneyBsdf(const MatEval<R>& m, const V3<R>& wo, const V3<R>& wi
     bool refl = nol > R(0);
     if (!refl && m.specTrans <= R(0)) return r;
     if (!refl) {
-        R anl = std::fabs(nol), anv = std::fabs(nov);
+        R anl = mfabs(nol), anv = mfabs(nov);
         if (anl < R(1e-5) || anv < R(1e-5)) return r;
         R eta = entering ? (R(1) / m.ior) : m.ior;
         V3<R> ht = -normalize(wo * eta + wi);
-        R hov = std::fabs(dot(ht, wo));
+        R hov = mfabs(dot(ht, wo));
         R ax, ay;
         computeAlpha(m.roughness, m.anisotropic, ax, ay);
         Frame<R> fr(N);
         V3<R> hl = fr.toLocal(ht);
-        R D = D_GGX_Aniso(std::fabs(hl.z), hl.x, hl.y, ax, ay);
-        R G = G2_Smith(anl, anv, std::sqrt(ax * ay));
+        R D = D_GGX_Aniso(mfabs(hl.z), hl.x, hl.y, ax, ay);
+        R G = G2_Smith(anl, anv, msqrt(ax * ay));
         R F = F_Dielectric(hov, eta);
         R sd = dot(wo, ht) * eta + dot(wi, ht);
-        R factor = std::fabs(dot(wo, ht)) * std::fabs(dot(wi, ht)) / (anv * anl * sd * sd + R(1e-7));
+        R factor = mfabs(dot(wo, ht)) * mfabs(dot(wi, ht)) / (anv * anl * sd * sd + R(1e-7));
         R tw = (R(1) - m.metallic) * m.specTrans;
         V3<R> ft = m.base * tw * D * G * (R(1) - F) * factor;
         R wd, ws, wc, wt, wsh;
         lobeWeights(m, wd, ws, wc, wt, wsh);
-        R jac = eta * eta * std::fabs(dot(wi, ht)) / (sd * sd + R(1e-7));
-        R tp = D * G1_Smith(anv, std::sqrt(ax * ay)) * hov / (anv + R(1e-7)) * jac;
+        R jac = eta * eta * mfabs(dot(wi, ht)) / (sd * sd + R(1e-7));
+        R tp = D * G1_Smith(anv, msqrt(ax * ay)) * hov / (anv + R(1e-7)) * jac;
         r.f = ft;
         r.pdf = wt * tp;
         return r;
@@ -780,7 +792,7 @@ BsdfEval<R> evalDisneyBsdf(const MatEval<R>& m, const V3<R>& wo, const V3<R>& wi
     }
     {
         R D = D_GGX_Aniso(noh, hl.x, hl.y, ax, ay);
-        R G = G2_Smith(nol, nov, std::sqrt(ax * ay));
+        R G = G2_Smith(nol, nov, msqrt(ax * ay));
         V3<R> F = F_SchlickV(hol, computeF0(m));
         f += D * G * F / (R(4) * nol * nov + R(1e-7));
         pdf += ws * (pdfVNDF(fr.toLocal(wo), hl, ax, ay) / (R(4) * hol + R(1e-7)));
@@ -882,7 +894,7 @@ R triHit(const V3<R>& o, const V3<R>& d, const pt_vertex& a, const pt_vertex& b,
     V3<R> e1 = v3of<R>(b.position) - p0, e2 = v3of<R>(c.position) - p0;
     V3<R> pv = cross(d, e2);
     R det = dot(e1, pv);
-    if (std::fabs(det) < R(FLT_EPSILON)) return R(-1);
+    if (mfabs(det) < R(FLT_EPSILON)) return R(-1);
     R inv = R(1) / det;
     V3<R> tv = o - p0;
     u = dot(tv, pv) * inv;
@@ -998,7 +1010,7 @@ template <class R> R envPdf(const oracle_scene* s, const V3<R>& d) {  // light_p
     dirToUv(d, u, v);
     int x = static_cast<int>(clampr(u * R(s->env.w), R(0), R(s->env.w - 1)));
     int y = static_cast<int>(clampr(v * R(s->env.h), R(0), R(s->env.h - 1)));
-    R st = std::sqrt(fmx(R(0), R(1) - d.y * d.y));
+    R st = msqrt(fmx(R(0), R(1) - d.y * d.y));
     if (st <= R(1e-5)) return R(0);
     return R(s->envPmf[static_cast<size_t>(y) * s->env.w + x]) * R(s->env.w * s->env.h) / (K<R>::TwoPi * K<R>::Pi * st);
 }
@@ -1011,7 +1023,7 @@ template <class R> R emissivePdf(const oracle_scene* s, int prim, R dist, const 
     V3<R> c = cross(p1 - p0, p2 - p0);
     R area = R(0.5) * length(c);
     if (area <= R(0)) return R(0);
-    R cl = std::fabs(dot(normalize(c), -d));
+    R cl = mfabs(dot(normalize(c), -d));
     if (cl <= R(1e-5)) return R(0);
     return pmf * dist * dist / (area * cl);
 }
@@ -1050,7 +1062,7 @@ LightSample<R> sampleAreaLight(const oracle_scene* s, const V3<R>& p, const V3<R
     const pt_vertex &va = s->v[f.v0], &vb = s->v[f.v1], &vc = s->v[f.v2];
     R t1 = R(rng.get1D());
     R t2 = R(rng.get1D());
-    R su0 = std::sqrt(t1);  // light_sampler.h:58-71
+    R su0 = msqrt(t1);  // light_sampler.h:58-71
     R b0 = R(1) - su0, b1 = t2 * su0, b2 = R(1) - b0 - b1;
     V3<R> lp = v3of<R>(va.position) * b0 + v3of<R>(vb.position) * b1 + v3of<R>(vc.position) * b2;
     V3<R> ln = normalize(v3of<R>(va.normal) * b0 + v3of<R>(vb.normal) * b1 + v3of<R>(vc.normal) * b2);
@@ -1059,12 +1071,12 @@ LightSample<R> sampleAreaLight(const oracle_scene* s, const V3<R>& p, const V3<R
     V3<R> tl = lp - p;
     R d2 = dot(tl, tl);
     if (d2 <= R(1e-8)) return ls;
-    R dist = std::sqrt(d2);
+    R dist = msqrt(d2);
     V3<R> wi = tl / dist;
     if (fmx(R(0), dot(n, wi)) <= R(0)) return ls;
     R area = R(0.5) * length(cross(e1, e2));
     if (area <= R(0)) return ls;
-    R cl = std::fabs(dot(ln, -wi));
+    R cl = mfabs(dot(ln, -wi));
     if (cl <= R(1e-5)) return ls;
     R pdf = (R(ref.pmf) / area) * d2 / cl;
     if (pdf <= R(0)) return ls;
@@ -1122,7 +1134,7 @@ V3<R> punctualLights(const oracle_scene* s, const V3<R>& p, const MatEval<R>& m,
             V3<R> tl = v3of<R>(l.position) - p;
             R d2 = dot(tl, tl);
             if (d2 < R(1e-8)) continue;
-            dist = std::sqrt(d2);
+            dist = msqrt(d2);
             wi = tl / dist;
             R att = dist >= R(l.range) ? R(0) : R(1) / (dist * dist + R(1e-7));  // punctual_light.h:36-39
             if (l.type == 2) {                                                  // :28-34
@@ -1149,6 +1161,51 @@ V3<R> punctualLights(const oracle_scene* s, const V3<R>& p, const MatEval<R>& m,
 }
 
 // ------------------------------------------------------- integrators/path.cu
+// The shading point of a closest hit (path.cu:86-103): position, the
+// interpolated normal and uv (triangle.h:40-49), the entering test with the
+// double-sided flip, the uv transform (shading_utils.cuh:17-27) and the
+// normal map.  The reference's tangent flip for double-sided back faces
+// (path.cu:101-102) tests `!entering && doubleSided` after the flip above
+// has set entering, so it never fires and the tangent is used as
+// interpolated.
+template <class R>
+struct SurfacePt {
+    V3<R> P, sn, gn, N, wo;
+    R tu, tv;      // interpolated uv (the alpha test reads them untransformed)
+    float mu, mv;  // after the uv transform
+    bool entering;
+    const pt_material* mat;
+};
+template <class R>
+SurfacePt<R> surfaceAt(const oracle_scene* s, const Hit<R>& h, const V3<R>& o, const V3<R>& d) {
+    SurfacePt<R> sp;
+    const pt_face& f = s->f[h.prim];
+    const pt_vertex &va = s->v[f.v0], &vb = s->v[f.v1], &vc = s->v[f.v2];
+    R w = R(1) - h.u - h.v;
+    sp.P = o + h.t * d;
+    sp.sn = normalize(v3of<R>(va.normal) * w + v3of<R>(vb.normal) * h.u + v3of<R>(vc.normal) * h.v);
+    sp.tu = R(va.u) * w + R(vb.u) * h.u + R(vc.u) * h.v;
+    sp.tv = R(va.v) * w + R(vb.v) * h.u + R(vc.v) * h.v;
+    const pt_material& mat = s->m[f.material_id];
+    sp.mat = &mat;
+    sp.entering = dot(sp.sn, d) < R(0);
+    if (mat.double_sided && !sp.entering) { sp.gn = -sp.sn; sp.entering = true; }
+    else sp.gn = sp.entering ? sp.sn : -sp.sn;
+    sp.wo = -d;
+    float mu = static_cast<float>(sp.tu), mv = static_cast<float>(sp.tv);
+    if (!(mat.uv_rotation == 0.0f && mat.uv_scale_x == 1.0f && mat.uv_scale_y == 1.0f && mat.uv_offset_x == 0.0f &&
+          mat.uv_offset_y == 0.0f)) {  // shading_utils.cuh:17-27
+        float cr = std::cos(mat.uv_rotation), sr = std::sin(mat.uv_rotation);
+        float ru = cr * mu + sr * mv, rv = -sr * mu + cr * mv;
+        mu = ru * mat.uv_scale_x + mat.uv_offset_x;
+        mv = rv * mat.uv_scale_y + mat.uv_offset_y;
+    }
+    sp.mu = mu;
+    sp.mv = mv;
+    sp.N = normalMap<R>(s, mat, sp.gn, va, vb, vc, h.u, h.v, mu, mv);
+    return sp;
+}
+
 template <class R>
 V3<R> pathTrace(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, int maxDepth, int x, int y,
                 Sampler& rng, Ctx& c) {
@@ -1167,7 +1224,7 @@ V3<R> pathTrace(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, 
         V3<R> focal = pos + dir * (R(cf.focal_distance) / dot(dir, fwd));
         R l1 = R(rng.get1D());
         R l2 = R(rng.get1D());
-        R r = std::sqrt(l1) * R(cf.aperture);
+        R r = msqrt(l1) * R(cf.aperture);
         R th = K<R>::TwoPi * l2;
         pos = pos + (rgt * (r * std::cos(th)) + cup * (r * std::sin(th)));
         dir = normalize(focal - pos);
@@ -1189,29 +1246,12 @@ V3<R> pathTrace(const oracle_scene* s, const pt_camera_frame& cf, int W, int H, 
             }
             break;
         }
-        const pt_face& f = s->f[h.prim];
-        const pt_vertex &va = s->v[f.v0], &vb = s->v[f.v1], &vc = s->v[f.v2];
-        R w = R(1) - h.u - h.v;
-        V3<R> P = o + h.t * d;
-        V3<R> sn = normalize(v3of<R>(va.normal) * w + v3of<R>(vb.normal) * h.u + v3of<R>(vc.normal) * h.v);
-        R tu = R(va.u) * w + R(vb.u) * h.u + R(vc.u) * h.v;
-        R tv = R(va.v) * w + R(vb.v) * h.u + R(vc.v) * h.v;
-        const pt_material& mat = s->m[f.material_id];
-        bool entering = dot(sn, d) < R(0);
-        V3<R> gn;
-        if (mat.double_sided && !entering) { gn = -sn; entering = true; }
-        else gn = entering ? sn : -sn;
-        V3<R> wo = -d;
-        float mu = static_cast<float>(tu), mv = static_cast<float>(tv);
-        if (!(mat.uv_rotation == 0.0f && mat.uv_scale_x == 1.0f && mat.uv_scale_y == 1.0f && mat.uv_offset_x == 0.0f &&
-              mat.uv_offset_y == 0.0f)) {  // shading_utils.cuh:17-27
-            float cr = std::cos(mat.uv_rotation), sr = std::sin(mat.uv_rotation);
-            float ru = cr * mu + sr * mv, rv = -sr * mu + cr * mv;
-            mu = ru * mat.uv_scale_x + mat.uv_offset_x;
-            mv = rv * mat.uv_scale_y + mat.uv_offset_y;
-        }
-        V3<R> N = normalMap<R>(s, mat, gn, va, vb, vc, h.u, h.v, mu, mv);
-        MatEval<R> me = fetchMaterial<R>(s, mat, mu, mv);
+        const SurfacePt<R> sp = surfaceAt(s, h, o, d);
+        const pt_material& mat = *sp.mat;
+        const V3<R>&P = sp.P, &gn = sp.gn, &wo = sp.wo, &N = sp.N;
+        const R tu = sp.tu, tv = sp.tv;
+        const bool entering = sp.entering;
+        MatEval<R> me = fetchMaterial<R>(s, mat, sp.mu, sp.mv);
         if (isEmissive(mat)) {
             R mis = R(1);
             if (depth > 0 && !lastDelta) mis = powerHeuristic(lastPdf, emissivePdf(s, h.prim, h.t, d));
@@ -1328,7 +1368,7 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
         V3<R> focal = pos + dir * (R(cf.focal_distance) / dot(dir, fwd));
         R l1 = R(rng.get1D());
         R l2 = R(rng.get1D());
-        R r = std::sqrt(l1) * R(cf.aperture);
+        R r = msqrt(l1) * R(cf.aperture);
         R th = K<R>::TwoPi * l2;
         pos = pos + (rgt * (r * std::cos(th)) + cup * (r * std::sin(th)));
         dir = normalize(focal - pos);
@@ -1354,28 +1394,12 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
             }
             break;
         }
-        const pt_face& f = s->f[h.prim];
-        const pt_vertex &va = s->v[f.v0], &vb = s->v[f.v1], &vc = s->v[f.v2];
-        R w = R(1) - h.u - h.v;
-        V3<R> P = o + h.t * d;
-        V3<R> sn = normalize(v3of<R>(va.normal) * w + v3of<R>(vb.normal) * h.u + v3of<R>(vc.normal) * h.v);
-        R tu = R(va.u) * w + R(vb.u) * h.u + R(vc.u) * h.v;
-        R tvv = R(va.v) * w + R(vb.v) * h.u + R(vc.v) * h.v;
-        const pt_material& mat = s->m[f.material_id];
-        bool entering = dot(sn, d) < R(0);
-        V3<R> gn;
-        if (mat.double_sided && !entering) { gn = -sn; entering = true; }
-        else gn = entering ? sn : -sn;
-        V3<R> wo = -d;
-        float mu = tu, mv = tvv;
-        if (!(mat.uv_rotation == 0.0f && mat.uv_scale_x == 1.0f && mat.uv_scale_y == 1.0f && mat.uv_offset_x == 0.0f &&
-              mat.uv_offset_y == 0.0f)) {
-            float cr = std::cos(mat.uv_rotation), sr = std::sin(mat.uv_rotation);
-            float ru = cr * mu + sr * mv, rv = -sr * mu + cr * mv;
-            mu = ru * mat.uv_scale_x + mat.uv_offset_x;
-            mv = rv * mat.uv_scale_y + mat.uv_offset_y;
-        }
-        V3<R> N = normalMap<R>(s, mat, gn, va, vb, vc, h.u, h.v, mu, mv);
+        const SurfacePt<R> sp = surfaceAt(s, h, o, d);
+        const pt_material& mat = *sp.mat;
+        const V3<R>&P = sp.P, &gn = sp.gn, &wo = sp.wo, &N = sp.N;
+        const R tu = sp.tu, tvv = sp.tv;
+        const bool entering = sp.entering;
+        const float mu = sp.mu, mv = sp.mv;
         MatEval<R> me = fetchMaterial<R>(s, mat, mu, mv);
         if (isEmissive(mat)) {
             R mis = R(1);
@@ -1405,7 +1429,7 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
             }
         }
         tv.kind = 2;
-        tv.mat = static_cast<int>(f.material_id);
+        tv.mat = static_cast<int>(sp.mat - s->m.data());
         tv.mu = mu;
         tv.mv = mv;
         tv.wo = wo;
@@ -1432,7 +1456,7 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
                     V3<R> tl = v3of<R>(l.position) - P;
                     R d2 = dot(tl, tl);
                     if (d2 < R(1e-8)) continue;
-                    dist = std::sqrt(d2);
+                    dist = msqrt(d2);
                     wi = tl / dist;
                     R att = dist >= R(l.range) ? R(0) : R(1) / (dist * dist + R(1e-7));
                     if (l.type == 2) {
@@ -1496,45 +1520,163 @@ std::vector<TapeVertex> pathTape(const oracle_scene* s, const pt_camera_frame& c
     return tape;
 }
 
-// The detached estimator of a tape under the parameters of scene sp (no clamp).
-V3<double> tapeEval(const oracle_scene* sp, const std::vector<TapeVertex>& tape) {
-    V3<double> L(0), T(1);
+// ------------------------------------------ reverse-mode type (host adjoint)
+// AD is a double with an index into a per-thread list of nodes (-1: a
+// constant).  Every operation on a non-constant records its operands and
+// local partial derivatives; reads of registered parameters (material base
+// colour, roughness, metallic; texels of textures and of the environment)
+// become leaves.  One backward sweep over the list then gives the derivative
+// of a path's radiance with respect to every parameter the path read.  The
+// integrator's templates run on AD unchanged (their math calls go through
+// msqrt / mpow / mfabs / fmn / fmx, overloaded here); a min or max passes the
+// derivative of the operand it returns, the first one on a tie.
+struct ADNode {
+    int32_t a, b;
+    double da, db;
+};
+struct ADTape {
+    std::vector<ADNode> nodes;
+    std::vector<std::pair<int32_t, int64_t>> leaves;  // node, parameter id
+    // parameter arrays: materials (5 ids each), then texel channels of every
+    // texture in order, then of the environment
+    const pt_material* mats = nullptr;
+    size_t nmat = 0;
+    struct Block { const float* p; size_t n; int64_t base; };
+    std::vector<Block> blocks;
+    int64_t nparams = 0;
+    int64_t paramId(const float* p) const {
+        const char* c = reinterpret_cast<const char*>(p);
+        const char* m0 = reinterpret_cast<const char*>(mats);
+        if (mats && c >= m0 && c < m0 + nmat * sizeof(pt_material)) {
+            const size_t off = static_cast<size_t>(c - m0), i = off / sizeof(pt_material);
+            const size_t f = off % sizeof(pt_material);
+            int k = -1;
+            if (f == offsetof(pt_material, base_color) + 0) k = 0;
+            else if (f == offsetof(pt_material, base_color) + 4) k = 1;
+            else if (f == offsetof(pt_material, base_color) + 8) k = 2;
+            else if (f == offsetof(pt_material, roughness)) k = 3;
+            else if (f == offsetof(pt_material, metallic)) k = 4;
+            return k < 0 ? -1 : static_cast<int64_t>(5 * i + k);
+        }
+        for (const Block& b : blocks)
+            if (p >= b.p && p < b.p + b.n) return b.base + (p - b.p);
+        return -1;
+    }
+};
+thread_local ADTape* tlTape = nullptr;
+
+struct AD {
+    double v;
+    int32_t i;
+    constexpr AD() : v(0.0), i(-1) {}
+    constexpr AD(double x) : v(x), i(-1) {}
+    AD(double x, int32_t idx) : v(x), i(idx) {}
+};
+inline AD adOp(double v, int32_t a, double da, int32_t b, double db) {
+    if (a < 0 && b < 0) return AD(v);
+    tlTape->nodes.push_back({a, b, da, db});
+    return AD(v, static_cast<int32_t>(tlTape->nodes.size() - 1));
+}
+inline AD operator+(AD a, AD b) { return adOp(a.v + b.v, a.i, 1.0, b.i, 1.0); }
+inline AD operator-(AD a, AD b) { return adOp(a.v - b.v, a.i, 1.0, b.i, -1.0); }
+inline AD operator*(AD a, AD b) { return adOp(a.v * b.v, a.i, b.v, b.i, a.v); }
+inline AD operator/(AD a, AD b) {
+    const double q = a.v / b.v;
+    return adOp(q, a.i, 1.0 / b.v, b.i, -q / b.v);
+}
+inline AD operator-(AD a) { return adOp(-a.v, a.i, -1.0, -1, 0.0); }
+inline AD& operator+=(AD& a, AD b) { return a = a + b; }
+inline AD& operator-=(AD& a, AD b) { return a = a - b; }
+inline AD& operator*=(AD& a, AD b) { return a = a * b; }
+inline AD& operator/=(AD& a, AD b) { return a = a / b; }
+inline bool operator<(AD a, AD b) { return a.v < b.v; }
+inline bool operator>(AD a, AD b) { return a.v > b.v; }
+inline bool operator<=(AD a, AD b) { return a.v <= b.v; }
+inline bool operator>=(AD a, AD b) { return a.v >= b.v; }
+inline bool operator==(AD a, AD b) { return a.v == b.v; }
+inline bool operator!=(AD a, AD b) { return a.v != b.v; }
+inline AD msqrt(AD a) {
+    const double r = std::sqrt(a.v);
+    return adOp(r, a.i, r > 0.0 ? 0.5 / r : 0.0, -1, 0.0);
+}
+inline AD mpow(AD a, AD b) {
+    const double r = std::pow(a.v, b.v);
+    const double da = a.v != 0.0 ? b.v * std::pow(a.v, b.v - 1.0) : 0.0;
+    const double db = a.v > 0.0 ? r * std::log(a.v) : 0.0;
+    return adOp(r, a.i, da, b.i, db);
+}
+inline AD mfabs(AD a) { return a.v < 0.0 ? -a : a; }
+inline AD fmn(AD a, AD b) { return (b.v < a.v || std::isnan(a.v)) ? b : a; }
+inline AD fmx(AD a, AD b) { return (b.v > a.v || std::isnan(a.v)) ? b : a; }
+template <>
+inline AD paramRead<AD>(const float* p) {
+    const int64_t id = tlTape ? tlTape->paramId(p) : -1;
+    if (id < 0) return AD(*p);
+    tlTape->nodes.push_back({-1, -1, 0.0, 0.0});
+    const int32_t n = static_cast<int32_t>(tlTape->nodes.size() - 1);
+    tlTape->leaves.push_back({n, id});
+    return AD(*p, n);
+}
+
+template <class R> V3<R> lift(const V3<double>& a) { return V3<R>(R(a.x), R(a.y), R(a.z)); }
+
+// sampleEnvironmentRadiance (envmap.h:34-45, x 2 pi) at a fixed direction,
+// the texels in R.
+template <class R> V3<R> envRadianceAt(const oracle_scene* s, const V3<double>& d) {
+    if (s->env.w <= 0) return V3<R>(0);
+    double u, v;
+    dirToUv(d, u, v);
+    R c[4];
+    tex2DT<R>(s->env.rgba.data(), s->env.w, s->env.h, s->env.wrapS, s->env.wrapT, static_cast<float>(u),
+              static_cast<float>(v), c);
+    return V3<R>(c[0], c[1], c[2]) * K<R>::TwoPi;
+}
+
+// The detached estimator of a tape under the parameters of scene sp (no
+// clamp): in double for the finite differences, in AD for the adjoint.
+template <class R>
+V3<R> tapeEvalT(const oracle_scene* sp, const std::vector<TapeVertex>& tape) {
+    V3<R> L(0), T(1);
     for (const TapeVertex& v : tape) {
         if (v.kind == 0) {
-            L += T * envRadiance(sp, d3(v.d)) * v.mis;
+            L += T * envRadianceAt<R>(sp, d3(v.d)) * R(v.mis);
             break;
         }
         if (v.kind == 1) {
-            L += T * v.Le * v.mis;
+            L += T * lift<R>(v.Le) * R(v.mis);
             break;
         }
         if (v.kind == 3) continue;
-        MatEval<double> me = fetchMaterial<double>(sp, sp->m[v.mat], v.mu, v.mv);
-        const V3<double> wo = d3(v.wo), N = d3(v.N);
+        MatEval<R> me = fetchMaterial<R>(sp, sp->m[v.mat], v.mu, v.mv);
+        const V3<R> wo = lift<R>(d3(v.wo)), N = lift<R>(d3(v.N));
         if (v.nee) {
-            V3<double> ld(0);
-            if (v.hasA) ld += v.LA * evalDisneyBsdf(me, wo, d3(v.wiA), N, true).f * v.kA;
-            if (v.hasE) ld += envRadiance(sp, d3(v.wiE)) * evalDisneyBsdf(me, wo, d3(v.wiE), N, true).f * v.kE;
+            V3<R> ld(0);
+            if (v.hasA) ld += lift<R>(v.LA) * evalDisneyBsdf(me, wo, lift<R>(d3(v.wiA)), N, true).f * R(v.kA);
+            if (v.hasE)
+                ld += envRadianceAt<R>(sp, d3(v.wiE)) * evalDisneyBsdf(me, wo, lift<R>(d3(v.wiE)), N, true).f * R(v.kE);
             L += T * ld;
-            V3<double> pl(0);
-            for (const auto& p : v.punct) pl += p.second * evalDisneyBsdf(me, wo, d3(p.first), N, true).f;
+            V3<R> pl(0);
+            for (const auto& p : v.punct) pl += lift<R>(p.second) * evalDisneyBsdf(me, wo, lift<R>(d3(p.first)), N, true).f;
             L += T * pl;
         }
         if (!v.valid) break;
-        const V3<double> wi = d3(v.wi);
-        V3<double> weight;
+        const V3<R> wi = lift<R>(d3(v.wi));
+        V3<R> weight;
         if (v.delta) {
-            if (dot(N, wi) < 0.0) weight = me.base;
-            else weight = F_SchlickV(fmx(dot(normalize(wo + wi), wi), 0.0), computeF0(me));
+            if (dot(N, wi) < R(0.0)) weight = me.base;
+            else weight = F_SchlickV(fmx(dot(normalize(wo + wi), wi), R(0.0)), computeF0(me));
         } else {
-            weight = evalDisneyBsdf(me, wo, wi, N, v.entering).f / (v.pdf0 + 1e-7);
+            weight = evalDisneyBsdf(me, wo, wi, N, v.entering).f / R(v.pdf0 + 1e-7);
         }
         T *= weight;
-        T *= v.beer;
+        T *= lift<R>(v.beer);
         if (v.rrDead) break;
-        T /= v.surv;
+        T /= R(v.surv);
     }
     return L;
+}
+inline V3<double> tapeEval(const oracle_scene* sp, const std::vector<TapeVertex>& tape) {
+    return tapeEvalT<double>(sp, tape);
 }
 
 }  // namespace orc
@@ -1709,6 +1851,29 @@ int oracle_sample_light(const oracle_scene* s, int env, const double p[3], const
     *pdf = ls.pdf;
     return 1;
 }
+void oracle_fetch_material(const oracle_scene* s, const pt_material* m, float u, float v, float out[20]) {
+    const MatEval<float> e = fetchMaterial<float>(s, *m, u, v);
+    const float vals[20] = {e.base.x, e.base.y, e.base.z, e.metallic, e.roughness, e.specTrans, e.ior, e.clearcoat,
+                            e.clearcoatRoughness, e.sheen, e.sheenTint, e.subsurface, e.anisotropic,
+                            e.sheenColor.x, e.sheenColor.y, e.sheenColor.z, e.emissive.x, e.emissive.y, e.emissive.z,
+                            e.specularFactor};
+    std::memcpy(out, vals, sizeof(vals));
+}
+
+int oracle_surface(const oracle_scene* s, const float o3[3], const float d3v[3], float out[16]) {
+    const V3<float> o(o3[0], o3[1], o3[2]);
+    const V3<float> d = normalize(V3<float>(d3v[0], d3v[1], d3v[2]));  // core/ray.h:13
+    const Hit<float> h = intersectScene(s, o, d);
+    std::memset(out, 0, 16 * sizeof(float));
+    if (!h.hit) return 0;
+    const SurfacePt<float> sp = surfaceAt(s, h, o, d);
+    const float vals[16] = {1.0f, h.t, sp.P.x, sp.P.y, sp.P.z, sp.sn.x, sp.sn.y, sp.sn.z, sp.gn.x, sp.gn.y, sp.gn.z,
+                            sp.N.x, sp.N.y, sp.N.z, sp.entering ? 1.0f : 0.0f,
+                            static_cast<float>(s->order[h.prim])};
+    std::memcpy(out, vals, sizeof(vals));
+    return 1;
+}
+
 void oracle_tex2d(const float* rgba, int w, int h, int ws, int wt, float u, float v, float out[4]) {
     tex2D(rgba, w, h, wrapMode(ws), wrapMode(wt), u, v, out);
 }
@@ -1795,31 +1960,43 @@ void oracle_microfacet(const float* in, const float* f0, const float* wo3, const
     out[24] = w.x; out[25] = w.y; out[26] = w.z;
 }
 
-void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
-                       uint32_t frame, const float* weights, int nparams, const int* kind, const int* pa, const int* pb,
-                       const int* pc, double h, int threads, double* grad, double* value) {
+}  // extern "C"
+
+namespace {
+
+template <class Fn>
+void parallelFor(size_t n, int nt, Fn&& fn) {
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        while (true) {
+            size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            fn(i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nt; ++i) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
+int threadCount(int threads) {
+    return threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+}
+
+bool weighted(const float* w, size_t px) { return w[3 * px] != 0.0f || w[3 * px + 1] != 0.0f || w[3 * px + 2] != 0.0f; }
+
+// The tape of every path of the batch (kernelPathTrace's sampler seeding,
+// path.cu:216-228) at pixels whose weights are not all zero.
+std::vector<std::vector<TapeVertex>> recordTapes(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth,
+                                                 int sampler, uint32_t seed, uint32_t frame, const float* weights,
+                                                 int nt) {
     pt_camera_frame cf;
     oracle_camera_frame(cam, &cf);
     const int W = cam->width, H = cam->height;
-    const size_t npaths = static_cast<size_t>(W) * H * spf;
-    std::vector<std::vector<TapeVertex>> tapes(npaths);
-    std::vector<float> f32(static_cast<size_t>(W) * H * 3, 0.0f);
-    int nt = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
-    auto parallel = [&](size_t n, auto&& fn) {
-        std::atomic<size_t> next{0};
-        auto work = [&]() {
-            while (true) {
-                size_t i = next.fetch_add(1);
-                if (i >= n) break;
-                fn(i);
-            }
-        };
-        std::vector<std::thread> pool;
-        for (int i = 1; i < nt; ++i) pool.emplace_back(work);
-        work();
-        for (auto& t : pool) t.join();
-    };
-    parallel(static_cast<size_t>(W) * H, [&](size_t px) {
+    std::vector<std::vector<TapeVertex>> tapes(static_cast<size_t>(W) * H * spf);
+    parallelFor(static_cast<size_t>(W) * H, nt, [&](size_t px) {
+        if (!weighted(weights, px)) return;
         const int x = static_cast<int>(px % W), y = static_cast<int>(px / W);
         Ctx c;
         for (int i = 0; i < spf; ++i) {
@@ -1829,9 +2006,23 @@ void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int
             tapes[px * spf + i] = pathTape(s, cf, W, H, maxDepth, x, y, rng, c);
         }
     });
+    return tapes;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
+                       uint32_t frame, const float* weights, int nparams, const int* kind, const int* pa, const int* pb,
+                       const int* pc, double h, int threads, double* grad, double* value) {
+    const int W = cam->width, H = cam->height;
+    const int nt = threadCount(threads);
+    const auto tapes = recordTapes(s, cam, spf, maxDepth, sampler, seed, frame, weights, nt);
     auto image_dot = [&](const oracle_scene* sp) {
         std::vector<double> part(static_cast<size_t>(W) * H, 0.0);
-        parallel(static_cast<size_t>(W) * H, [&](size_t px) {
+        parallelFor(static_cast<size_t>(W) * H, nt, [&](size_t px) {
+            if (!weighted(weights, px)) return;
             double acc = 0.0;
             for (int i = 0; i < spf; ++i) {
                 V3<double> L = tapeEval(sp, tapes[px * spf + i]);
@@ -1865,6 +2056,84 @@ void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int
         fp = static_cast<float>(x0 + h);
         fm = static_cast<float>(x0 - h);
         grad[k] = (image_dot(&sp) - image_dot(&sm)) / (static_cast<double>(fp) - static_cast<double>(fm));
+    }
+}
+
+void oracle_adjoint(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
+                    uint32_t frame, const float* weights, int threads, double* g_mat, double* g_tex, double* g_env,
+                    double* value) {
+    const int W = cam->width, H = cam->height;
+    const int nt = threadCount(threads);
+    const auto tapes = recordTapes(s, cam, spf, maxDepth, sampler, seed, frame, weights, nt);
+    ADTape proto;
+    proto.mats = s->m.data();
+    proto.nmat = s->m.size();
+    int64_t base = static_cast<int64_t>(5 * s->m.size());
+    const int64_t tex0 = base;
+    for (const Texture& t : s->tex) {
+        proto.blocks.push_back({t.rgba.data(), t.rgba.size(), base});
+        base += static_cast<int64_t>(t.rgba.size());
+    }
+    const int64_t env0 = base;
+    if (s->env.w > 0) {
+        proto.blocks.push_back({s->env.rgba.data(), s->env.rgba.size(), base});
+        base += static_cast<int64_t>(s->env.rgba.size());
+    }
+    proto.nparams = base;
+    // per-thread gradients, summed in thread order at the end
+    std::vector<std::vector<double>> part(nt);
+    std::vector<double> pxval(static_cast<size_t>(W) * H, 0.0);
+    std::atomic<size_t> next{0};
+    auto work = [&](int ti) {
+        ADTape tape = proto;
+        std::vector<double>& g = part[ti];
+        g.assign(static_cast<size_t>(proto.nparams), 0.0);
+        std::vector<double> adj;
+        tlTape = &tape;
+        while (true) {
+            const size_t px = next.fetch_add(1);
+            if (px >= static_cast<size_t>(W) * H) break;
+            if (!weighted(weights, px)) continue;
+            double acc = 0.0;
+            for (int i = 0; i < spf; ++i) {
+                tape.nodes.clear();
+                tape.leaves.clear();
+                const V3<AD> L = tapeEvalT<AD>(s, tapes[px * spf + i]);
+                adj.assign(tape.nodes.size(), 0.0);
+                const AD Lc[3] = {L.x, L.y, L.z};
+                for (int c = 0; c < 3; ++c) {
+                    const double w = weights[3 * px + c];
+                    acc += w * std::fmin(Lc[c].v, 100.0);
+                    // the clamp at 100 (path.cu:201-204) passes no derivative above it
+                    if (Lc[c].i >= 0 && Lc[c].v < 100.0) adj[Lc[c].i] += w / spf;
+                }
+                for (size_t k = tape.nodes.size(); k-- > 0;) {
+                    const double a = adj[k];
+                    if (a == 0.0) continue;
+                    const ADNode& n = tape.nodes[k];
+                    if (n.a >= 0) adj[n.a] += a * n.da;
+                    if (n.b >= 0) adj[n.b] += a * n.db;
+                }
+                for (const auto& lf : tape.leaves) g[lf.second] += adj[lf.first];
+            }
+            pxval[px] = acc / spf;
+        }
+        tlTape = nullptr;
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nt; ++i) pool.emplace_back(work, i);
+    work(0);
+    for (auto& t : pool) t.join();
+    std::vector<double> g(static_cast<size_t>(proto.nparams), 0.0);
+    for (const auto& p : part)
+        for (size_t k = 0; k < g.size(); ++k) g[k] += p[k];
+    if (g_mat) std::memcpy(g_mat, g.data(), sizeof(double) * 5 * s->m.size());
+    if (g_tex) std::memcpy(g_tex, g.data() + tex0, sizeof(double) * static_cast<size_t>(env0 - tex0));
+    if (g_env && s->env.w > 0) std::memcpy(g_env, g.data() + env0, sizeof(double) * s->env.rgba.size());
+    if (value) {
+        double sum = 0.0;
+        for (double v : pxval) sum += v;
+        *value = sum;
     }
 }
 

@@ -92,6 +92,19 @@ int oracle_sample_light(const oracle_scene* s, int env, const double p[3], const
 /* Bilinear texture fetch with CUDA normalized-coordinate semantics. */
 void oracle_tex2d(const float* rgba, int w, int h, int wrap_s, int wrap_t, float u, float v, float out[4]);
 
+/* fetchMaterial (materials/disney_bsdf.cuh:53-122) of material m with the
+ * scene's textures at texture coordinates (u, v), in float: out[20] = base
+ * rgb, metallic, roughness, specTrans, ior, clearcoat, clearcoatRoughness,
+ * sheen, sheenTint, subsurface, anisotropic, sheenColor rgb, emissive rgb,
+ * specularFactor. */
+void oracle_fetch_material(const oracle_scene* s, const pt_material* m, float u, float v, float out[20]);
+/* The shading point the integrator builds at the closest hit of ray (o, d)
+ * (path.cu:86-103), in float: out[16] = 1, t, P xyz, interpolated normal xyz,
+ * the normal on the ray's side (double-sided flip) xyz, the shading normal
+ * after the normal map xyz, entering, original face id.  Returns 0 (out all
+ * zero) on a miss. */
+int oracle_surface(const oracle_scene* s, const float o[3], const float d[3], float out[16]);
+
 /* Renders pixels [x0,x1) x [y0,y1) of one batch (frame) of spf samples per
  * pixel, as kernelPathTrace does (integrators/path.cu:209-242): out gets the
  * batch mean RGB per pixel (row-major over the rectangle, 3 floats).
@@ -115,6 +128,17 @@ void oracle_render(const oracle_scene* s, const pt_camera* cam, int spf, int max
 void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int max_depth, int sampler, uint32_t seed,
                        uint32_t frame, const float* weights, int nparams, const int* kind, const int* a, const int* b,
                        const int* c, double h, int threads, double* grad, double* value);
+
+/* The host reference adjoint: the gradient of  sum_pixels weights . I_batch
+ * (the same batch and paths as oracle_adjoint_fd) with respect to every
+ * parameter at once, by reverse-mode differentiation in double of the
+ * detached estimator of each recorded path.  g_mat (nm x 5: base colour rgb,
+ * roughness, metallic), g_tex (the texel channels of every texture, in
+ * texture order, 4 per texel), g_env (4 per environment texel); any may be
+ * null.  Pixels whose weights are all zero are not traced. */
+void oracle_adjoint(const oracle_scene* s, const pt_camera* cam, int spf, int max_depth, int sampler, uint32_t seed,
+                    uint32_t frame, const float* weights, int threads, double* g_mat, double* g_tex, double* g_env,
+                    double* value);
 
 #ifdef __cplusplus
 }

@@ -24,8 +24,10 @@ FP64 (no reference values exist: disney_bsdf.cuh needs CUDA):
 
 eval returns f times cos_l.  A wrong alpha mapping, a dropped 1/4 or pi, a
 Smith term of the wrong form or a Fresnel on the wrong angle fails these.
-Isotropic materials only: for anisotropy the reference evaluates G with
-alpha = sqrt(ax ay), not the anisotropic Lambda of the literature.  The
+Anisotropic materials: D is the anisotropic GGX with Burley's aspect
+mapping of (roughness, anisotropic) to (alpha_x, alpha_y); the reference
+evaluates G with alpha = sqrt(ax ay), not the anisotropic Lambda of the
+literature.  The
 reference adds 1e-7 to several denominators; inputs keep the relative
 effect below 1e-5."""
 import numpy as np
@@ -182,3 +184,35 @@ def test_transmission_is_walters_rough_dielectric_btdf(roughness):
         assert np.allclose(f, ft, rtol=2e-5), (wo, wi, f, ft)
         assert p == pytest.approx(pdf, rel=2e-5), (wo, wi, p, pdf)
         checked += 1
+
+
+@pytest.mark.parametrize("aniso", [0.3, 0.8])
+def test_anisotropic_alpha_is_burleys_aspect_mapping(aniso):
+    # Burley 2012 (addendum B): aspect = sqrt(1 - 0.9 anisotropic), alpha_x =
+    # r^2 / aspect, alpha_y = r^2 aspect; D is anisotropic GGX in the frame
+    # math/frame.h builds around N = +z (tangent +y, bitangent -x); the
+    # reference's G and VNDF G1 use the isotropic alpha sqrt(alpha_x alpha_y).
+    # A metal isolates the specular lobe (its selection weight is 1).
+    r = 0.6
+    mat = _material(metallic=1.0, roughness=r, anisotropic=aniso)
+    base = np.array([0.8, 0.6, 0.4])
+    n = np.array([0.0, 0.0, 1.0])
+    t, b = np.array([0.0, 1.0, 0.0]), np.array([-1.0, 0.0, 0.0])
+    aspect = np.sqrt(1.0 - 0.9 * aniso)
+    ax, ay = max(r * r / aspect, 0.001), max(r * r * aspect, 0.001)
+    ag = np.sqrt(ax * ay)
+    rng = np.random.default_rng(13)
+    for _ in range(60):
+        wo, wi = _upper(rng, n), _upper(rng, n)
+        h = (wo + wi) / np.linalg.norm(wo + wi)
+        hx, hy, hz = h @ t, h @ b, h @ n
+        D = 1.0 / (np.pi * ax * ay * ((hx / ax) ** 2 + (hy / ay) ** 2 + hz ** 2) ** 2)
+        cl, cv = wi @ n, wo @ n
+        G = 1.0 / (1.0 + _smith_lambda(cl, ag) + _smith_lambda(cv, ag))
+        F = _schlick(base, h @ wi)
+        ef = D * G * F / (4 * cl * cv) * cl
+        g1 = 1.0 / (1.0 + _smith_lambda(cv, ag))
+        epdf = g1 * max(0.0, wo @ h) * D / cv / (4 * (h @ wi))
+        f, pdf = oracle.bsdf_eval(mat, wo, wi, n, True, fp64=True)
+        assert np.allclose(f, ef, rtol=2e-5), (wo, wi, f, ef)
+        assert pdf == pytest.approx(epdf, rel=2e-5)
