@@ -138,23 +138,32 @@ inline float computeRelMSE(const float* img, const float* ref, int width, int he
 }
 inline float rmseToDb(float rmse) { return rmse <= 0.0f ? 100.0f : -20.0f * std::log10(rmse); }
 
+// The convergence table of film/quality_metrics.h:41-84, one row per
+// checkpoint: sampler,spp,rmse,psnr_db,time_ms plus relmse as a sixth column.
+inline std::string convergenceRow(const std::string& label, const QualityMetrics& m) {
+    char row[160];
+    std::snprintf(row, sizeof(row), "%s,%d,%.6f,%.2f,%.1f,%.8f", label.c_str(), m.spp, m.rmse, m.psnr, m.timeMs,
+                  m.relMSE);
+    return row;
+}
 inline void writeConvergenceCSV(const std::string& path, const std::vector<QualityMetrics>& data,
                                 const std::string& label) {
-    bool exists = std::ifstream(path).good();
+    const bool fresh = !std::ifstream(path).good();
     std::ofstream f(path, std::ios::app);
     if (!f) {
         std::cerr << "Cannot open " << path << " for writing" << std::endl;
         return;
     }
-    if (!exists) f << "sampler,spp,rmse,psnr_db,time_ms,relmse\n";
-    for (const auto& m : data)
-        f << label << "," << m.spp << "," << m.rmse << "," << m.psnr << "," << m.timeMs << "," << m.relMSE << "\n";
+    if (fresh) f << "sampler,spp,rmse,psnr_db,time_ms,relmse\n";
+    for (const auto& m : data) f << convergenceRow(label, m) << "\n";
+    std::cout << "Convergence data appended to " << path << " (" << label << ", " << data.size() << " checkpoints)"
+              << std::endl;
 }
 inline void printConvergenceTable(const std::vector<QualityMetrics>& data, const std::string& label) {
-    std::printf("%-8s %8s %12s %10s %12s %12s\n", "sampler", "spp", "rmse", "psnr(dB)", "time(ms)", "relMSE");
+    std::printf("\n=== Convergence: %s ===\n%8s %12s %12s %12s %14s\n", label.c_str(), "SPP", "RMSE", "PSNR(dB)",
+                "Time(ms)", "relMSE");
     for (const auto& m : data)
-        std::printf("%-8s %8d %12.6f %10.2f %12.1f %12.6f\n", label.c_str(), m.spp, m.rmse, m.psnr, m.timeMs,
-                    m.relMSE);
+        std::printf("%8d %12.6f %12.2f %12.1f %14.8f\n", m.spp, m.rmse, m.psnr, m.timeMs, m.relMSE);
 }
 
 class Renderer {

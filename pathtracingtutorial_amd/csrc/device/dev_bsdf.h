@@ -87,7 +87,11 @@ struct TexTaps {
     uint32_t idx[4];
     float w[4];
 };
+// The texel coordinate u W - 0.5 rounds the product and the difference on
+// their own (no multiply-add fusion): its fraction is quantized to 1/256, and
+// a fused form can land one step away from the documented arithmetic.
 __device__ __forceinline__ TexTaps tex_taps(int w, int h, int wrap_s, int wrap_t, float u, float v) {
+#pragma clang fp contract(off)
     if (wrap_s == 0) u = u - floorf(u);
     if (wrap_t == 0) v = v - floorf(v);
     float x = u * static_cast<float>(w) - 0.5f, y = v * static_cast<float>(h) - 0.5f;
@@ -108,16 +112,23 @@ __device__ __forceinline__ TexTaps tex_taps(int w, int h, int wrap_s, int wrap_t
     return t;
 }
 
+// The weighted sum of a bilinear lookup, each product and sum rounded on its own.
+__device__ __forceinline__ float4 texel_blend(float w00, float w10, float w01, float w11, float4 t00, float4 t10,
+                                              float4 t01, float4 t11) {
+#pragma clang fp contract(off)
+    return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
+                       w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+                       w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z,
+                       w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w);
+}
+
 __device__ __forceinline__ float4 tex_bilinear(const float4* texels, int w, int h, int wrap_s, int wrap_t, float u,
                                                float v) {
     TexTaps tp = tex_taps(w, h, wrap_s, wrap_t, u, v);
     float4 t00 = texels[tp.idx[0]], t10 = texels[tp.idx[1]];
     float4 t01 = texels[tp.idx[2]], t11 = texels[tp.idx[3]];
     float w00 = tp.w[0], w10 = tp.w[1], w01 = tp.w[2], w11 = tp.w[3];
-    return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
-                       w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
-                       w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z,
-                       w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w);
+    return texel_blend(w00, w10, w01, w11, t00, t10, t01, t11);
 }
 
 __device__ __forceinline__ bool tex_valid(const DevScene& s, int id) {

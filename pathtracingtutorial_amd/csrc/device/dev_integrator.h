@@ -26,8 +26,10 @@ struct TraceCtx {
 };
 
 // ---- environment --------------------------------------------------------
-// envmap.h:15-24
+// envmap.h:15-24 (each step rounded on its own: the texel fraction of u and v
+// is quantized to 1/256 by the lookup)
 __device__ __forceinline__ void dir_to_uv(F3 d, float& u, float& v) {
+#pragma clang fp contract(off)
     u = atan2f(d.z, d.x);
     v = asinf(d.y);
     u *= kInv2Pi;
@@ -308,17 +310,33 @@ struct Surface {
     float t;
 };
 
-// intersectTriangle's hit record (triangle.h:40-49).
+// Barycentric blend a w + b u + c v, every product and sum rounded on its own.
+__device__ __forceinline__ float bary_rn(float a, float b, float c, float w, float u, float v) {
+#pragma clang fp contract(off)
+    return a * w + b * u + c * v;
+}
+
+// intersectTriangle's hit record (triangle.h:40-49).  The point, normal and
+// texture coordinates round as the reference's host arithmetic does (no
+// multiply-add fusion): a texture's filter weights are quantized to 1/256
+// of a texel, and a last-bit difference in uv can move one step between
+// neighbouring texels.
 __device__ __forceinline__ Surface make_surface(const DevScene& s, F3 o, F3 d, const Hit& h) {
     Surface sf;
     uint4 f = s.faces[h.tri];
-    float w = 1.0f - h.u - h.v;
+    float w;
+    {
+#pragma clang fp contract(off)
+        w = 1.0f - h.u - h.v;
+        sf.p = f3(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
+    }
     sf.t = h.t;
-    sf.p = o + h.t * d;
-    sf.n = normalize(f3(s.vnrm[f.x]) * w + f3(s.vnrm[f.y]) * h.u + f3(s.vnrm[f.z]) * h.v);
+    const float4 n0 = s.vnrm[f.x], n1 = s.vnrm[f.y], n2 = s.vnrm[f.z];
+    sf.n = normalize_rn(f3(bary_rn(n0.x, n1.x, n2.x, w, h.u, h.v), bary_rn(n0.y, n1.y, n2.y, w, h.u, h.v),
+                           bary_rn(n0.z, n1.z, n2.z, w, h.u, h.v)));
     float2 t0 = s.vuv[f.x], t1 = s.vuv[f.y], t2 = s.vuv[f.z];
-    sf.u = t0.x * w + t1.x * h.u + t2.x * h.v;
-    sf.v = t0.y * w + t1.y * h.u + t2.y * h.v;
+    sf.u = bary_rn(t0.x, t1.x, t2.x, w, h.u, h.v);
+    sf.v = bary_rn(t0.y, t1.y, t2.y, w, h.u, h.v);
     sf.bu = h.u;
     sf.bv = h.v;
     sf.tri = h.tri;
@@ -375,28 +393,46 @@ __device__ __forceinline__ F3 normal_map(const DevScene& s, const pt_material& m
 }
 
 // ---- camera -------------------------------------------------------------------
-// emitRay (path.cu:31-59)
+// The thin lens of emitRay (path.cu:48-56): the origin moves to a point of
+// the aperture disk, the ray passes through the focal point.
+__device__ __forceinline__ void thin_lens(F3& pos, F3& dir, F3 fwd, F3 rgt, F3 cup, float focal_distance,
+                                          float aperture, float l1, float l2) {
+#pragma clang fp contract(off)
+    const float k = focal_distance / dot_rn(dir, fwd);
+    const F3 focal = f3(pos.x + dir.x * k, pos.y + dir.y * k, pos.z + dir.z * k);
+    const float r = sqrtf(l1) * aperture;
+    const float th = kTwoPi * l2;
+    const float rc = r * cosf(th), rs = r * sinf(th);
+    pos = f3(pos.x + (rgt.x * rc + cup.x * rs), pos.y + (rgt.y * rc + cup.y * rs), pos.z + (rgt.z * rc + cup.z * rs));
+    dir = normalize_rn(f3(focal.x - pos.x, focal.y - pos.y, focal.z - pos.z));
+}
+
+// emitRay (path.cu:31-59), every product and sum rounded on its own as in
+// the reference's host arithmetic (the ray decides which texels a lookup
+// weights, to 1/256 of a texel).
 __device__ __forceinline__ void camera_ray(const RenderParams& p, int x, int y, Sampler& rng, F3& o, F3& d) {
     float jx = rng.next();
     float jy = rng.next();
-    float u = (2.0f * (static_cast<float>(x) + jx) / static_cast<float>(p.width) - 1.0f) * p.tan_half_fov * p.aspect;
-    float v = (2.0f * (static_cast<float>(y) + jy) / static_cast<float>(p.height) - 1.0f) * p.tan_half_fov;
     F3 pos = f3(p.cam_pos[0], p.cam_pos[1], p.cam_pos[2]);
-    F3 fwd = f3(p.cam_fwd[0], p.cam_fwd[1], p.cam_fwd[2]);
-    F3 rgt = f3(p.cam_right[0], p.cam_right[1], p.cam_right[2]);
-    F3 cup = f3(p.cam_up[0], p.cam_up[1], p.cam_up[2]);
-    F3 dir = normalize(u * rgt + v * cup + fwd);
+    const F3 fwd = f3(p.cam_fwd[0], p.cam_fwd[1], p.cam_fwd[2]);
+    const F3 rgt = f3(p.cam_right[0], p.cam_right[1], p.cam_right[2]);
+    const F3 cup = f3(p.cam_up[0], p.cam_up[1], p.cam_up[2]);
+    F3 dir;
+    {
+#pragma clang fp contract(off)
+        const float u = (2.0f * (static_cast<float>(x) + jx) / static_cast<float>(p.width) - 1.0f) * p.tan_half_fov *
+                        p.aspect;
+        const float v = (2.0f * (static_cast<float>(y) + jy) / static_cast<float>(p.height) - 1.0f) * p.tan_half_fov;
+        dir = normalize_rn(f3(u * rgt.x + v * cup.x + fwd.x, u * rgt.y + v * cup.y + fwd.y,
+                              u * rgt.z + v * cup.z + fwd.z));
+    }
     if (p.aperture > 0.0f) {
-        F3 focal = pos + dir * (p.focal_distance / dot(dir, fwd));
-        float l1 = rng.next();
-        float l2 = rng.next();
-        float r = sqrtf(l1) * p.aperture;
-        float th = kTwoPi * l2;
-        pos = pos + (rgt * (r * cosf(th)) + cup * (r * sinf(th)));
-        dir = normalize(focal - pos);
+        const float l1 = rng.next();
+        const float l2 = rng.next();
+        thin_lens(pos, dir, fwd, rgt, cup, p.focal_distance, p.aperture, l1, l2);
     }
     o = pos;
-    d = normalize(dir);  // Ray's constructor normalizes (core/ray.h:13)
+    d = normalize_rn(dir);  // Ray's constructor normalizes (core/ray.h:13)
 }
 
 // ---- path ---------------------------------------------------------------------

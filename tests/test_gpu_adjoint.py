@@ -265,3 +265,68 @@ def test_wavefront_adjoint_punctual_lights(tmp_path):
     _assert_same_grads(out["path"], out["wavefront"], keys=("base_color", "roughness", "metallic"))
     assert float(out["wavefront"]["base_color"].abs().sum()) > 0
     assert torch.allclose(out["path"]["replay"], out["wavefront"]["replay"], rtol=1e-4, atol=1e-5)
+
+
+def _same_path_pixels(img, ref, tol=1e-5):
+    """Pixels whose batch mean equals the oracle's to float rounding in every
+    channel: all their paths took the oracle's branches."""
+    img = np.asarray(img, np.float64)[..., :3]
+    ref = np.asarray(ref, np.float64)
+    return np.all(np.abs(img - ref) <= tol * (1.0 + np.abs(ref)), axis=-1)
+
+
+def compare_arrays(name, gpu, host, rel=1e-3, floor=1e-5, frac=0.995):
+    """Element by element, |gpu - host| <= rel |host| + floor max|host| for at
+    least `frac` of the elements either side gives a gradient (all of them
+    when frac = 1), and the whole array within rel in the L1 norm.  Texel gradients pass through bilinear filter weights
+    quantized to 1/256 (the texture unit's, cuda_texture.cu:56-60), taken at
+    a lookup coordinate that the GPU's and the host's libm (atan2, asin of
+    the environment mapping) may round one ulp apart: now and then a weight
+    step moves between neighbouring texels; the L1 norm does not see it."""
+    gpu = np.asarray(gpu, np.float64)
+    host = np.asarray(host, np.float64)
+    scale = float(np.abs(host).max())
+    err = np.abs(gpu - host)
+    ok = err <= rel * np.abs(host) + floor * scale
+    nz = host != 0
+    big = np.abs(host) > 1e-3 * scale
+    worst = float((err[big] / np.abs(host[big])).max()) if big.any() else 0.0
+    l1 = float(err.sum() / max(np.abs(host).sum(), 1e-300))
+    print(f"{name}: {host.size} elements, {int(nz.sum())} nonzero, {int((~ok).sum())} outside tolerance, "
+          f"max |d| / max|host| {float(err.max()) / max(scale, 1e-300):.2e}, worst relative (|host| > 1e-3 max) "
+          f"{worst:.2e}, L1 {l1:.2e}")
+    touched = nz | (gpu != 0)  # a weight step can also reach a texel the other side gives weight 0
+    assert ok[touched].mean() >= frac, (name, np.argwhere(~ok)[:5], gpu[~ok][:5], host[~ok][:5])
+    assert l1 <= rel, (name, l1)
+    return float(err.max()) / max(scale, 1e-300)
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_adjoint_full_arrays_match_host_reference_adjoint(form):
+    # Every element of every gradient array (8 x 3 base colours, 8 roughness,
+    # 8 metallic, the 256^2 albedo texture, the 512x256 environment) against
+    # the oracle's FP64 reverse-mode adjoint of the same batch.  The weights
+    # are restricted to pixels whose forward image equals the oracle's
+    # (nearly all): there the GPU traced the oracle's paths.
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    sm = _diff_cornell(w=32, h=32, spf=4)
+    sm.setAdjointIntegrator(FORMS[form]())
+    cam, info = sm.camera(), sm.info()
+    dr = DiffRenderer(sm)
+    frame = 1
+    img = dr.forward_image(frame).cpu().numpy()
+    ora = _oracle_with_textures(sm)
+    ref_img, _ = ora.render(cam, info.sample_per_frame, info.max_depth, info.sampler, info.seed, frame=frame)
+    same = _same_path_pixels(img, ref_img)
+    assert same.mean() >= 0.98, same.mean()
+    rng = np.random.default_rng(21)
+    weights = (rng.uniform(-1.0, 1.0, (info.height, info.width, 3)) * same[..., None]).astype(np.float32)
+    g = torch.zeros((info.height, info.width, 4), dtype=torch.float32, device="cuda")
+    g[..., :3] = torch.from_numpy(weights).cuda()
+    out = dr.adjoint(frame, g)
+    host = ora.adjoint(cam, info.sample_per_frame, info.max_depth, info.sampler, info.seed, frame, weights)
+    compare_arrays("base_color", out["base_color"].cpu().numpy(), host["base_color"], frac=1.0)
+    compare_arrays("roughness", out["roughness"].cpu().numpy(), host["roughness"], frac=1.0)
+    compare_arrays("metallic", out["metallic"].cpu().numpy(), host["metallic"], frac=1.0)
+    compare_arrays("texture", out["texture"].cpu().numpy()[:, :3], host["textures"][0].reshape(-1, 4)[:, :3])
+    compare_arrays("env", out["env"].cpu().numpy()[..., :3], host["env"][..., :3])

@@ -37,3 +37,94 @@ def test_cli_renders_cornell_with_both_integrators(tmp_path):
 def test_cli_rejects_missing_scene(tmp_path):
     r = subprocess.run([PT, str(tmp_path / "nope.json")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
+
+
+def _small_cornell(tmp_path, size=64):
+    import json
+    scene = json.load(open(os.path.join(os.path.dirname(__file__), "..", "scenes", "cornell_box.json")))
+    scene["camera"]["resolution"] = [size, size]
+    scene["render"]["samplePerFrame"] = 4
+    path = tmp_path / "cornell_small.json"
+    path.write_text(json.dumps(scene))
+    return path
+
+
+@pytest.mark.skipif(not os.path.exists(PT), reason="pt CLI not built")
+def test_cli_benchmark_writes_the_convergence_table(tmp_path):
+    # main.cpp:106-146: a Sobol reference image, then PCG and Sobol at each
+    # checkpoint; the CSV has the reference's columns plus relMSE
+    scene = _small_cornell(tmp_path)
+    r = subprocess.run([PT, str(scene), "--benchmark", "--checkpoints", "4,8,16,32", "--reference-spp", "512",
+                        "--integrator", "path", "--csv", "conv.csv"], capture_output=True, text=True, timeout=300,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "=== Rendering reference image (512 spp, sobol) ===" in r.stdout
+    assert "=== Benchmark: PCG ===" in r.stdout and "=== Benchmark: Sobol ===" in r.stdout
+    assert (tmp_path / "benchmark_ref.hdr").exists()
+    lines = (tmp_path / "conv.csv").read_text().strip().splitlines()
+    assert lines[0] == "sampler,spp,rmse,psnr_db,time_ms,relmse"
+    rows = [l.split(",") for l in lines[1:]]
+    assert [(r_[0], int(r_[1])) for r_ in rows] == [(s, n) for s in ("pcg", "sobol") for n in (4, 8, 16, 32)]
+    for s in ("pcg", "sobol"):
+        rmse = [float(r_[2]) for r_ in rows if r_[0] == s]
+        psnr = [float(r_[3]) for r_ in rows if r_[0] == s]
+        rel = [float(r_[5]) for r_ in rows if r_[0] == s]
+        # more samples, less error (a 16x step: far beyond the noise of the estimate)
+        assert rel[-1] < 0.5 * rel[0] and rmse[-1] < 0.7 * rmse[0]
+        assert all(abs(p - (-20 * np.log10(e))) < 0.01 for p, e in zip(psnr, rmse))
+    # a given reference image is read instead of rendered
+    r2 = subprocess.run([PT, str(scene), "--benchmark", str(tmp_path / "benchmark_ref.hdr"), "--checkpoints", "4,8",
+                         "--integrator", "path", "--csv", "conv2.csv"], capture_output=True, text=True, timeout=300,
+                        cwd=tmp_path)
+    assert r2.returncode == 0, r2.stderr
+    assert "=== Reference image:" in r2.stdout
+    assert len((tmp_path / "conv2.csv").read_text().strip().splitlines()) == 5
+
+
+@pytest.mark.skipif(not os.path.exists(PT), reason="pt CLI not built")
+def test_cli_rejects_bad_option_values(tmp_path):
+    scene = _small_cornell(tmp_path)
+    for bad in (["--sampler", "halton"], ["--integrator", "bdpt"], ["--spp", "0"], ["--checkpoints", "4,,8"]):
+        r = subprocess.run([PT, str(scene), *bad], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+        assert r.returncode == 2 and "bad value" in r.stderr, bad
+
+
+def test_resize_and_reset_accumulation():
+    # renderer.cu:71-85: resize makes a new film and restarts accumulation;
+    # resetAccumulation restarts at frame 0, so the next batch repeats the first
+    sm = pt.SceneManager()
+    sm.generate(0)
+    cam = sm.camera()
+    cam.width, cam.height = 48, 40
+    sm.updateCamera(cam)
+    sm.setSamplePerFrame(2)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    r = pt.Renderer(sm, pt.PathIntegrator())
+    r.renderBatch()
+    first = r.downloadCurrentImage(1).copy()
+    r.renderBatch()
+    assert r.currentSpp() == 4
+    r.resetAccumulation()
+    assert r.currentSpp() == 0
+    r.renderBatch()
+    assert np.array_equal(r.downloadCurrentImage(1), first)
+    assert np.array_equal(r.downloadCurrentImage(0), first)  # the running mean of one batch is that batch
+    r.resize(32, 24)
+    assert (r.filmWidth(), r.filmHeight()) == (32, 24) and r.currentSpp() == 0
+    assert (sm.camera().width, sm.camera().height) == (32, 24)
+    r.renderBatch()
+    img = r.downloadCurrentImage(0)
+    assert img.shape == (24, 32, 4) and np.isfinite(img).all() and img[..., :3].mean() > 0
+    # the same pixels as a fresh 32x24 scene
+    sm2 = pt.SceneManager()
+    sm2.generate(0)
+    c2 = sm2.camera()
+    c2.width, c2.height = 32, 24
+    sm2.updateCamera(c2)
+    sm2.setSamplePerFrame(2)
+    sm2.buildAccel()
+    sm2.uploadToDevice()
+    r2 = pt.Renderer(sm2, pt.PathIntegrator())
+    r2.renderBatch()
+    assert np.array_equal(r2.downloadCurrentImage(0), img)

@@ -244,6 +244,17 @@ def _paired_batches(sm, ora, spf, batches, rect=None):
     return np.array(gpu), np.array(cpu), acc
 
 
+def _same_paths_check(gpu, cpu, frac=0.005, tol=1e-4):
+    """Batch by batch the GPU and the oracle draw the same random numbers per
+    path: nearly every pixel channel is the oracle's to float rounding (a few
+    differ where a rounding difference sent one of their paths another way)."""
+    g = np.asarray(gpu, np.float64)[..., :3]
+    c = np.asarray(cpu, np.float64)[..., :3]
+    bad = float((np.abs(g - c) > tol * (1.0 + np.abs(c))).mean())
+    assert bad <= frac, f"{bad:.4%} of pixel channels differ from the oracle's beyond {tol}"
+    return bad
+
+
 def _zscore_check(gpu, cpu, frac=0.005):
     k = gpu.shape[0]
     mg, mc = gpu.mean(0), cpu.mean(0)
@@ -268,6 +279,7 @@ def test_cornell_render_matches_oracle():
     ora = oracle.OracleScene.from_view(host_view(sm))
     gpu, cpu, acc = _paired_batches(sm, ora, 4, 12)
     assert np.isfinite(gpu).all() and gpu.max() <= 100.0
+    _same_paths_check(gpu, cpu, frac=0.001)
     _zscore_check(gpu, cpu)
     # running-mean accumulation (path.cu:234-241) equals the mean of the batch images
     assert np.allclose(acc, gpu.mean(0), rtol=1e-5, atol=1e-6)
@@ -287,7 +299,38 @@ def test_env_scene_render_matches_oracle():
     ora = oracle.OracleScene.from_view(host_view(sm))
     gpu, cpu, _ = _paired_batches(sm, ora, 4, 8)
     assert np.isfinite(gpu).all()
+    _same_paths_check(gpu, cpu, frac=0.004)
     _zscore_check(gpu, cpu, frac=0.01)
+
+
+@pytest.mark.parametrize("integ", ["path", "wavefront"])
+def test_thin_lens_render_matches_oracle(integ):
+    # emitRay's thin lens (path.cu:48-56): two more draws per camera ray, the
+    # origin on a disk of radius aperture, the ray through the focal point
+    sm = pt.SceneManager()
+    sm.generate(0)
+    cam = sm.camera()
+    cam.width, cam.height = 64, 64
+    cam.aperture, cam.focal_distance = 0.06, 1.6
+    sm.updateCamera(cam)
+    sm.setSamplePerFrame(4)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    ora = oracle.OracleScene.from_view(host_view(sm))
+    r = pt.Renderer(sm, pt.PathIntegrator() if integ == "path" else pt.WavefrontIntegrator())
+    gpu, cpu = [], []
+    for b in range(8):
+        r.renderBatch()
+        gpu.append(r.downloadCurrentImage(1)[..., :3].astype(np.float64))
+        img, _ = ora.render(cam, 4, 8, 0, 0, frame=b)
+        cpu.append(img.astype(np.float64))
+    gpu, cpu = np.array(gpu), np.array(cpu)
+    _same_paths_check(gpu, cpu, frac=0.002)
+    _zscore_check(gpu, cpu)
+    # the lens blurs: the image differs from the pinhole camera's
+    cam.aperture = 0.0
+    img0, _ = ora.render(cam, 4, 8, 0, 0, frame=0)
+    assert np.abs(img0 - cpu[0]).mean() > 1e-3
 
 
 def test_counters_and_ray_accounting():

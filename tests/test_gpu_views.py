@@ -85,3 +85,70 @@ def test_views_autograd_shapes():
     (img[..., :3] ** 2).mean().backward()
     assert rough.grad is not None and torch.isfinite(rough.grad).all()
     assert env.grad is not None and env.grad.shape == env.shape and env.grad.abs().sum() > 0
+
+
+def test_c4_views_adjoint_matches_host_reference_adjoint():
+    # The C4 workload (generate(5): 2,048,002 triangles, 16 ring cameras at
+    # 512^2, depth 10, gradients of the 1024^2 ground texture and the
+    # 1024x512 environment) at 2 spp, all 16 views in one adjoint schedule.
+    # The weights sit on three 24x24 tiles (ground, spheres, sky) of four
+    # views, on pixels whose forward image equals the oracle's; the whole
+    # texture and environment gradient arrays are compared with the sum of
+    # the oracle's FP64 reverse-mode adjoints of those views, and 32 texels
+    # and environment texels with the oracle's FP64 central differences.
+    import oracle
+    from pathtracingtutorial_amd.diff import MultiViewRenderer, ring_cameras
+    from test_gpu_adjoint import _oracle_with_textures, _same_path_pixels, compare_arrays
+    sm = pt.SceneManager()
+    sm.generate(5, 1)
+    sm.setSamplePerFrame(2)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    info = sm.info()
+    assert info.num_faces == 2048002 and (info.env_width, info.env_height) == (1024, 512)
+    cams = ring_cameras(16)
+    mv = MultiViewRenderer(sm, cams)
+    frame = 0
+    imgs = mv.forward_images(frame).cpu().numpy()
+    ora = _oracle_with_textures(sm)
+    rng = np.random.default_rng(17)
+    W = H = 512
+    weights = np.zeros((16, H, W, 3), np.float32)
+    tiles = [(244, 268, 60, 84), (244, 268, 244, 268), (244, 268, 440, 464)]  # ground, spheres, sky rows
+    views = [0, 5, 10, 13]
+    for v in views:
+        for (x0, x1, y0, y1) in tiles:
+            o, _ = ora.render(cams[v], 2, info.max_depth, info.sampler, info.seed, frame=frame * 16 + v,
+                              rect=(x0, x1, y0, y1))
+            same = _same_path_pixels(imgs[v, y0:y1, x0:x1], o)
+            assert same.mean() >= 0.85, (v, same.mean())
+            weights[v, y0:y1, x0:x1] = rng.uniform(-1, 1, (y1 - y0, x1 - x0, 3)) * same[..., None]
+    g = torch.zeros((16, H, W, 4), dtype=torch.float32, device="cuda")
+    g[..., :3] = torch.from_numpy(weights).cuda()
+    out = mv.adjoint(frame, g, (True, True, True, True, True))
+    tex = np.zeros((1024 * 1024, 4))
+    env = np.zeros((512, 1024, 4))
+    for v in views:
+        h = ora.adjoint(cams[v], 2, info.max_depth, info.sampler, info.seed, frame * 16 + v, weights[v])
+        tex += h["textures"][0].reshape(-1, 4)
+        env += h["env"]
+    # the 1024^2 ground texture is seen from afar: most touched texels get one
+    # or two lookups, so a 1/256 filter-weight step (see compare_arrays) shows
+    # on a larger share of them than in the close-up Cornell texture
+    rt = compare_arrays("C4 texture", out["texture"].cpu().numpy()[:, :3], tex[:, :3], frac=0.95)
+    re = compare_arrays("C4 env", out["env"].cpu().numpy()[..., :3], env[..., :3], frac=0.99)
+    print(f"C4 FP32 (GPU atomics) vs FP64 (host): max |d| / max |g|: texture {rt:.2e}, env {re:.2e}")
+    assert (np.abs(tex[:, :3]) > 0).sum() > 1000 and (np.abs(env[..., :3]) > 0).sum() > 1000
+    # central differences of the oracle's detached estimator on 32 scalars
+    flat_t = np.argsort(-np.abs(tex[:, :3]).reshape(-1))[:16]
+    flat_e = np.argsort(-np.abs(env[..., :3]).reshape(-1))[:16]
+    params = [(3, 0, int(k // 3), int(k % 3)) for k in flat_t] + [(4, 0, int(k // 3), int(k % 3)) for k in flat_e]
+    fd = np.zeros(len(params))
+    for v in views:
+        d, _ = ora.adjoint_fd(cams[v], 2, info.max_depth, info.sampler, info.seed, frame * 16 + v, weights[v],
+                              params, h=1e-4)
+        fd += d
+    gt, ge = out["texture"].cpu().numpy(), out["env"].cpu().numpy().reshape(-1, 4)
+    got = np.array([gt[b, c] if k == 3 else ge[b, c] for (k, a, b, c) in params])
+    scale = np.abs(fd).max()
+    assert np.all(np.abs(got - fd) <= 1e-3 * np.abs(fd) + 1e-5 * scale), list(zip(params, got, fd))
