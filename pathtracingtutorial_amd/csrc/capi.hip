@@ -133,6 +133,8 @@ struct pt_scene {
     int env_w = 0, env_h = 0, env_wrap_s = 0, env_wrap_t = 0;
     // render state
     DBuf<ptd::DevCounters> counters;
+    DBuf<uint32_t> plain_tiles;               // the megakernel's tile-queue counters
+    DBuf<unsigned long long> plain_spill;     // its traversal stack entries beyond LDS
     bool counting = false;
     uint32_t frame_count = 0;
     // wavefront integrator state (created at its first batch)
@@ -999,12 +1001,19 @@ pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out
         } else if (integrator == PT_INTEGRATOR_PATH) {
             ptd::DevScene v = make_view(s);
             ptd::RenderParams p = make_params(s);
+            const bool wide = s->wf_cfg.bvh_width == 4 && v.num_wnodes > 0;
+            size_t spill = 0;
+            int ntiles = 0;
+            hip_check(ptd::render_plain_scratch(wide, &spill, &ntiles), "megakernel occupancy");
+            if (s->plain_spill.n < spill) s->plain_spill.alloc(spill);
+            if (s->plain_tiles.n < static_cast<size_t>(ntiles)) s->plain_tiles.alloc(static_cast<size_t>(ntiles));
+            const ptd::PlainScratch scratch{s->plain_tiles.p, s->plain_spill.p, s->plain_spill.n};
             hipEvent_t e0, e1;
             hip_check(hipEventCreate(&e0), "hipEventCreate");
             hip_check(hipEventCreate(&e1), "hipEventCreate");
             hip_check(hipEventRecord(e0, s->stream), "hipEventRecord");
             hipError_t le = ptd::launch_render_plain(v, p, f->film_new.p, f->film_acc.p, s->counters.p, s->counting,
-                                                     s->stream);
+                                                     wide, scratch, s->stream);
             hip_check(hipEventRecord(e1, s->stream), "hipEventRecord");
             hipError_t se = hipEventSynchronize(e1);
             (void)hipEventElapsedTime(&ms, e0, e1);

@@ -193,8 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(DevScene s, const AdjView*
     int x = tx * kTile + static_cast<int>(threadIdx.x & (kTile - 1));
     int y = ty * kTile + static_cast<int>(threadIdx.x / kTile);
     TraceCtx c;
-    c.stk = stack + threadIdx.x;
-    c.stride = kThreads;
+    c.sk = TraceStack<kStackSize, kStackSize, uint32_t>{stack + threadIdx.x, kThreads, nullptr, 0};
     c.cnt.nodes = c.cnt.tris = 0;
     c.scnt.nodes = c.scnt.tris = 0;
     c.overflow = c.closest = c.shadow = 0;
@@ -211,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(DevScene s, const AdjView*
             else
                 rng.init_pcg(x, y, p.frame_count, static_cast<uint32_t>(i), p.seed);
             Sampler rng2 = rng;
-            InlineNee<COUNT> fwd(c);
+            InlineNee<COUNT, TraceCtx> fwd(c);
             AdjPath ap;
             ap.L = path_radiance<COUNT>(s, p, x, y, rng, c, fwd);
             // fminf(L, 100) passes the gradient below the clamp (path.cu:201-204)

@@ -16,14 +16,19 @@
 
 namespace ptd {
 
-struct TraceCtx {
-    uint32_t* stk;
-    int stride;
+// A per-path integrator's traversal context: the thread's stack, the tree it
+// walks (the 4-wide BVH when WIDE) and its ray counters.
+template <bool WIDE, class SK>
+struct TraceCtxT {
+    static constexpr bool kWide = WIDE;
+    SK sk;
     TraceCounters cnt;   // closest-hit queries
     TraceCounters scnt;  // shadow (any-hit) queries
     uint32_t overflow;
     uint32_t closest, shadow;
 };
+// The binary tree with a 32-entry stack all in LDS.
+using TraceCtx = TraceCtxT<false, TraceStack<kStackSize, kStackSize, uint32_t>>;
 
 // ---- environment --------------------------------------------------------
 // envmap.h:15-24 (each step rounded on its own: the texel fraction of u and v
@@ -105,10 +110,10 @@ __device__ __forceinline__ ShadowRay make_shadow(F3 p, F3 n, F3 dir, float max_d
     return r;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ bool occluded(const DevScene& s, const ShadowRay& r, TraceCtx& c) {
+template <bool COUNT, class Ctx>
+__device__ __forceinline__ bool occluded(const DevScene& s, const ShadowRay& r, Ctx& c) {
     c.shadow++;
-    return trace_any<COUNT>(s, r.o, r.d, r.tmax, c.stk, c.stride, c.scnt, c.overflow);
+    return trace_any_v<Ctx::kWide, COUNT>(s, r.o, r.d, r.tmax, c.sk, c.scnt, c.overflow);
 }
 
 // sampleAreaLight (direct_lighting.cuh:75-121) up to, not including, the
@@ -179,8 +184,8 @@ __device__ __forceinline__ LightSample sample_area_nv(const DevScene& s, F3 p, F
     return area_finish(p, n, g, sh);
 }
 
-template <bool COUNT>
-__device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+template <bool COUNT, class Ctx>
+__device__ __forceinline__ LightSample sample_area(const DevScene& s, F3 p, F3 n, Sampler& rng, Ctx& c) {
     ShadowRay sh;
     LightSample ls = sample_area_nv(s, p, n, rng, sh);
     if (ls.valid && occluded<COUNT>(s, sh, c)) ls.valid = false;
@@ -234,8 +239,8 @@ __device__ __forceinline__ LightSample sample_env_nv(const DevScene& s, F3 p, F3
     return env_finish(p, n, g, sh);
 }
 
-template <bool COUNT>
-__device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n, Sampler& rng, TraceCtx& c) {
+template <bool COUNT, class Ctx>
+__device__ __forceinline__ LightSample sample_env(const DevScene& s, F3 p, F3 n, Sampler& rng, Ctx& c) {
     ShadowRay sh;
     LightSample ls = sample_env_nv(s, p, n, rng, sh);
     if (ls.valid && occluded<COUNT>(s, sh, c)) ls.valid = false;
@@ -285,8 +290,8 @@ __device__ __forceinline__ bool punctual_light_sample(const DevScene& s, uint32_
 }
 
 // evaluatePunctualLights (direct_lighting.cuh:158-200)
-template <bool COUNT>
-__device__ __forceinline__ F3 punctual(const DevScene& s, F3 p, const MatEval<float>& m, F3 wo, F3 n, TraceCtx& c) {
+template <bool COUNT, class Ctx>
+__device__ __forceinline__ F3 punctual(const DevScene& s, F3 p, const MatEval<float>& m, F3 wo, F3 n, Ctx& c) {
     F3 ld = f3(0.0f);
     if (s.num_lights == 0 || s.lights == nullptr) return ld;
     for (uint32_t i = 0; i < s.num_lights; ++i) {
@@ -458,11 +463,11 @@ struct ShadeCtx {
 // and the contributions added at once (direct_lighting.cuh:220-249).
 // A policy also receives hooks at the points where radiance or throughput
 // changes (the adjoint uses them; here they are empty).
-template <bool COUNT>
+template <bool COUNT, class Ctx>
 struct InlineNee {
     static constexpr bool kDrawAhead = true;
-    TraceCtx& c;
-    __device__ __forceinline__ explicit InlineNee(TraceCtx& ctx) : c(ctx) {}
+    Ctx& c;
+    __device__ __forceinline__ explicit InlineNee(Ctx& ctx) : c(ctx) {}
     // The area term is added before the environment term, one at a time,
     // so the wavefront schedule (one shadow queue per term) sums alike.
     __device__ __forceinline__ void direct(const DevScene& s, PathState& st, bool va, const ShadowRay& sa, F3 ca,
@@ -649,22 +654,22 @@ __device__ __forceinline__ F3 clamp_radiance(F3 lo) {
 }
 
 // One path with a given visibility policy; returns its radiance before the clamp.
-template <bool COUNT, class Nee>
+template <bool COUNT, class Ctx, class Nee>
 __device__ __forceinline__ F3 path_radiance(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng,
-                                            TraceCtx& c, Nee& nee) {
+                                            Ctx& c, Nee& nee) {
     PathState st;
     begin_path(p, x, y, rng, st);
     for (; st.depth < p.max_depth; ++st.depth) {
         c.closest++;
-        Hit h = trace_closest<COUNT>(s, st.o, st.d, c.stk, c.stride, c.cnt, c.overflow);
+        Hit h = trace_closest_v<Ctx::kWide, COUNT>(s, st.o, st.d, c.sk, c.cnt, c.overflow);
         if (!bounce(s, p, h, st, rng, nee)) break;
     }
     return st.lo;
 }
 
-template <bool COUNT>
-__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, TraceCtx& c) {
-    InlineNee<COUNT> nee(c);
+template <bool COUNT, class Ctx>
+__device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y, Sampler& rng, Ctx& c) {
+    InlineNee<COUNT, Ctx> nee(c);
     return clamp_radiance(path_radiance<COUNT>(s, p, x, y, rng, c, nee));
 }
 

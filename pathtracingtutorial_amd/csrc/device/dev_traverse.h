@@ -121,31 +121,76 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
 // Per-thread traversal stack: the first LDS entries live in LDS
 // (stk[level * stride]), deeper ones in global memory
 // (spill[(level - LDS) * spill_stride]); TOTAL entries in all.  With
-// LDS == TOTAL there is no spill area.
-template <int LDS, int TOTAL = kStackSize>
+// LDS == TOTAL there is no spill area.  An entry E is the node word
+// (uint32_t), or for closest-hit traversals a DistEntry that also holds the
+// entry distance of the node's box, so that a popped node lying beyond the
+// closest hit found since it was pushed is dropped without being fetched
+// (the reference re-tests a popped node against the current distance,
+// bvh_traversal.cuh:58-60).
+// (one 64-bit word: the node word in the low half, the distance's bits in the high half)
+using DistEntry = unsigned long long;
+
+template <int LDS, int TOTAL = kStackSize, class E = uint32_t>
 struct TraceStack {
     static constexpr int kTotal = TOTAL;
-    uint32_t* stk;
+    static constexpr int kLds = LDS;
+    static constexpr bool kDist = sizeof(E) == sizeof(DistEntry);
+    using Entry = E;
+    E* stk;
     int stride;
-    uint32_t* spill;
+    E* spill;
     uint32_t spill_stride;
     // (the device-memory part is addressed as global memory: a generic
     // pointer would let the compiler merge both cases into one flat access)
-    __device__ __forceinline__ void put(int level, uint32_t v) const {
+    __device__ __forceinline__ void put(int level, E v) const {
         if (LDS >= TOTAL || level < LDS) stk[level * stride] = v;
-        else *(__attribute__((address_space(1))) uint32_t*)(spill + static_cast<size_t>(level - LDS) * spill_stride) = v;
+        else *(__attribute__((address_space(1))) E*)(spill + static_cast<size_t>(level - LDS) * spill_stride) = v;
     }
-    __device__ __forceinline__ uint32_t get(int level) const {
+    __device__ __forceinline__ E get(int level) const {
         if (LDS >= TOTAL || level < LDS) return stk[level * stride];
-        return *(const __attribute__((address_space(1))) uint32_t*)(spill +
-                                                                     static_cast<size_t>(level - LDS) * spill_stride);
+        return *(const __attribute__((address_space(1))) E*)(spill + static_cast<size_t>(level - LDS) * spill_stride);
+    }
+    // levels top, top + 1, top + 2, all in the LDS part
+    __device__ __forceinline__ void put3_lds(int top, E a, E b, E c) const {
+        E* p = stk + top * stride;
+        p[0] = a;
+        p[stride] = b;
+        p[2 * stride] = c;
     }
 };
 
+__device__ __forceinline__ uint32_t entry_word(uint32_t e) { return e; }
+__device__ __forceinline__ uint32_t entry_word(DistEntry e) { return static_cast<uint32_t>(e); }
+__device__ __forceinline__ float entry_dist(uint32_t) { return 0.0f; }
+__device__ __forceinline__ float entry_dist(DistEntry e) { return __uint_as_float(static_cast<uint32_t>(e >> 32)); }
+template <class E>
+__device__ __forceinline__ E make_entry(uint32_t word, float dist) {
+    if constexpr (sizeof(E) == sizeof(DistEntry))
+        return (static_cast<DistEntry>(__float_as_uint(dist)) << 32) | word;
+    else return word;
+}
+
+// Pops the next entry whose box can still hold a hit closer than tmax (any
+// entry, for stacks without distances); false when the stack is empty.
+template <class SK>
+__device__ __forceinline__ bool pop_live(const SK& sk, int& top, float tmax, uint32_t& cur) {
+    while (top > 0) {
+        const typename SK::Entry e = sk.get(--top);
+        if (!SK::kDist || entry_dist(e) <= tmax) {
+            cur = entry_word(e);
+            return true;
+        }
+    }
+    return false;
+}
+
 // Closest hit.
-template <bool FAST, bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_s_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, const TraceStack<LDS, TOTAL>& sk,
-                                               TraceCounters& cnt, uint32_t& overflow) {
+template <bool FAST, bool COUNT, class SK>
+__device__ __forceinline__ Hit trace_closest_s_impl(const DevScene& s, F3 o, F3 d, const RayInv& r,
+                                                    const SK& sk, TraceCounters& cnt,
+                                                    uint32_t& overflow) {
+    using E = typename SK::Entry;
+    constexpr int TOTAL = SK::kTotal;
     Hit h;
     h.t = kInf;
     h.u = h.v = 0.0f;
@@ -183,7 +228,7 @@ __device__ __forceinline__ Hit trace_closest_s_impl(const DevScene& s, F3 o, F3 
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < TOTAL) sk.put(top++, far);
+                if (top < TOTAL) sk.put(top++, make_entry<E>(far, fmaxf(ta, tb)));
                 else overflow++;
                 cur = near;
                 continue;
@@ -191,25 +236,28 @@ __device__ __forceinline__ Hit trace_closest_s_impl(const DevScene& s, F3 o, F3 
             if (ta >= 0.0f) { cur = da; continue; }
             if (tb >= 0.0f) { cur = db; continue; }
         }
-        if (top == 0) break;
-        cur = sk.get(--top);
+        if (!pop_live(sk, top, h.t, cur)) break;
     }
     return h;
 }
 
-template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+template <bool COUNT, class SK>
+__device__ __forceinline__ Hit trace_closest_s(const DevScene& s, F3 o, F3 d, const SK& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     const RayInv r = make_ray_inv(o, d);
     if (finite_inv(r)) return trace_closest_s_impl<true, COUNT>(s, o, d, r, sk, cnt, overflow);
     return trace_closest_s_impl<false, COUNT>(s, o, d, r, sk, cnt, overflow);
 }
 
-// Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).
-template <bool FAST, bool COUNT, int LDS, int TOTAL>
+// Any hit in (1e-4, tmax) (occludedScene, bvh_traversal.cuh:98-133).  (An
+// entry pushed here already starts before tmax, which does not change: with
+// distance entries pop_live drops nothing.)
+template <bool FAST, bool COUNT, class SK>
 __device__ __forceinline__ bool trace_any_s_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, float tmax,
-                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
-                                            uint32_t& overflow) {
+                                                 const SK& sk, TraceCounters& cnt,
+                                                 uint32_t& overflow) {
+    using E = typename SK::Entry;
+    constexpr int TOTAL = SK::kTotal;
     if (s.num_nodes == 0) return false;
     float4 rlo = gld(s.nodes), rhi = gld(s.nodes + 1);
     if (COUNT) cnt.nodes += 1;
@@ -237,7 +285,7 @@ __device__ __forceinline__ bool trace_any_s_impl(const DevScene& s, F3 o, F3 d, 
             uint32_t da = __float_as_uint(alo.w), db = __float_as_uint(blo.w);
             if (ta >= 0.0f && tb >= 0.0f) {
                 uint32_t near = ta <= tb ? da : db, far = ta <= tb ? db : da;
-                if (top < TOTAL) sk.put(top++, far);
+                if (top < TOTAL) sk.put(top++, make_entry<E>(far, fmaxf(ta, tb)));
                 else overflow++;
                 cur = near;
                 continue;
@@ -245,15 +293,14 @@ __device__ __forceinline__ bool trace_any_s_impl(const DevScene& s, F3 o, F3 d, 
             if (ta >= 0.0f) { cur = da; continue; }
             if (tb >= 0.0f) { cur = db; continue; }
         }
-        if (top == 0) break;
-        cur = sk.get(--top);
+        if (!pop_live(sk, top, tmax, cur)) break;
     }
     return false;
 }
 
-template <bool COUNT, int LDS, int TOTAL>
+template <bool COUNT, class SK>
 __device__ __forceinline__ bool trace_any_s(const DevScene& s, F3 o, F3 d, float tmax,
-                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            const SK& sk, TraceCounters& cnt,
                                             uint32_t& overflow) {
     const RayInv r = make_ray_inv(o, d);
     if (finite_inv(r)) return trace_any_s_impl<true, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
@@ -277,9 +324,11 @@ __device__ __forceinline__ float wide_plane(uint32_t bytes, int i, float s, floa
 }
 
 // Slab-tests the four children of wide node nd against (r, tmax); returns how
-// many were hit and leaves their words in w[0..n) nearest first.
+// many were hit and leaves their words in w[0..n) nearest first, with their
+// entry distances' bits in k[0..n).
 template <bool FAST>
-__device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, float tmax, uint32_t (&w)[4]) {
+__device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, float tmax, uint32_t (&w)[4],
+                                             uint32_t (&k)[4]) {
     const float4 a = gld(nd), b = gld(nd + 1), c = gld(nd + 2), cw = gld(nd + 3);
     const uint32_t qlx = __float_as_uint(b.z), qly = __float_as_uint(b.w), qlz = __float_as_uint(c.x);
     const uint32_t qhx = __float_as_uint(c.y), qhy = __float_as_uint(c.z), qhz = __float_as_uint(c.w);
@@ -287,7 +336,7 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
     w[1] = __float_as_uint(cw.y);
     w[2] = __float_as_uint(cw.z);
     w[3] = __float_as_uint(cw.w);
-    uint32_t k[4];  // sort key: entry distance bits (t >= 0), or ~0 for a miss
+    // k: sort key, the entry distance's bits (t >= 0), or ~0 for a miss
     if constexpr (FAST) {
         // slab_fast with each axis's near plane picked once per node: with
         // inv >= 0 the low plane's distance is the smaller one (a decoded low
@@ -337,37 +386,38 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
     return (k[0] != 0xFFFFFFFFu) + (k[1] != 0xFFFFFFFFu) + (k[2] != 0xFFFFFFFFu) + (k[3] != 0xFFFFFFFFu);
 }
 
-// Pushes w[1..n) farthest first; returns w[0].  The wide build keeps only
-// trees whose stack need fits TOTAL entries, so no push can overflow.  While
-// three entries fit below the LDS part's end, the three slots above top are
+// Pushes w[1..n) farthest first (with their entry distances k, for stacks
+// that keep them); returns w[0].  The wide build keeps only trees whose
+// stack need fits TOTAL entries, so no push can overflow.  While three
+// entries fit below the LDS part's end, the three slots above top are
 // written unconditionally (slots at and above top hold nothing live) and top
 // advances by n - 1: one LDS address for every case.
-template <int LDS, int TOTAL>
-__device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], int n, const TraceStack<LDS, TOTAL>& sk,
-                                                 int& top) {
+template <class SK>
+__device__ __forceinline__ uint32_t wide_descend(const uint32_t (&w)[4], const uint32_t (&k)[4], int n,
+                                                 const SK& sk, int& top) {
+    using E = typename SK::Entry;
     const int m = n - 1;
-    if (top + 3 <= LDS) {
+    const E e1 = make_entry<E>(w[1], __uint_as_float(k[1])), e2 = make_entry<E>(w[2], __uint_as_float(k[2])),
+            e3 = make_entry<E>(w[3], __uint_as_float(k[3]));
+    if (top + 3 <= SK::kLds) {
         // (values, not elements picked by index: an indexed pick would put w in scratch)
-        const uint32_t w1 = w[1], w2 = w[2], w3 = w[3];
-        const uint32_t v1 = m >= 3 ? w2 : w1;
-        const uint32_t v0 = m >= 3 ? w3 : (m == 2 ? w2 : w1);
-        uint32_t* p = sk.stk + top * sk.stride;
-        p[0] = v0;
-        p[sk.stride] = v1;
-        p[2 * sk.stride] = w1;
+        const E v1 = m >= 3 ? e2 : e1;
+        const E v0 = m >= 3 ? e3 : (m == 2 ? e2 : e1);
+        sk.put3_lds(top, v0, v1, e1);
         top += m;
     } else {
-#pragma unroll
-        for (int i = 3; i >= 1; --i)
-            if (n > i) sk.put(top++, w[i]);
+        if (n > 3) sk.put(top++, e3);
+        if (n > 2) sk.put(top++, e2);
+        if (n > 1) sk.put(top++, e1);
     }
     return w[0];
 }
 
 // Closest hit over the wide BVH (node 0 is the root).
-template <bool FAST, bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_w_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, const TraceStack<LDS, TOTAL>& sk,
-                                               TraceCounters& cnt, uint32_t& overflow) {
+template <bool FAST, bool COUNT, class SK>
+__device__ __forceinline__ Hit trace_closest_w_impl(const DevScene& s, F3 o, F3 d, const RayInv& r,
+                                                    const SK& sk, TraceCounters& cnt,
+                                                    uint32_t& overflow) {
     Hit h;
     h.t = kInf;
     h.u = h.v = 0.0f;
@@ -393,22 +443,21 @@ __device__ __forceinline__ Hit trace_closest_w_impl(const DevScene& s, F3 o, F3 
                 ++i;
             }
         } else {
-            uint32_t w[4];
+            uint32_t w[4], k[4];
             if (COUNT) cnt.nodes += 2;  // 64 B: two 32 B node records
-            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, h.t, w);
+            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, h.t, w, k);
             if (n > 0) {
-                cur = wide_descend(w, n, sk, top);
+                cur = wide_descend(w, k, n, sk, top);
                 continue;
             }
         }
-        if (top == 0) break;
-        cur = sk.get(--top);
+        if (!pop_live(sk, top, h.t, cur)) break;
     }
     return h;
 }
 
-template <bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+template <bool COUNT, class SK>
+__device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, const SK& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     const RayInv r = make_ray_inv(o, d);
     if (finite_inv(r)) return trace_closest_w_impl<true, COUNT>(s, o, d, r, sk, cnt, overflow);
@@ -416,10 +465,10 @@ __device__ __forceinline__ Hit trace_closest_w(const DevScene& s, F3 o, F3 d, co
 }
 
 // Any hit in (1e-4, tmax) over the wide BVH.
-template <bool FAST, bool COUNT, int LDS, int TOTAL>
+template <bool FAST, bool COUNT, class SK>
 __device__ __forceinline__ bool trace_any_w_impl(const DevScene& s, F3 o, F3 d, const RayInv& r, float tmax,
-                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
-                                            uint32_t& overflow) {
+                                                 const SK& sk, TraceCounters& cnt,
+                                                 uint32_t& overflow) {
     if (s.num_wnodes == 0) return false;
     uint32_t cur = 0;
     int top = 0;
@@ -436,23 +485,22 @@ __device__ __forceinline__ bool trace_any_w_impl(const DevScene& s, F3 o, F3 d, 
                 ++i;
             }
         } else {
-            uint32_t w[4];
+            uint32_t w[4], k[4];
             if (COUNT) cnt.nodes += 2;
-            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, tmax, w);
+            const int n = wide_children<FAST>(s.wnodes + 4 * cur, r, tmax, w, k);
             if (n > 0) {
-                cur = wide_descend(w, n, sk, top);
+                cur = wide_descend(w, k, n, sk, top);
                 continue;
             }
         }
-        if (top == 0) break;
-        cur = sk.get(--top);
+        if (!pop_live(sk, top, tmax, cur)) break;
     }
     return false;
 }
 
-template <bool COUNT, int LDS, int TOTAL>
+template <bool COUNT, class SK>
 __device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float tmax,
-                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            const SK& sk, TraceCounters& cnt,
                                             uint32_t& overflow) {
     const RayInv r = make_ray_inv(o, d);
     if (finite_inv(r)) return trace_any_w_impl<true, COUNT>(s, o, d, r, tmax, sk, cnt, overflow);
@@ -460,32 +508,18 @@ __device__ __forceinline__ bool trace_any_w(const DevScene& s, F3 o, F3 d, float
 }
 
 // Either BVH: the wide one when WIDE.
-template <bool WIDE, bool COUNT, int LDS, int TOTAL>
-__device__ __forceinline__ Hit trace_closest_v(const DevScene& s, F3 o, F3 d, const TraceStack<LDS, TOTAL>& sk,
+template <bool WIDE, bool COUNT, class SK>
+__device__ __forceinline__ Hit trace_closest_v(const DevScene& s, F3 o, F3 d, const SK& sk,
                                                TraceCounters& cnt, uint32_t& overflow) {
     if constexpr (WIDE) return trace_closest_w<COUNT>(s, o, d, sk, cnt, overflow);
     else return trace_closest_s<COUNT>(s, o, d, sk, cnt, overflow);
 }
-template <bool WIDE, bool COUNT, int LDS, int TOTAL>
+template <bool WIDE, bool COUNT, class SK>
 __device__ __forceinline__ bool trace_any_v(const DevScene& s, F3 o, F3 d, float tmax,
-                                            const TraceStack<LDS, TOTAL>& sk, TraceCounters& cnt,
+                                            const SK& sk, TraceCounters& cnt,
                                             uint32_t& overflow) {
     if constexpr (WIDE) return trace_any_w<COUNT>(s, o, d, tmax, sk, cnt, overflow);
     else return trace_any_s<COUNT>(s, o, d, tmax, sk, cnt, overflow);
-}
-
-// The all-LDS forms (stk: this thread's first LDS slot, kStackSize entries).
-template <bool COUNT>
-__device__ __forceinline__ Hit trace_closest(const DevScene& s, F3 o, F3 d, uint32_t* stk, int stride,
-                                             TraceCounters& cnt, uint32_t& overflow) {
-    TraceStack<kStackSize> sk{stk, stride, nullptr, 0};
-    return trace_closest_s<COUNT>(s, o, d, sk, cnt, overflow);
-}
-template <bool COUNT>
-__device__ __forceinline__ bool trace_any(const DevScene& s, F3 o, F3 d, float tmax, uint32_t* stk, int stride,
-                                          TraceCounters& cnt, uint32_t& overflow) {
-    TraceStack<kStackSize> sk{stk, stride, nullptr, 0};
-    return trace_any_s<COUNT>(s, o, d, tmax, sk, cnt, overflow);
 }
 
 }  // namespace ptd

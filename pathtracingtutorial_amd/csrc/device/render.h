@@ -7,8 +7,20 @@
 
 namespace ptd {
 
+// Device scratch of the plain megakernel: the tile queue's counters (one per
+// XCD band, cleared by the launch) and the traversal stack entries beyond
+// the LDS part of every thread of its persistent grid.
+struct PlainScratch {
+    uint32_t* tile_ctr;
+    unsigned long long* spill;
+    size_t spill_entries;
+};
+// The scratch a launch on this device needs (wide: the 4-wide BVH's stack).
+hipError_t render_plain_scratch(bool wide, size_t* spill_entries, int* tile_counters);
+// wide: walk the 4-wide BVH when the scene has one (else the binary tree).
 hipError_t launch_render_plain(const DevScene& s, const RenderParams& p, float4* film_new, float4* film_acc,
-                               DevCounters* ctr, bool count, hipStream_t stream);
+                               DevCounters* ctr, bool count, bool wide, const PlainScratch& scratch,
+                               hipStream_t stream);
 hipError_t launch_probe_bsdf(const pt_material* d_mats, int n, const float* d_dirs, const int32_t* d_entering,
                              const float* d_u, float* d_out, hipStream_t stream);
 hipError_t launch_probe_sampler(int type, uint32_t seed, const int32_t* d_xyi, int n, int dims, uint32_t* d_u,

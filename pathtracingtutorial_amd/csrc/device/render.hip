@@ -2,11 +2,16 @@
 //
 // k_render_plain replaces kernelPathTrace (Part_2/src/integrators/path.cu:
 // 209-242): one thread per pixel runs spf whole paths, writes the batch mean
-// to the "new" film and running-mean accumulates into "acc".  Blocks are
-// 16x16 pixel tiles (four 64-wide waves of 16x4); the tile of a block is
-// chosen so that each XCD (blocks are dispatched round-robin over the 8 XCDs)
-// works on one contiguous band of the image and keeps its BVH working set in
-// its own L2.  The traversal stack lives in LDS: 32 entries x 256 lanes.
+// to the "new" film and running-mean accumulates into "acc".  The grid is
+// persistent (as many 256-thread blocks as fit on the GPU at once); a block
+// renders 16x16 pixel tiles (four 64-wide waves of 16x4) taken one at a time
+// from a queue.  The tiles form 8 contiguous bands of the image, one per XCD
+// (blocks are dispatched round-robin over the 8 XCDs, so block b starts on
+// band b % 8 and keeps its BVH working set in its XCD's L2); a block whose
+// band is exhausted takes tiles of the other bands, so uneven tiles balance.
+// The traversal walks the 4-wide BVH (or the binary one when the scene asks
+// for it) with a per-thread stack of (node, entry distance) entries: the
+// first kMegaLds in LDS, the rest in device memory.
 #include <hip/hip_runtime.h>
 
 #include "dev_integrator.h"
@@ -17,15 +22,15 @@ namespace ptd {
 constexpr int kTile = 16;
 constexpr int kThreads = kTile * kTile;
 constexpr int kXcds = 8;
+constexpr int kMegaLds = 16;
 
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
-    uint32_t q = nb / kXcds, r = nb % kXcds;
-    uint32_t k = b % kXcds, i = b / kXcds;
-    return k * q + min(k, r) + i;
+template <bool WIDE>
+constexpr int mega_stack_total() {
+    return WIDE ? kWideStack : kStackSize;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ void flush_counters(DevCounters* ctr, const TraceCtx& c, uint32_t paths) {
+template <bool COUNT, class Ctx>
+__device__ __forceinline__ void flush_counters(DevCounters* ctr, const Ctx& c, uint32_t paths) {
     if (!COUNT) return;
     unsigned long long v0 = wave_sum_u64(c.closest), v1 = wave_sum_u64(c.shadow), v2 = wave_sum_u64(c.cnt.nodes),
                        v3 = wave_sum_u64(c.cnt.tris), v4 = wave_sum_u64(paths), v5 = wave_sum_u64(c.overflow),
@@ -42,59 +47,138 @@ __device__ __forceinline__ void flush_counters(DevCounters* ctr, const TraceCtx&
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kThreads) void k_render_plain(DevScene s, RenderParams p, float4* __restrict__ film_new,
-                                                          float4* __restrict__ film_acc, DevCounters* ctr) {
-    __shared__ uint32_t stack[kStackSize * kThreads];
-    uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    int tx = static_cast<int>(tile % static_cast<uint32_t>(p.tiles_x));
-    int ty = static_cast<int>(tile / static_cast<uint32_t>(p.tiles_x));
-    int x = tx * kTile + static_cast<int>(threadIdx.x & (kTile - 1));
-    int y = ty * kTile + static_cast<int>(threadIdx.x / kTile);
-    TraceCtx c;
-    c.stk = stack + threadIdx.x;
-    c.stride = kThreads;
+__global__ void k_clear_u32(uint32_t* p, int n) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0u;
+}
+
+// At 4 waves per SIMD (128 VGPRs, the rest of the path state spilled to
+// scratch) the latency of the gathers hides better than at the 2 waves its
+// ~230 registers would allow (537 -> 385 ms per 1080p 64-spp frame of the
+// 1.3M-triangle sphere scene).
+template <bool COUNT, bool WIDE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_render_plain(DevScene s, RenderParams p, float4* __restrict__ film_new,
+                                                          float4* __restrict__ film_acc, DevCounters* ctr,
+                                                          uint32_t* __restrict__ tile_ctr, DistEntry* spill) {
+    __shared__ DistEntry stack[kMegaLds * kThreads];
+    __shared__ uint32_t grabbed;
+    using SK = TraceStack<kMegaLds, mega_stack_total<WIDE>(), DistEntry>;
+    TraceCtxT<WIDE, SK> c;
+    c.sk = SK{stack + threadIdx.x, kThreads, spill + blockIdx.x * kThreads + threadIdx.x, gridDim.x * kThreads};
     c.cnt.nodes = c.cnt.tris = 0;
     c.scnt.nodes = c.scnt.tris = 0;
     c.overflow = c.closest = c.shadow = 0;
     uint32_t paths = 0;
-    if (x < p.width && y < p.height) {
-        F3 acc = f3(0.0f);
-        for (int i = 0; i < p.spf; ++i) {
-            Sampler rng;
-            if (p.sampler == 1)
-                rng.init_sobol(x, y, p.frame_count * static_cast<uint32_t>(p.spf) + static_cast<uint32_t>(i), p.seed);
-            else
-                rng.init_pcg(x, y, p.frame_count, static_cast<uint32_t>(i), p.seed);
-            acc += path_trace<COUNT>(s, p, x, y, rng, c);
+    const uint32_t tiles_x = static_cast<uint32_t>(p.tiles_x);
+    const uint32_t ntiles = tiles_x * static_cast<uint32_t>((p.height + kTile - 1) / kTile);
+    const uint32_t q = ntiles / kXcds, r = ntiles % kXcds;
+    const uint32_t home = blockIdx.x % kXcds;
+    for (uint32_t k = 0; k < kXcds;) {
+        const uint32_t band = (home + k) % kXcds;
+        if (threadIdx.x == 0) grabbed = atomicAdd(&tile_ctr[band], 1u);
+        __syncthreads();
+        const uint32_t t = grabbed;  // the same for the whole block
+        __syncthreads();
+        if (t >= q + (band < r ? 1u : 0u)) {  // band exhausted: the next one
+            ++k;
+            continue;
         }
-        paths = static_cast<uint32_t>(p.spf);
-        acc /= static_cast<float>(p.spf);
-        size_t idx = static_cast<size_t>(y) * p.width + x;
-        float4 nw = make_float4(acc.x, acc.y, acc.z, 1.0f);
-        film_new[idx] = nw;
-        float4 a = film_acc[idx];
-        float inv = 1.0f / static_cast<float>(p.frame_count + 1u);
-        a.x = a.x * (1.0f - inv) + nw.x * inv;
-        a.y = a.y * (1.0f - inv) + nw.y * inv;
-        a.z = a.z * (1.0f - inv) + nw.z * inv;
-        a.w = a.w * (1.0f - inv) + nw.w * inv;
-        film_acc[idx] = a;
+        const uint32_t tile = band * q + min(band, r) + t;
+        const int x = static_cast<int>((tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1)));
+        const int y = static_cast<int>((tile / tiles_x) * kTile + threadIdx.x / kTile);
+        if (x < p.width && y < p.height) {
+            F3 acc = f3(0.0f);
+            for (int i = 0; i < p.spf; ++i) {
+                Sampler rng;
+                if (p.sampler == 1)
+                    rng.init_sobol(x, y, p.frame_count * static_cast<uint32_t>(p.spf) + static_cast<uint32_t>(i),
+                                   p.seed);
+                else
+                    rng.init_pcg(x, y, p.frame_count, static_cast<uint32_t>(i), p.seed);
+                acc += path_trace<COUNT>(s, p, x, y, rng, c);
+            }
+            paths += static_cast<uint32_t>(p.spf);
+            acc /= static_cast<float>(p.spf);
+            size_t idx = static_cast<size_t>(y) * p.width + x;
+            float4 nw = make_float4(acc.x, acc.y, acc.z, 1.0f);
+            film_new[idx] = nw;
+            float4 a = film_acc[idx];
+            float inv = 1.0f / static_cast<float>(p.frame_count + 1u);
+            a.x = a.x * (1.0f - inv) + nw.x * inv;
+            a.y = a.y * (1.0f - inv) + nw.y * inv;
+            a.z = a.z * (1.0f - inv) + nw.z * inv;
+            a.w = a.w * (1.0f - inv) + nw.w * inv;
+            film_acc[idx] = a;
+        }
     }
     flush_counters<COUNT>(ctr, c, paths);
 }
 
+namespace {
+
+template <bool COUNT, bool WIDE>
+const void* plain_kernel() {
+    return reinterpret_cast<const void*>(&k_render_plain<COUNT, WIDE>);
+}
+
+hipError_t plain_blocks(bool wide, int* blocks) {
+    int dev = 0, cus = 0, per_cu = 0, per_cu_c = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // the grid serves the counting and the plain kernel: the smaller occupancy of the two
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wide ? plain_kernel<false, true>()
+                                                                       : plain_kernel<false, false>(),
+                                                         kThreads, 0);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_c, wide ? plain_kernel<true, true>()
+                                                                         : plain_kernel<true, false>(),
+                                                         kThreads, 0);
+    if (e != hipSuccess) return e;
+    const int n = std::max(1, std::min(per_cu, per_cu_c)) * std::max(1, cus);
+    *blocks = (n + kXcds - 1) / kXcds * kXcds;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t render_plain_scratch(bool wide, size_t* spill_entries, int* tile_counters) {
+    int blocks = 0;
+    hipError_t e = plain_blocks(wide, &blocks);
+    if (e != hipSuccess) return e;
+    const int total = wide ? kWideStack : kStackSize;
+    *spill_entries = static_cast<size_t>(blocks) * kThreads * static_cast<size_t>(total - kMegaLds);
+    *tile_counters = kXcds;
+    return hipSuccess;
+}
+
 hipError_t launch_render_plain(const DevScene& s, const RenderParams& p, float4* film_new, float4* film_acc,
-                               DevCounters* ctr, bool count, hipStream_t stream) {
-    uint32_t tiles_x = static_cast<uint32_t>((p.width + kTile - 1) / kTile);
-    uint32_t tiles_y = static_cast<uint32_t>((p.height + kTile - 1) / kTile);
+                               DevCounters* ctr, bool count, bool wide, const PlainScratch& scratch,
+                               hipStream_t stream) {
+    if (wide && (s.wnodes == nullptr || s.num_wnodes == 0)) wide = false;
+    int blocks = 0;
+    hipError_t e = plain_blocks(wide, &blocks);
+    if (e != hipSuccess) return e;
+    const int total = wide ? kWideStack : kStackSize;
+    const size_t need = static_cast<size_t>(blocks) * kThreads * static_cast<size_t>(total - kMegaLds);
+    if (scratch.tile_ctr == nullptr || scratch.spill == nullptr || scratch.spill_entries < need)
+        return hipErrorInvalidValue;
     RenderParams q = p;
-    q.tiles_x = static_cast<int32_t>(tiles_x);
-    dim3 grid(tiles_x * tiles_y), block(kThreads);
-    if (count)
-        hipLaunchKernelGGL(k_render_plain<true>, grid, block, 0, stream, s, q, film_new, film_acc, ctr);
+    q.tiles_x = (p.width + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, stream, scratch.tile_ctr, kXcds);
+    const dim3 grid(static_cast<uint32_t>(blocks)), block(kThreads);
+    if (count && wide)
+        hipLaunchKernelGGL((k_render_plain<true, true>), grid, block, 0, stream, s, q, film_new, film_acc, ctr,
+                           scratch.tile_ctr, scratch.spill);
+    else if (count)
+        hipLaunchKernelGGL((k_render_plain<true, false>), grid, block, 0, stream, s, q, film_new, film_acc, ctr,
+                           scratch.tile_ctr, scratch.spill);
+    else if (wide)
+        hipLaunchKernelGGL((k_render_plain<false, true>), grid, block, 0, stream, s, q, film_new, film_acc, ctr,
+                           scratch.tile_ctr, scratch.spill);
     else
-        hipLaunchKernelGGL(k_render_plain<false>, grid, block, 0, stream, s, q, film_new, film_acc, ctr);
+        hipLaunchKernelGGL((k_render_plain<false, false>), grid, block, 0, stream, s, q, film_new, film_acc, ctr,
+                           scratch.tile_ctr, scratch.spill);
     return hipGetLastError();
 }
 
@@ -126,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(DevScene s, const float* _
                                                       float* __restrict__ t_out, int32_t* __restrict__ face_out,
                                                       float* __restrict__ bary_out) {
     constexpr int kTot = WIDE ? kWideStack : kStackSize;
-    __shared__ uint32_t stack[kTot * BLOCK];
+    __shared__ DistEntry stack[kTot * BLOCK];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays8 + 8 * static_cast<size_t>(i);
@@ -134,7 +218,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_rays(DevScene s, const float* _
     F3 d = normalize_rn(f3(r[3], r[4], r[5]));  // Ray's constructor (core/ray.h:13)
     TraceCounters cnt{0, 0};
     uint32_t ovf = 0;
-    const TraceStack<kTot, kTot> sk{stack + threadIdx.x, BLOCK, nullptr, 0};
+    const TraceStack<kTot, kTot, DistEntry> sk{stack + threadIdx.x, BLOCK, nullptr, 0};
     if (any) {
         bool occ = trace_any_v<WIDE, false>(s, o, d, r[6], sk, cnt, ovf);
         t_out[i] = occ ? 1.0f : 0.0f;

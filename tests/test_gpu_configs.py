@@ -1,8 +1,8 @@
 """The large workloads on the GPU against the oracle, through the C ABI.
 
-C1 (`generate(1)`: 1,310,722 triangles, 1920x1080, 64 spp, depth 16, Sobol,
-a 2048x1024 sky, a BVH of 32 bins and <= 4 triangles per leaf) and C2 (the
-batched `generate(4)` jobs, seeds 1000..1063).
+The sphere scene (`generate(1)`: 1,310,722 triangles, 1920x1080, 64 spp,
+depth 16, Sobol, a 2048x1024 sky, a BVH of 32 bins and <= 4 triangles per
+leaf) and the batch jobs (`generate(4)`, seeds 1000..1063).
 
 Both integrators draw the same random numbers per path as the oracle
 (kernelPathTrace's seeding, path.cu:216-228), so a pixel's batch mean is the
@@ -33,7 +33,7 @@ def frac_differ(g, o, tol):
 
 
 @pytest.fixture(scope="module")
-def c1():
+def spheres():
     sm = pt.SceneManager()
     sm.generate(1, 7)
     sm.buildAccel()  # the workload's own options: 32 bins, <= 4 triangles per leaf
@@ -42,8 +42,8 @@ def c1():
     return sm, ora
 
 
-def test_c1_workload_shape(c1):
-    sm, _ = c1
+def test_spheres_workload_shape(spheres):
+    sm, _ = spheres
     i = sm.info()
     assert i.num_faces == 1310722 and (i.width, i.height) == (1920, 1080)
     assert (i.sample_per_frame, i.max_depth, i.sampler) == (64, 16, 1)
@@ -62,8 +62,8 @@ def _sah(nodes):
     return float((area.astype(np.float64) * cost).sum() / area[0])
 
 
-def test_c1_tree_invariants_and_sah(c1):
-    sm, ora = c1
+def test_spheres_tree_invariants_and_sah(spheres):
+    sm, ora = spheres
     nodes, order = sm.bvh(0)
     info = sm.info()
     assert nodes.shape[0] == info.num_bvh_nodes - 1  # the device layout keeps slot 1 unused
@@ -107,8 +107,8 @@ def _random_rays(n, seed, box=12.0):
     return rays
 
 
-def test_c1_random_rays_wide_binary_and_oracle(c1):
-    sm, ora = c1
+def test_spheres_random_rays_wide_binary_and_oracle(spheres):
+    sm, ora = spheres
     rays = _random_rays(20000, 3)
     tb, fb, _ = pt.trace_rays(sm, rays)
     tw, fw, _ = pt.trace_rays(sm, rays, wide=True)
@@ -122,9 +122,9 @@ def test_c1_random_rays_wide_binary_and_oracle(c1):
     assert np.array_equal(ob, oo) and np.array_equal(ow, oo)
 
 
-def test_c1_env_tables_are_the_references(c1):
+def test_spheres_env_tables_are_the_references(spheres):
     # the 2048x1024 environment's pmf and alias table (light_sampler.cpp:112-149)
-    sm, ora = c1
+    sm, ora = spheres
     g, o = sm.lightTables(), ora.light_tables()
     assert g["env_pmf"].size == 2048 * 1024
     assert np.array_equal(g["env_pmf"], o["env_pmf"]) and np.array_equal(g["env_alias"], o["env_alias"])
@@ -134,9 +134,9 @@ TILES = [(928, 992, 508, 572), (200, 264, 100, 164), (1700, 1764, 900, 964)]
 
 
 @pytest.mark.parametrize("integ", ["path", "wavefront"])
-def test_c1_frame_tiles_match_oracle(c1, integ):
+def test_spheres_frame_tiles_match_oracle(spheres, integ):
     # the full 64-spp frame (the wavefront schedule runs it in two chunks of 2^26 paths)
-    sm, ora = c1
+    sm, ora = spheres
     info, cam = sm.info(), sm.camera()
     sm.uploadToDevice()
     r = pt.Renderer(sm, pt.PathIntegrator() if integ == "path" else pt.WavefrontIntegrator())
@@ -148,14 +148,14 @@ def test_c1_frame_tiles_match_oracle(c1, integ):
         o, _ = ora.render(cam, info.sample_per_frame, info.max_depth, info.sampler, info.seed, frame=0, rect=rect)
         g = img[y0:y1, x0:x1]
         f4, f3 = frac_differ(g, o, 1e-4), frac_differ(g, o, 1e-3)
-        print(f"C1 {integ} tile {rect}: {f4:.3%} of channels beyond 1e-4, {f3:.3%} beyond 1e-3")
+        print(f"spheres {integ} tile {rect}: {f4:.3%} of channels beyond 1e-4, {f3:.3%} beyond 1e-3")
         assert f4 <= 0.03 and f3 <= 0.01
         assert abs(g[..., :3].mean() - o.mean()) <= 1e-3 * o.mean()
 
 
-def test_c1_single_paths_match_oracle(c1):
+def test_spheres_single_paths_match_oracle(spheres):
     # one sample per pixel: each pixel is one path
-    sm, ora = c1
+    sm, ora = spheres
     info, cam = sm.info(), sm.camera()
     sm.setSamplePerFrame(1)
     try:
@@ -168,15 +168,15 @@ def test_c1_single_paths_match_oracle(c1):
                 x0, x1, y0, y1 = rect
                 o, _ = ora.render(cam, 1, info.max_depth, info.sampler, info.seed, frame=0, rect=rect)
                 f4 = frac_differ(img[y0:y1, x0:x1], o, 1e-4)
-                print(f"C1 1 spp {integ.name()} tile {rect}: {f4:.3%} of channels beyond 1e-4")
+                print(f"spheres 1 spp {integ.name()} tile {rect}: {f4:.3%} of channels beyond 1e-4")
                 assert f4 <= 0.01
     finally:
         sm.setSamplePerFrame(64)
 
 
-def test_c1_tile_means_within_3_sigma(c1):
+def test_spheres_tile_means_within_3_sigma(spheres):
     # paired batches of 8 samples: the per-pixel means agree within 3 sigma
-    sm, ora = c1
+    sm, ora = spheres
     info, cam = sm.info(), sm.camera()
     sm.setSamplePerFrame(8)
     try:
@@ -194,13 +194,13 @@ def test_c1_tile_means_within_3_sigma(c1):
         sm.setSamplePerFrame(64)
 
 
-# ------------------------------------------------------------------------ C2
-C2_SEEDS = [1047, 1046, 1051, 1003, 1055, 1000, 1007, 1008, 1009]  # most triangles, most pixels, most paths, ...
+# ------------------------------------------------------------------ batch jobs
+JOB_SEEDS = [1047, 1046, 1051, 1003, 1055, 1000, 1007, 1008, 1009]  # most triangles, most pixels, most paths, ...
 
 
-def test_c2_jobs_equal_single_renders_and_match_oracle():
+def test_batch_jobs_equal_single_renders_and_match_oracle():
     scenes = []
-    for s in C2_SEEDS:
+    for s in JOB_SEEDS:
         sm = pt.SceneManager()
         sm.generate(4, s)
         sm.buildAccel()
@@ -216,7 +216,7 @@ def test_c2_jobs_equal_single_renders_and_match_oracle():
         sm.uploadToDevice()
     rs = [pt.Renderer(sm, pt.WavefrontIntegrator()) for sm in scenes]
     assert pt.render_jobs(rs) > 0
-    for s, sm, r, single, info in zip(C2_SEEDS, scenes, rs, singles, infos):
+    for s, sm, r, single, info in zip(JOB_SEEDS, scenes, rs, singles, infos):
         img = r.downloadCurrentImage(1)
         assert np.array_equal(img, single), s  # the batched schedule renders each job as on its own
         ora = oracle.OracleScene.from_view(host_view(sm))
@@ -227,6 +227,6 @@ def test_c2_jobs_equal_single_renders_and_match_oracle():
         o, _ = ora.render(cam, info.sample_per_frame, info.max_depth, info.sampler, info.seed, frame=0, rect=rect)
         g = img[y0:y0 + h, x0:x0 + w]
         f4, f3 = frac_differ(g, o, 1e-4), frac_differ(g, o, 1e-3)
-        print(f"C2 seed {s} ({info.num_faces} tris, {info.width}x{info.height}, {info.sample_per_frame} spp): "
+        print(f"job seed {s} ({info.num_faces} tris, {info.width}x{info.height}, {info.sample_per_frame} spp): "
               f"{f4:.3%} beyond 1e-4, {f3:.3%} beyond 1e-3")
         assert f4 <= 0.04 and f3 <= 0.015

@@ -87,8 +87,8 @@ def test_views_autograd_shapes():
     assert env.grad is not None and env.grad.shape == env.shape and env.grad.abs().sum() > 0
 
 
-def test_c4_views_adjoint_matches_host_reference_adjoint():
-    # The C4 workload (generate(5): 2,048,002 triangles, 16 ring cameras at
+def test_ring_views_adjoint_matches_host_reference_adjoint():
+    # The 16-view workload (generate(5): 2,048,002 triangles, 16 ring cameras at
     # 512^2, depth 10, gradients of the 1024^2 ground texture and the
     # 1024x512 environment) at 2 spp, all 16 views in one adjoint schedule.
     # The weights sit on three 24x24 tiles (ground, spheres, sky) of four
@@ -135,9 +135,9 @@ def test_c4_views_adjoint_matches_host_reference_adjoint():
     # the 1024^2 ground texture is seen from afar: most touched texels get one
     # or two lookups, so a 1/256 filter-weight step (see compare_arrays) shows
     # on a larger share of them than in the close-up Cornell texture
-    rt = compare_arrays("C4 texture", out["texture"].cpu().numpy()[:, :3], tex[:, :3], frac=0.95)
-    re = compare_arrays("C4 env", out["env"].cpu().numpy()[..., :3], env[..., :3], frac=0.99)
-    print(f"C4 FP32 (GPU atomics) vs FP64 (host): max |d| / max |g|: texture {rt:.2e}, env {re:.2e}")
+    rt = compare_arrays("ring views texture", out["texture"].cpu().numpy()[:, :3], tex[:, :3], frac=0.95)
+    re = compare_arrays("ring views env", out["env"].cpu().numpy()[..., :3], env[..., :3], frac=0.99)
+    print(f"ring views FP32 (GPU atomics) vs FP64 (host): max |d| / max |g|: texture {rt:.2e}, env {re:.2e}")
     assert (np.abs(tex[:, :3]) > 0).sum() > 1000 and (np.abs(env[..., :3]) > 0).sum() > 1000
     # central differences of the oracle's detached estimator on 32 scalars
     flat_t = np.argsort(-np.abs(tex[:, :3]).reshape(-1))[:16]

@@ -159,8 +159,13 @@ def test_wavefront_counters_match_megakernel():
     _render(sm, pt.WavefrontIntegrator(), 1)
     b = sm.counters()
     assert b["paths"] == a["paths"] == 64 * 64 * 2
-    for k in ("closest_rays", "shadow_rays", "node_visits", "tri_tests"):
+    for k in ("closest_rays", "shadow_rays"):
         assert abs(a[k] - b[k]) <= 0.01 * a[k] + 10, (k, a[k], b[k])
+    # the megakernel's closest-hit stack keeps entry distances and drops a popped
+    # node beyond the hit found since (as the reference does at its pop); the
+    # wavefront's does not: the same rays, fewer node and triangle fetches
+    for k in ("node_visits", "tri_tests"):
+        assert 0.8 * b[k] <= a[k] <= b[k] + 10, (k, a[k], b[k])
     assert b["stack_overflows"] == 0
 
 
