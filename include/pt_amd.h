@@ -192,7 +192,7 @@ enum { PT_INTEGRATOR_PATH = 0, PT_INTEGRATOR_WAVEFRONT = 1 };
  * bvh_width 4 a ray whose closest hit is an exact-distance tie between
  * triangles may take the other one). */
 typedef struct pt_wavefront_config {
-    int32_t chunk_spp;        /* paths in flight = chunk_spp x all jobs' pixels, at most 2^26 (default 64) */
+    int32_t chunk_spp;        /* paths in flight = chunk_spp x all jobs' pixels, at most 2^28 (default 64) */
     int32_t extend_block;     /* threads per block of the traversal kernels: 64, 128 or 256 */
     int32_t shade_block;      /* threads per block of the shading kernel: 64, 128 or 256 */
     int32_t waves_per_simd;   /* persistent grid: waves per SIMD, 0 = occupancy limit */

@@ -13,7 +13,7 @@
 namespace ptd {
 
 struct WavefrontConfig {
-    int chunk_spp = 64;         // paths in flight = chunk_spp x (pixels of all jobs), at most 2^26
+    int chunk_spp = 64;         // paths in flight = chunk_spp x (pixels of all jobs), at most 2^28
     int extend_block = 128;     // threads per block of the traversal kernels
     int shade_block = 128;      // threads per block of the shading kernel
     int waves_per_simd = 0;     // persistent grid size: 0 = from the occupancy calculator

@@ -1737,10 +1737,6 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
         return hipErrorInvalidValue;
     }
     const unsigned long long want = std::max(1ull, static_cast<unsigned long long>(std::max(1, cfg.chunk_spp)) * pixels);
-    // at most 2^26 paths in flight (~15 GB of path state and queues)
-    const unsigned long long cap = std::min<unsigned long long>(1ull << 26, want);
-    const uint32_t C = static_cast<uint32_t>(std::min(cap, paths));
-    const int nchunks = static_cast<int>((paths + C - 1) / C);
     if (wf->num_cus == 0) {
         int dev = 0;
         (void)hipGetDevice(&dev);
@@ -1749,11 +1745,25 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
             wf->num_cus = 256;
     }
     const uint32_t njobs = static_cast<uint32_t>(jobs.size());
-    if (!wf->ensure(C, static_cast<uint32_t>(pixels), max_lights, max_mat, njobs) ||
-        (G && !wf->ensure_adjoint(C))) {
-        if (err) *err = "wavefront: out of device memory for path state";
-        return hipErrorOutOfMemory;
+    // At most 2^28 paths in flight (~56 GB of path state and queues, the
+    // adjoint's replay state ~30 GB more): a 1080p frame of 64 samples runs as
+    // one chunk, whose bounce loop drains once (every chunk ends in a tail of
+    // nearly empty launches).  A device that cannot hold that many gets half
+    // as many, down to 2^22.
+    unsigned long long cap = std::min<unsigned long long>(1ull << 28, want);
+    uint32_t C = 0;
+    while (true) {
+        C = static_cast<uint32_t>(std::min(cap, paths));
+        if (wf->ensure(C, static_cast<uint32_t>(pixels), max_lights, max_mat, njobs) &&
+            (!G || wf->ensure_adjoint(C)))
+            break;
+        if (cap <= (1ull << 22)) {
+            if (err) *err = "wavefront: out of device memory for path state";
+            return hipErrorOutOfMemory;
+        }
+        cap /= 2;
     }
+    const int nchunks = static_cast<int>((paths + C - 1) / C);
     wf->st.njobs = njobs;
     wf->st.chunk_paths = C;
     wf->st.grab_extend = static_cast<uint32_t>(cfg.grab_extend);

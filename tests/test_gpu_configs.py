@@ -133,14 +133,19 @@ def test_spheres_env_tables_are_the_references(spheres):
 TILES = [(928, 992, 508, 572), (200, 264, 100, 164), (1700, 1764, 900, 964)]
 
 
-@pytest.mark.parametrize("integ", ["path", "wavefront"])
+@pytest.mark.parametrize("integ", ["path", "wavefront", "wavefront-chunked"])
 def test_spheres_frame_tiles_match_oracle(spheres, integ):
-    # the full 64-spp frame (the wavefront schedule runs it in two chunks of 2^26 paths)
+    # the full 64-spp frame: the wavefront schedule holds its 132.7M paths in
+    # one chunk, or (chunk_spp 16) walks them in four
     sm, ora = spheres
     info, cam = sm.info(), sm.camera()
     sm.uploadToDevice()
+    cfg = sm.wavefrontConfig()
+    if integ == "wavefront-chunked":
+        sm.setWavefrontConfig(chunk_spp=16)
     r = pt.Renderer(sm, pt.PathIntegrator() if integ == "path" else pt.WavefrontIntegrator())
     r.renderBatch()
+    sm.setWavefrontConfig(**cfg)
     img = r.downloadCurrentImage(1)
     assert np.isfinite(img).all() and img[..., :3].max() <= 100.0
     for rect in TILES:
