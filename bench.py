@@ -46,6 +46,26 @@ def algorithmic_bytes(c: dict) -> int:
     return 32 * c["node_visits"] + 36 * c["tri_tests"] + 48 * rays
 
 
+# Bytes the wavefront shading kernel moves per counted event (the workload has
+# an environment light and no emitting triangles, so a light sample is the
+# environment's): a shaded item reads its queue entry (4), path record (64)
+# and hit (16); a continuing path's record is written back (64, counted with
+# the continuation); an escape reads 4 bilinear
+# environment texels (64) and the texel's pmf (4, the MIS weight); a surface
+# hit reads the face (16), three vertex normals (48) and uvs (24) and the
+# 256-byte material record; a light sample reads the alias entry (8), the
+# pmf (4) and 4 texels (64); a shadow-queue entry is 48 bytes written, a
+# queued continuation 4 + its record 64, a radiance update 16 read + 16
+# written.
+SHADE_EVENT_BYTES = {"shade_items": 4 + 64 + 16, "shade_env_lookups": 64 + 4, "shade_hits": 16 + 48 + 24 + 256,
+                     "shade_light_samples": 8 + 4 + 64, "shade_shadow_entries": 48, "shade_continuations": 4 + 64,
+                     "shade_radiance_adds": 32}
+
+
+def shade_bytes(c: dict) -> int:
+    return sum(n * c[k] for k, n in SHADE_EVENT_BYTES.items())
+
+
 def profiled_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
     summaries (profiles/*.md, FETCH_SIZE + WRITE_SIZE runs scaled by the copy-kernel
@@ -348,6 +368,15 @@ def main(argv=None) -> int:
                    "tris_per_ray": round(closest_tris / max(1, c["closest_rays"]), 2)}
         extra = {"stages_ms_per_frame": {k: round(v["ms"], 2) for k, v in st.items()},
                  "launches_per_frame": ext["launches"]}
+        sh = st["shade"]
+        sh_s = sh["ms"] / max(1, sh["launches"]) / 1000.0
+        sh_bytes = shade_bytes(c) / max(1, sh["launches"] * a.steps)
+        sh_ach = sh_bytes / sh_s / 1e9
+        shade_roofline = {"bound": "hbm", "kernel": "k_wf_shade", "achieved": round(sh_ach, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sh_ach / HBM_PEAK_GBS, 4),
+                          "traffic": profiled_traffic("k_wf_shade", WORKLOAD), "kernel_ms": round(sh_s * 1000, 3),
+                          "bytes_per_launch": int(sh_bytes),
+                          "events_per_frame": {k: c[k] // a.steps for k in SHADE_EVENT_BYTES}}
     else:
         st = None
         kernel_s = (sum(R["kernel_ms"]) / len(R["kernel_ms"])) / 1000.0
@@ -386,6 +415,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": int(bytes_per_launch), **per_ray, **extra},
     }
     if head == "wavefront":
+        out["shade_roofline"] = shade_roofline
         out["config"]["wavefront"] = sm.wavefrontConfig()
         out["config"]["wavefront_autotuned"] = tuned is not None
     if "path" in runs and head != "path":

@@ -178,6 +178,15 @@ typedef struct pt_counters {
     uint64_t stack_overflows;
     uint64_t shadow_node_visits; /* the any-hit (shadow ray) share of node_visits */
     uint64_t shadow_tri_tests;   /* the any-hit share of tri_tests */
+    /* events of the wavefront integrator's shading kernel (zero for the
+     * megakernel), from which its algorithmic bytes follow: */
+    uint64_t shade_items;        /* closest-hit results shaded */
+    uint64_t shade_env_lookups;  /* escaped rays that read the environment */
+    uint64_t shade_hits;         /* surface hits (interpolated surface + material read) */
+    uint64_t shade_light_samples;  /* shading points that drew the light samples (NEE) */
+    uint64_t shade_shadow_entries; /* shadow-queue entries written */
+    uint64_t shade_continuations;  /* rays queued for the next bounce */
+    uint64_t shade_radiance_adds;  /* read-modify-writes of a path's radiance sum */
 } pt_counters;
 
 typedef struct pt_scene pt_scene;

@@ -148,7 +148,10 @@ class SceneInfo(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("paths", C.c_uint64), ("stack_overflows", C.c_uint64),
-                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64)]
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("shade_items", C.c_uint64), ("shade_env_lookups", C.c_uint64), ("shade_hits", C.c_uint64),
+                ("shade_light_samples", C.c_uint64), ("shade_shadow_entries", C.c_uint64),
+                ("shade_continuations", C.c_uint64), ("shade_radiance_adds", C.c_uint64)]
 
 
 class WavefrontConfig(C.Structure):

@@ -1494,6 +1494,13 @@ pt_status pt_counters_get(pt_scene* s, pt_counters* out) {
         out->stack_overflows = c.stack_overflows;
         out->shadow_node_visits = c.shadow_node_visits;
         out->shadow_tri_tests = c.shadow_tri_tests;
+        out->shade_items = c.shade_items;
+        out->shade_env_lookups = c.shade_env_lookups;
+        out->shade_hits = c.shade_hits;
+        out->shade_light_samples = c.shade_light_samples;
+        out->shade_shadow_entries = c.shade_shadow_entries;
+        out->shade_continuations = c.shade_continuations;
+        out->shade_radiance_adds = c.shade_radiance_adds;
     });
 }
 

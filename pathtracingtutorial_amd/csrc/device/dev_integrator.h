@@ -488,20 +488,27 @@ struct InlineNee {
 // current ray: returns true when the path continues with the new ray in st.
 // NEE contributions are premultiplied by the throughput and handed to the
 // visibility policy together with their shadow rays.
+// A ray that escaped (path.cu:71-82): the environment's radiance, MIS-weighted
+// against the environment sampling after a non-delta bounce; the path ends.
+template <class Nee>
+__device__ __forceinline__ void escape(const DevScene& s, PathState& st, Nee& nee) {
+    float eu = 0.0f, ev = 0.0f;
+    if (s.env != nullptr) dir_to_uv(st.d, eu, ev);
+    F3 env = env_radiance_uv(s, eu, ev);
+    if (!is_zero(env)) {
+        float mis = 1.0f;
+        if (st.depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf_uv(s, st.d, eu, ev));
+        nee.on_env_miss(s, st, mis);
+        st.lo += st.tp * env * mis;
+    }
+}
+
 template <class Nee>
 __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p, const Hit& h, PathState& st,
                                        Sampler& rng, Nee& nee) {
     const int depth = st.depth;
     if (h.tri == 0xFFFFFFFFu) {
-        float eu = 0.0f, ev = 0.0f;
-        if (s.env != nullptr) dir_to_uv(st.d, eu, ev);
-        F3 env = env_radiance_uv(s, eu, ev);
-        if (!is_zero(env)) {
-            float mis = 1.0f;
-            if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf_uv(s, st.d, eu, ev));
-            nee.on_env_miss(s, st, mis);
-            st.lo += st.tp * env * mis;
-        }
+        escape(s, st, nee);
         return false;
     }
     const bool has_area = s.num_emissive > 0 && s.emissive != nullptr;

@@ -73,6 +73,9 @@ struct RenderParams {
 struct DevCounters {
     unsigned long long closest_rays, shadow_rays, node_visits, tri_tests, paths, stack_overflows;
     unsigned long long shadow_node_visits, shadow_tri_tests;  // the any-hit share of node_visits / tri_tests
+    // the wavefront shading kernel's events (pt_counters)
+    unsigned long long shade_items, shade_env_lookups, shade_hits, shade_light_samples, shade_shadow_entries,
+        shade_continuations, shade_radiance_adds;
 };
 
 }  // namespace ptd

@@ -14,8 +14,8 @@ namespace ptd {
 
 struct WavefrontConfig {
     int chunk_spp = 64;         // paths in flight = chunk_spp x (pixels of all jobs), at most 2^28
-    int extend_block = 128;     // threads per block of the traversal kernels
-    int shade_block = 128;      // threads per block of the shading kernel
+    int extend_block = 64;      // threads per block of the traversal kernels
+    int shade_block = 256;      // threads per block of the shading kernel
     int waves_per_simd = 0;     // persistent grid size: 0 = from the occupancy calculator
     int sort_by_material = 1;   // counting-sort the shading queue by material
     int use_graph = 1;          // capture the bounce loop of a chunk in a hipGraph
@@ -25,7 +25,7 @@ struct WavefrontConfig {
     // queue items a wave takes per atomic (64 .. 1024, a power of two) in the
     // closest-hit, shading and shadow kernels: larger grabs cost fewer
     // same-address atomics, smaller ones keep the rays in flight coherent
-    int grab_extend = 128;
+    int grab_extend = 256;
     int grab_shade = 1024;
     int grab_shadow = 64;
     // 4: traverse the 4-wide BVH with quantized child boxes when every job's

@@ -47,11 +47,12 @@ constexpr int C_SACNT = 16;  // [8] area shadow queue segment counts
 constexpr int C_SECNT = 24;  // [8] environment shadow queue segment counts
 constexpr int C_SP = 32;     // punctual shadow queue count (one flat queue)
 constexpr int C_CHUNK = 33;  // chunk index within the batch
-constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72;
-// per cursor set (5 of them, in the order above): a bit mask of drained
+constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72, C_CUR_MISS = 80;
+constexpr int C_MISS0 = 88;  // [8] first item of each segment's escape bin in the material-sorted queue
+// per cursor set (6 of them, in the order above): a bit mask of drained
 // segments, each mask on a 128-byte line of its own
 constexpr int C_DRAIN = 128, C_DRAIN_STRIDE = 32;
-constexpr int C_WORDS = C_DRAIN + 5 * C_DRAIN_STRIDE;
+constexpr int C_WORDS = C_DRAIN + 6 * C_DRAIN_STRIDE;
 
 // A job as the kernels see it (device memory, one entry per job).
 struct JobDesc {
@@ -98,6 +99,7 @@ struct WfState {
     float4 *sa_x, *se_x, *sp_x;   // kPayload per shadow entry: its NEE sensitivities
     const AdjointGrads* grads;    // parameter gradient buffers (device copy)
     const JobDesc* jobs;
+    DevCounters* shade_ctr;  // the shading kernel's event counters (counted runs only, else null)
     uint32_t njobs;
     uint32_t capacity, sp_capacity;
     uint32_t chunk_paths;              // C
@@ -171,14 +173,18 @@ struct QueueWalk {
     uint32_t* cursors;
     uint32_t* drained;
     const uint32_t* counts;
+    const uint32_t* starts;  // per segment: its first item instead of base(k) (or null)
+    const uint32_t* stops;   // per segment: its end instead of base(k) + counts[k] (or null)
     uint32_t n, step;
     int seg, tried;
     uint32_t cur, lim;  // the wave's current grab [cur, lim)
 
     __device__ __forceinline__ QueueWalk(uint32_t* ctrl, int cur_slot, const uint32_t* cnt, uint32_t items,
-                                         uint32_t grab)
+                                         uint32_t grab, const uint32_t* seg_starts = nullptr,
+                                         const uint32_t* seg_stops = nullptr)
         : cursors(ctrl + cur_slot), drained(ctrl + C_DRAIN + C_DRAIN_STRIDE * ((cur_slot - C_CUR_EXT) / kSeg)),
-          counts(cnt), n(items), step(grab), seg(static_cast<int>(blockIdx.x % kSeg)), tried(0), cur(0), lim(0) {}
+          counts(cnt), starts(seg_starts), stops(seg_stops), n(items), step(grab),
+          seg(static_cast<int>(blockIdx.x % kSeg)), tried(0), cur(0), lim(0) {}
 
     __device__ __forceinline__ void skip() {
         seg = (seg + 1) % kSeg;
@@ -196,8 +202,9 @@ struct QueueWalk {
                 while (tried < kSeg && ((m >> seg) & 1u)) skip();
             }
             if (tried >= kSeg) return false;
-            const uint32_t s0 = seg_base(n, seg);
-            const uint32_t s1 = counts ? s0 + counts[seg] : seg_base(n, seg + 1);
+            const uint32_t b = seg_base(n, seg);
+            const uint32_t s0 = starts ? starts[seg] : b;
+            const uint32_t s1 = stops ? stops[seg] : (counts ? b + counts[seg] : seg_base(n, seg + 1));
             if (s1 <= s0) {
                 skip();
                 continue;
@@ -250,14 +257,19 @@ __device__ __forceinline__ void store_path(const WfState& w, uint32_t p, const P
 // (an emitter hit or an escape, no more than once per bounce) goes onto the
 // path's sum, and a bounce that added none leaves the sum untouched.  Each
 // path is in a queue once, and the shadow kernels that also add to the sum
-// run in other launches.
-__device__ __forceinline__ void store_path_added(const WfState& w, uint32_t p, const PathState& st, uint4 r) {
-    PathRec& R = w.path[p];
-    R.o = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
-    R.d = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
-    uint32_t bits = static_cast<uint32_t>(st.depth) | (st.last_delta ? 0x80000000u : 0u);
-    R.tp = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
-    R.rng = r;
+// run in other launches.  The record of a path that ends here is dead (the
+// per-pixel sums and the adjoint's seed read only the sum; a replay starts
+// from fresh camera rays): it is written only for a continuing path.
+__device__ __forceinline__ void store_path_added(const WfState& w, uint32_t p, const PathState& st, uint4 r,
+                                                 bool cont) {
+    if (cont) {
+        PathRec& R = w.path[p];
+        R.o = make_float4(st.o.x, st.o.y, st.o.z, 0.0f);
+        R.d = make_float4(st.d.x, st.d.y, st.d.z, st.last_pdf);
+        uint32_t bits = static_cast<uint32_t>(st.depth) | (st.last_delta ? 0x80000000u : 0u);
+        R.tp = make_float4(st.tp.x, st.tp.y, st.tp.z, __uint_as_float(bits));
+        R.rng = r;
+    }
     if (!is_zero(st.lo)) {
         const float4 l = w.lo[p];
         w.lo[p] = make_float4(l.x + st.lo.x, l.y + st.lo.y, l.z + st.lo.z, 0.0f);
@@ -463,8 +475,10 @@ __global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins)
     }
     __syncthreads();
     uint32_t acc = part_sum[t];
+    const uint32_t e_miss = (nbins - 1) * kSortParts;  // the escape bin (the last), part 0
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t v = at(e);
+        if (e == e_miss) w.ctrl[C_MISS0 + seg] = acc;
         at(e) = acc;
         acc += v;
     }
@@ -490,6 +504,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot,
 // segment the path's wave is working on; punctual rays with one atomic each.
 struct DeferNee {
     static constexpr bool kDrawAhead = true;
+    uint32_t light_samples = 0, entries = 0;  // events, for the counted runs
     uint32_t path;
     uint32_t base;  // first slot of the segment in the shadow queues
     uint32_t *sa_count, *se_count;
@@ -500,6 +515,8 @@ struct DeferNee {
     __device__ __forceinline__ void direct(const DevScene&, PathState&, bool a, const ShadowRay& ra, F3 xa, bool e,
                                            const ShadowRay& re, F3 xe, const LightSample&, const LightSample&,
                                            const ShadeCtx&) {
+        light_samples += 1;
+        entries += (a ? 1u : 0u) + (e ? 1u : 0u);
         const uint32_t ka = base + wave_append(sa_count, a);
         if (a) {
             sa_o[ka] = make_float4(ra.o.x, ra.o.y, ra.o.z, ra.tmax);
@@ -532,6 +549,26 @@ struct DeferNee {
             sp_o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
             sp_d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(path));
             sp_c[k] = make_float4(c.x, c.y, c.z, 0.0f);
+            entries += 1;
+        }
+    }
+};
+
+// Per-lane event counts of the shading kernel (pt_counters' shade_* fields).
+struct ShadeEvents {
+    uint32_t items = 0, env_lookups = 0, hits = 0, light_samples = 0, entries = 0, conts = 0, radiance_adds = 0;
+    __device__ __forceinline__ void flush(DevCounters* c) const {
+        const unsigned long long v[7] = {wave_sum_u64(items), wave_sum_u64(env_lookups), wave_sum_u64(hits),
+                                         wave_sum_u64(light_samples), wave_sum_u64(entries), wave_sum_u64(conts),
+                                         wave_sum_u64(radiance_adds)};
+        if (lane_id() == 0) {
+            atomicAdd(&c->shade_items, v[0]);
+            atomicAdd(&c->shade_env_lookups, v[1]);
+            atomicAdd(&c->shade_hits, v[2]);
+            atomicAdd(&c->shade_light_samples, v[3]);
+            atomicAdd(&c->shade_shadow_entries, v[4]);
+            atomicAdd(&c->shade_continuations, v[5]);
+            atomicAdd(&c->shade_radiance_adds, v[6]);
         }
     }
 };
@@ -566,9 +603,11 @@ struct ShadeItem {
 
 // WAVES > 0 asks the compiler for at least that many waves per SIMD (fewer
 // registers, possibly spills): the shading kernel's occupancy knob.
+// split: the queue is material-sorted and its escape bin is left to
+// k_wf_shade_escape (the items below each segment's C_MISS0 are shaded here).
 template <int BLOCK, int WAVES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8))) void k_wf_shade(
-    WfState w, int in_slot, int use_sorted) {
+    WfState w, int in_slot, int use_sorted, int split) {
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
@@ -576,12 +615,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
-    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
+    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade, nullptr, split ? w.ctrl + C_MISS0 : nullptr);
     // Three items per lane in flight: the one being shaded (A), the next one
     // whose path record and hit are being loaded (B) and the one after it
     // whose queue entry is being loaded (C).  Shading is a chain of
     // dependent gathers at two waves per SIMD; loading B and C during A
     // takes the queue entry -> path record steps off the chain.
+    ShadeEvents ev;
     ShadeItem b, c;
     b.has = qw.next(b.i, b.end);
     b.seg = qw.seg;
@@ -634,13 +674,75 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
                 r = sampler_state(rng, r);
                 if (cont) st.depth += 1;
                 cont = cont && st.depth < rp.max_depth;
-                store_path_added(w, p, st, r);
+                store_path_added(w, p, st, r, cont);
+                if (w.shade_ctr) {
+                    ev.items += 1;
+                    const bool miss = h.tri == 0xFFFFFFFFu;
+                    ev.env_lookups += (miss && job.scene.env != nullptr) ? 1u : 0u;
+                    ev.hits += miss ? 0u : 1u;
+                    ev.light_samples += nee.light_samples;
+                    ev.entries += nee.entries;
+                    ev.conts += cont ? 1u : 0u;
+                    ev.radiance_adds += is_zero(st.lo) ? 0u : 1u;
+                }
             }
         }
         // the continuation goes to the segment its item came from
         const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg], cont);
         if (cont) qout[ko] = p;
     }
+    if (w.shade_ctr) ev.flush(w.shade_ctr);
+}
+
+// The escapes of the bounce (the material-sorted queue's last bin, from each
+// segment's C_MISS0 to its end): the environment's radiance onto the path's
+// sum, the path ends.  Half of a bounce's items can be escapes; here they
+// take a few registers at full occupancy instead of a slot of the surface
+// kernel, whose registers hold a whole bounce.  The arithmetic is bounce()'s
+// own (escape()).
+struct EscapeNee {
+    __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState&, float) {}
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_wf_shade_escape(WfState w,
+                                                                                                     int in_slot) {
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    QueueWalk qw(w.ctrl, C_CUR_MISS, counts, n, w.grab_shade, w.ctrl + C_MISS0, nullptr);
+    ShadeEvents ev;
+    uint32_t i, end;
+    while (qw.next(i, end)) {
+        if (i >= end) continue;
+        const uint32_t p = w.sorted[i];
+        const PathRec& R = w.path[p];
+        const float4 o = R.o, d = R.d, tp = R.tp;
+        const uint4 r = R.rng;
+        const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
+        PathState st;
+        st.o = f3(o);
+        st.d = f3(d);
+        st.last_pdf = d.w;
+        st.tp = f3(tp);
+        const uint32_t bits = __float_as_uint(tp.w);
+        st.depth = static_cast<int>(bits & 0xFFFFu);
+        st.last_delta = (bits >> 31) != 0;
+        st.lo = f3(0.0f);
+        if (st.depth >= job.params.max_depth) continue;  // a job of smaller depth than the batch's stops here
+        EscapeNee nee;
+        escape(job.scene, st, nee);
+        if (!is_zero(st.lo)) {
+            const float4 l = w.lo[p];
+            w.lo[p] = make_float4(l.x + st.lo.x, l.y + st.lo.y, l.z + st.lo.z, 0.0f);
+        }
+        if (w.shade_ctr) {
+            ev.items += 1;
+            ev.env_lookups += job.scene.env != nullptr ? 1u : 0u;
+            ev.radiance_adds += is_zero(st.lo) ? 0u : 1u;
+        }
+    }
+    if (w.shade_ctr) ev.flush(w.shade_ctr);
 }
 
 __device__ __forceinline__ void add_unoccluded(const WfState& w, uint32_t p, float4 c, bool atomic) {
@@ -1434,7 +1536,7 @@ struct ChunkShape {
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
-    int extend, shade;
+    int extend, shade, escape;
     int ashade, ashadow;  // the adjoint's replay kernels
 };
 
@@ -1534,14 +1636,17 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         {
             Timed t(tm, WF_SHADE);
             const int us = cs.sort ? 1 : 0;
+            const int split = cs.sort ? 1 : 0;  // escapes to k_wf_shade_escape
             if (replay)
                 hipLaunchKernelGGL((k_wfa_shade<SB>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us);
             else if (cfg.shade_waves == 4)
-                hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
             else if (cfg.shade_waves == 3)
-                hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 3>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
             else
-                hipLaunchKernelGGL((k_wf_shade<SB, 0>), dim3(g.shade), dim3(SB), 0, stream, w, in, us);
+                hipLaunchKernelGGL((k_wf_shade<SB, 0>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
+            if (!replay && split)
+                hipLaunchKernelGGL((k_wf_shade_escape<SB>), dim3(g.escape), dim3(SB), 0, stream, w, in);
         }
         {
             Timed t(tm, WF_SHADOW);
@@ -1592,6 +1697,7 @@ Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg, bool wide) {
                         : cfg.shade_waves == 3 ? reinterpret_cast<const void*>(&k_wf_shade<SB, 3>)
                                                : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
     g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
+    g.escape = grid_for(reinterpret_cast<const void*>(&k_wf_shade_escape<SB>), SB, wf.num_cus, cfg.waves_per_simd);
     g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
     g.ashadow = wide ? std::min(grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true, true>), EB,
                                          wf.num_cus, cfg.waves_per_simd),
@@ -1769,6 +1875,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     wf->st.grab_extend = static_cast<uint32_t>(cfg.grab_extend);
     wf->st.grab_shade = static_cast<uint32_t>(cfg.grab_shade);
     wf->st.grab_shadow = static_cast<uint32_t>(cfg.grab_shadow);
+    wf->st.shade_ctr = count ? ctr : nullptr;
     wf->st.total_paths = paths;
     wf->st.total_pixels = static_cast<uint32_t>(pixels);
     ChunkShape cs;
