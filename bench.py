@@ -97,7 +97,9 @@ def cpu_baseline(sm, target_s: float = 12.0) -> dict:
     build_s = time.perf_counter() - t0
     cam = v.camera
     info = sm.info()
-    cores = os.cpu_count() or 1
+    # the host share this process may use (the OpenMP-style thread cap where
+    # one is set, else the affinity mask)
+    cores = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0)) or 1
     # calibrate on a 16-row strip at 1 spp, then render the whole frame at the
     # spp that gives ~target_s of work (frame 0 of the same sampler sequence)
     y0 = cam.height // 2
@@ -111,7 +113,19 @@ def cpu_baseline(sm, target_s: float = 12.0) -> dict:
     rays = c["closest_rays"] + c["shadow_rays"]
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"the full {cam.width}x{cam.height} frame at {spp} spp ({c['paths']} paths, {rays} rays, "
-                      f"{dt:.1f} s); oracle BVH build {build_s:.1f} s"}
+                      f"{dt:.1f} s); oracle BVH build {build_s:.1f} s",
+            "cpu": _cpu_model(), "build": oracle.lib().oracle_build_flags().decode()}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_integrator(sm, integ, steps: int, warmup: int) -> dict:

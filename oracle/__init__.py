@@ -36,6 +36,7 @@ _SIGS = {
                                  C.c_uint32, C.c_int, C.c_int]),
     "oracle_scene_add_texture": (C.c_int, [_P, _FP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "oracle_scene_destroy": (None, [_P]),
+    "oracle_build_flags": (C.c_char_p, []),
     "oracle_bvh_num_nodes": (C.c_uint32, [_P]),
     "oracle_bvh_get": (None, [_P, _FP, _U32P]),
     "oracle_trace": (None, [_P, _FP, C.c_int, C.c_int, _FP, _I32P, _FP]),

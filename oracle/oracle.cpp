@@ -2011,7 +2011,13 @@ std::vector<std::vector<TapeVertex>> recordTapes(const oracle_scene* s, const pt
 
 }  // namespace
 
+#ifndef ORACLE_CXX_FLAGS
+#define ORACLE_CXX_FLAGS "unknown"
+#endif
+
 extern "C" {
+
+const char* oracle_build_flags() { return ORACLE_CXX_FLAGS; }
 
 void oracle_adjoint_fd(const oracle_scene* s, const pt_camera* cam, int spf, int maxDepth, int sampler, uint32_t seed,
                        uint32_t frame, const float* weights, int nparams, const int* kind, const int* pa, const int* pb,

@@ -114,6 +114,9 @@ void oracle_render(const oracle_scene* s, const pt_camera* cam, int spf, int max
                    uint32_t frame, int x0, int x1, int y0, int y1, int threads, int fp64, float* out,
                    uint64_t* counters);
 
+/* The compiler command line this library was built with. */
+const char* oracle_build_flags(void);
+
 /* Gradient of  sum_pixels weights . I_batch  (I_batch = the batch mean image
  * of `frame`, weights W*H*3) with respect to nparams scalars, by central
  * differences with step h of the detached estimator in double precision:

@@ -131,7 +131,7 @@ def test_adjoint_matches_fp64_detached_fd(form):
     params = _params(sm, rng, 32) + [(3, 0, int(tex_g.argmax()), 1), (4, 0, int(env_g.argmax()), 2)]
     ora = _oracle_with_textures(sm)
     fd, value = ora.adjoint_fd(cam, info.sample_per_frame, info.max_depth, info.sampler, info.seed, frame, weights,
-                               params, h=1e-3)
+                               params, h=1e-4)
     # the recorded paths evaluate to the forward image
     gpu_value = float((img[..., :3].cpu().numpy().astype(np.float64) * weights).sum())
     assert abs(gpu_value - value) <= 1e-4 * abs(value) + 1e-3

@@ -67,7 +67,7 @@ def test_struct_sizes_agree_with_bindings():
     assert list(sizes)[:5] == [64, 16, 256, 112, 80]
 
 
-def test_default_material_matches_reference(golden):
+def test_default_material_matches_reference(golden, tmp_path):
     # materials/material.h defaults, byte for byte (the reference's Material{})
     sm = pt.SceneManager()
     sm.generate(0)
@@ -75,8 +75,7 @@ def test_default_material_matches_reference(golden):
     d = _capi.Material.from_buffer_copy(ref)
     assert abs(d.base_color.x - 0.8) < 1e-7 and d.roughness == np.float32(0.5) and d.ior == np.float32(1.5)
     # a JSON material with no overrides is the default material
-    p = os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp"
-    path = os.path.join(p, "default_mat_scene.json")
+    path = str(tmp_path / "default_mat_scene.json")
     with open(path, "w") as f:
         json.dump({"materials": {"m": {}}, "objects": [
             {"type": "quad", "vertices": [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], "material": "m"}]}, f)
