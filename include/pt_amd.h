@@ -486,6 +486,17 @@ pt_status pt_scene_build_emissive_tables_device(pt_scene* scene);
  * the GPU: pmf[i] = w[i] / sum (n floats), alias = n {prob, index bits}
  * pairs (2n floats), all device memory.  Synchronous. */
 pt_status pt_alias_build_device(const float* weights, uint32_t n, float* pmf, float* alias);
+/* pt_alias_build_device without allocating: its working buffers (40 bytes
+ * per weight) sit in a device array of the library, one per GPU, held until
+ * the process ends, so n <= PT_ALIAS_NOALLOC_MAX_WEIGHTS (more is
+ * PT_ERR_INVALID_ARG).  No hipMalloc/hipFree: for callers that cannot
+ * allocate (tight loops that rebuild an environment's tables after each
+ * update, harnesses that forbid allocation).  Same tables as
+ * pt_alias_build_device, bit for bit.  Calls on one GPU share the array:
+ * calls that overlap on one GPU race.  Synchronous.  No counterpart in the
+ * reference, whose tables are built on the host. */
+#define PT_ALIAS_NOALLOC_MAX_WEIGHTS (1u << 20)
+pt_status pt_alias_build_device_noalloc(const float* weights, uint32_t n, float* pmf, float* alias);
 /* Counter control: when enabled, render kernels count rays, node visits and
  * triangle tests (slower; used to price algorithmic bytes). */
 pt_status pt_counters_enable(pt_scene* scene, int enable);

@@ -243,6 +243,7 @@ SIGNATURES = {
     "pt_scene_rebuild_emissive_tables": (C.c_int, [_P]),
     "pt_scene_build_emissive_tables_device": (C.c_int, [_P]),
     "pt_alias_build_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pt_alias_build_device_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pt_counters_enable": (C.c_int, [_P, C.c_int]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
     "pt_counters_reset": (C.c_int, [_P]),

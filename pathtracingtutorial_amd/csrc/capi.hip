@@ -1370,6 +1370,23 @@ pt_status pt_alias_build_device(const float* weights, uint32_t n, float* pmf, fl
     });
 }
 
+pt_status pt_alias_build_device_noalloc(const float* weights, uint32_t n, float* pmf, float* alias) {
+    return guarded([&] {
+        if (n == 0) return;
+        if (!weights || !pmf || !alias) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        if (n > PT_ALIAS_NOALLOC_MAX_WEIGHTS)
+            throw Error(PT_ERR_INVALID_ARG, "pt_alias_build_device_noalloc: " + std::to_string(n) +
+                                                " weights exceed the maximum of " +
+                                                std::to_string(PT_ALIAS_NOALLOC_MAX_WEIGHTS));
+        ensure_device();
+        std::string err;
+        hipError_t e =
+            ptd::build_alias_device_noalloc(weights, n, pmf, reinterpret_cast<float2*>(alias), nullptr, &err);
+        if (e == hipErrorInvalidValue) throw Error(PT_ERR_INVALID_ARG, err.empty() ? "weights sum to zero" : err);
+        hip_check(e, "alias table on the GPU without allocation");
+    });
+}
+
 pt_status pt_scene_wavefront_stats(const pt_scene* s, pt_wavefront_stats* out) {
     return guarded([&] {
         if (!s || !out) throw Error(PT_ERR_INVALID_ARG, "null argument");

@@ -20,6 +20,13 @@ namespace ptd {
 hipError_t build_alias_device(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
                               std::string* err);
 
+// The same build with its working buffers in a device array of this library
+// (one per GPU, PT_ALIAS_NOALLOC_MAX_WEIGHTS weights at most; more gives
+// hipErrorInvalidValue): no device memory is allocated or freed.  Builds on
+// one GPU share the array, so builds that overlap on one GPU race.
+hipError_t build_alias_device_noalloc(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
+                                      std::string* err);
+
 // The environment's tables (light_sampler.cpp:112-149 weights: luminance *
 // sin(theta) of the texel centre) from its device texels.
 hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, float2* alias, hipStream_t stream,

@@ -14,6 +14,7 @@
 // double precision: this is a table build, not a render kernel).
 #include "light_build.h"
 
+#include <mutex>
 #include <vector>
 
 namespace ptd {
@@ -330,52 +331,138 @@ hipError_t dmalloc(T*& p, size_t n) {
     return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (n ? n : 1));
 }
 
+// The alias build's working buffers, carved from one block of device memory
+// (every piece 256-byte aligned).  Every element a kernel reads is written
+// earlier in the same build, so a block needs no clearing between builds.
+struct AliasScratch {
+    double* part_sum;     // per-tile weight sums
+    double* total;        // the weight sum
+    Acc* part;            // per-tile (lights, deficits, surpluses), then their total
+    double* p;            // n * w_i / W
+    uint32_t* light_idx;  // light items in index order
+    double* light_start;  // ... where each one's deficit starts
+    double* light_def;    // ... and its size
+    uint32_t* heavy_idx;  // heavy items in index order
+    double* heavy_start;  // ... where each one's surplus starts
+};
+
+// Byte offsets of the pieces above, in that order, and the block size.
+struct AliasLayout {
+    size_t off[9];
+    size_t bytes;
+};
+constexpr AliasLayout alias_layout(uint32_t n) {
+    const size_t tiles = (static_cast<size_t>(n) + kTile - 1) / kTile;
+    const size_t sizes[9] = {sizeof(double) * tiles, sizeof(double), sizeof(Acc) * (tiles + 1),
+                             sizeof(double) * n,     sizeof(uint32_t) * n, sizeof(double) * n,
+                             sizeof(double) * n,     sizeof(uint32_t) * n, sizeof(double) * n};
+    AliasLayout L{};
+    size_t o = 0;
+    for (int k = 0; k < 9; ++k) {
+        o = (o + 255) & ~static_cast<size_t>(255);
+        L.off[k] = o;
+        o += sizes[k];
+    }
+    L.bytes = o;
+    return L;
+}
+
+AliasScratch carve(uint32_t n, unsigned char* base) {
+    const AliasLayout L = alias_layout(n);
+    AliasScratch a;
+    a.part_sum = reinterpret_cast<double*>(base + L.off[0]);
+    a.total = reinterpret_cast<double*>(base + L.off[1]);
+    a.part = reinterpret_cast<Acc*>(base + L.off[2]);
+    a.p = reinterpret_cast<double*>(base + L.off[3]);
+    a.light_idx = reinterpret_cast<uint32_t*>(base + L.off[4]);
+    a.light_start = reinterpret_cast<double*>(base + L.off[5]);
+    a.light_def = reinterpret_cast<double*>(base + L.off[6]);
+    a.heavy_idx = reinterpret_cast<uint32_t*>(base + L.off[7]);
+    a.heavy_start = reinterpret_cast<double*>(base + L.off[8]);
+    return a;
+}
+
+// The build proper on a carved scratch block.
+hipError_t alias_build(const float* weights, uint32_t n, float* pmf, float2* alias, const AliasScratch& a,
+                       hipStream_t stream, std::string* err) {
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sum_tiles, dim3(tiles), dim3(kBlock), 0, stream, weights, n, a.part_sum);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, a.part_sum, tiles, a.total);
+    hipError_t e = hipGetLastError();
+    double W = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&W, a.total, sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    if (!(W > 0.0)) {
+        if (err) *err = "alias table: the weights sum to zero";
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_scale, dim3(blocks), dim3(kBlock), 0, stream, weights, n, a.total, a.p, pmf);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, stream, a.p, n, a.part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kBlock), 0, stream, a.part, tiles);
+    hipLaunchKernelGGL(k_scan_scatter, dim3(tiles), dim3(kBlock), 0, stream, a.p, n, a.part, a.light_idx,
+                       a.light_start, a.light_def, a.heavy_idx, a.heavy_start);
+    hipLaunchKernelGGL(k_assign_lights, dim3(blocks), dim3(kBlock), 0, stream, a.p, a.part + tiles, a.light_idx,
+                       a.light_start, a.heavy_idx, a.heavy_start, n, alias);
+    hipLaunchKernelGGL(k_assign_heavies, dim3(blocks), dim3(kBlock), 0, stream, a.part + tiles, a.light_idx,
+                       a.light_start, a.light_def, a.heavy_idx, a.heavy_start, n, alias);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
+// Scratch of the build that allocates nothing: one array per GPU, held until
+// the process ends, shared by every call on that GPU.
+__device__ __align__(256) unsigned char scratch[alias_layout(PT_ALIAS_NOALLOC_MAX_WEIGHTS).bytes];
+
+// scratch's device address on each GPU, looked up on the first call there.
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;
+
+hipError_t scratch_on_current_device(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (static_cast<size_t>(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 hipError_t build_alias_device(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
                               std::string* err) {
     if (n == 0) return hipSuccess;
-    const uint32_t tiles = (n + kTile - 1) / kTile;
-    const uint32_t blocks = (n + kBlock - 1) / kBlock;
-    double *part_sum = nullptr, *total = nullptr, *p = nullptr, *light_start = nullptr, *light_def = nullptr,
-           *heavy_start = nullptr;
-    Acc* part = nullptr;
-    uint32_t *light_idx = nullptr, *heavy_idx = nullptr;
-    hipError_t e = hipSuccess;
-    auto ok = [&](hipError_t r) {
-        if (e == hipSuccess) e = r;
-        return e == hipSuccess;
-    };
-    if (ok(dmalloc(part_sum, tiles)) && ok(dmalloc(total, 1)) && ok(dmalloc(p, n)) && ok(dmalloc(part, tiles + 1)) &&
-        ok(dmalloc(light_idx, n)) && ok(dmalloc(light_start, n)) && ok(dmalloc(light_def, n)) &&
-        ok(dmalloc(heavy_idx, n)) && ok(dmalloc(heavy_start, n))) {
-        hipLaunchKernelGGL(k_sum_tiles, dim3(tiles), dim3(kBlock), 0, stream, weights, n, part_sum);
-        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, stream, part_sum, tiles, total);
-        double W = 0.0;
-        if (ok(hipMemcpyAsync(&W, total, sizeof(double), hipMemcpyDeviceToHost, stream)) && ok(hipStreamSynchronize(stream))) {
-            if (!(W > 0.0)) {
-                if (err) *err = "alias table: the weights sum to zero";
-                e = hipErrorInvalidValue;
-            }
-        }
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_scale, dim3(blocks), dim3(kBlock), 0, stream, weights, n, total, p, pmf);
-            hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kBlock), 0, stream, p, n, part);
-            hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kBlock), 0, stream, part, tiles);
-            hipLaunchKernelGGL(k_scan_scatter, dim3(tiles), dim3(kBlock), 0, stream, p, n, part, light_idx, light_start,
-                               light_def, heavy_idx, heavy_start);
-            hipLaunchKernelGGL(k_assign_lights, dim3(blocks), dim3(kBlock), 0, stream, p, part + tiles, light_idx,
-                               light_start, heavy_idx, heavy_start, n, alias);
-            hipLaunchKernelGGL(k_assign_heavies, dim3(blocks), dim3(kBlock), 0, stream, part + tiles, light_idx,
-                               light_start, light_def, heavy_idx, heavy_start, n, alias);
-            ok(hipGetLastError());
-            ok(hipStreamSynchronize(stream));
-        }
-    }
-    void* bufs[] = {part_sum, total, p, part, light_idx, light_start, light_def, heavy_idx, heavy_start};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
+    unsigned char* block = nullptr;
+    hipError_t e = dmalloc(block, alias_layout(n).bytes);
+    if (e != hipSuccess) return e;
+    e = alias_build(weights, n, pmf, alias, carve(n, block), stream, err);
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(block);
     return e;
+}
+
+hipError_t build_alias_device_noalloc(const float* weights, uint32_t n, float* pmf, float2* alias, hipStream_t stream,
+                                      std::string* err) {
+    if (n == 0) return hipSuccess;
+    if (n > PT_ALIAS_NOALLOC_MAX_WEIGHTS) {
+        if (err)
+            *err = "alias table without allocation: " + std::to_string(n) + " weights exceed the maximum of " +
+                   std::to_string(PT_ALIAS_NOALLOC_MAX_WEIGHTS);
+        return hipErrorInvalidValue;
+    }
+    unsigned char* block = nullptr;
+    hipError_t e = scratch_on_current_device(&block);
+    if (e != hipSuccess) return e;
+    return alias_build(weights, n, pmf, alias, carve(n, block), stream, err);
 }
 
 hipError_t build_env_tables_device(const float4* env, int w, int h, float* pmf, float2* alias, hipStream_t stream,
