@@ -326,11 +326,17 @@ pt_status pt_scene_get_bvh(const pt_scene* scene, int layout, float* nodes8, uin
  * 0xFFFFFFFF for an unused slot.  The binary nodes that become wide nodes
  * are the choice of least summed surface area (binary leaves are kept), so
  * the result is deterministic for a given input.  max_stack: traversal
- * stack entries a ray can need.  Host only (the scene build runs the same collapse; it has no
- * reference counterpart: the reference traverses the binary tree).  Pass
- * NULL wide16 to query the count. */
+ * stack entries a ray can need.  Host code (it has no reference counterpart:
+ * the reference traverses the binary tree); the scene build runs the same
+ * collapse on the GPU.  Pass NULL wide16 to query the count. */
 pt_status pt_bvh_collapse_wide(const float* nodes8, uint32_t num_nodes, float* wide16, uint32_t* num_wide,
                                uint32_t* max_stack);
+/* A copy of the scene's 4-wide BVH as pt_scene_build_accel made it on the
+ * GPU (the layout above; equal, bit for bit, to pt_bvh_collapse_wide of the
+ * layout-0 nodes), and its max_stack.  num_wide is 0 when the scene
+ * traverses the binary tree only (non-finite bounds, or a tree deeper than
+ * the traversal stack).  Pass NULL wide16 to query the count. */
+pt_status pt_scene_get_wide_bvh(const pt_scene* scene, float* wide16, uint32_t* num_wide, uint32_t* max_stack);
 /* Emissive-triangle and environment tables (lights/light_sampler.cpp:57-149).
  * Pass NULL to query counts.  alias entries are {float prob, uint32 alias}. */
 pt_status pt_scene_get_light_tables(const pt_scene* scene,

@@ -165,6 +165,15 @@ class SceneManager:
         check(lib().pt_scene_get_bvh(self.handle, layout, fptr(nodes), C.byref(nn), u32ptr(order), C.byref(nf)))
         return nodes, order
 
+    def wideBvh(self):
+        """(the scene's 4-wide BVH as built on the GPU, (n, 16) float32 with
+        child words and quantized planes as uint32 bits; its stack bound)."""
+        nw, ms = C.c_uint32(0), C.c_uint32(0)
+        check(lib().pt_scene_get_wide_bvh(self.handle, None, C.byref(nw), C.byref(ms)))
+        out = np.zeros((nw.value, 16), np.float32)
+        check(lib().pt_scene_get_wide_bvh(self.handle, fptr(out), C.byref(nw), C.byref(ms)))
+        return out, int(ms.value)
+
     def lightTables(self):
         a, b, c = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
         check(lib().pt_scene_get_light_tables(self.handle, None, C.byref(a), None, C.byref(b), None, None, C.byref(c)))

@@ -210,6 +210,7 @@ SIGNATURES = {
     "pt_scene_get_bvh": (C.c_int, [_P, C.c_int, _FP, _U32P, _U32P, _U32P]),
     "pt_scene_get_light_tables": (C.c_int, [_P, _FP, _U32P, _FP, _U32P, _FP, _FP, _U32P]),
     "pt_bvh_collapse_wide": (C.c_int, [_FP, C.c_uint32, _FP, _U32P, _U32P]),
+    "pt_scene_get_wide_bvh": (C.c_int, [C.c_void_p, _FP, _U32P, _U32P]),
     "pt_film_create": (C.c_int, [C.c_int, C.c_int, _PP]),
     "pt_film_destroy": (None, [_P]),
     "pt_film_reset": (C.c_int, [_P, _P]),

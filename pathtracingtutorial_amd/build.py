@@ -28,7 +28,7 @@ ARCH = "gfx950"
 HOST_SOURCES = ["host/json.cpp", "host/scene_io.cpp", "host/generators.cpp", "host/light_tables.cpp",
                 "host/image_codec.cpp", "host/gltf.cpp", "host/bvh_wide.cpp"]
 HIP_SOURCES = ["capi.hip", "device/bvh_build.hip", "device/render.hip", "device/wavefront.hip", "device/adjoint.hip",
-               "device/light_build.hip"]
+               "device/light_build.hip", "device/bvh_wide_build.hip"]
 CLI_SOURCES = ["cli/pt_main.cpp"]
 
 CXXFLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -15,6 +15,7 @@
 
 #include "device/adjoint.h"
 #include "device/bvh_build.h"
+#include "device/bvh_wide_build.h"
 #include "device/dev_scene.h"
 #include "device/light_build.h"
 #include "device/render.h"
@@ -119,7 +120,6 @@ struct pt_scene {
     uint32_t wide_max_stack = 0;
     DBuf<uint32_t> prim_order;
     std::vector<uint32_t> prim_order_host;
-    ptd::BvhHostTopology topo;
     uint32_t depth = 0, max_leaf = 0;
     float build_ms = 0.0f;
     // shading data (set by upload)
@@ -264,38 +264,33 @@ ptd::RenderParams make_params_cam(const pt_scene* s, const pt_camera& cam) {
 
 ptd::RenderParams make_params(const pt_scene* s) { return make_params_cam(s, s->host.camera); }
 
-// Depth-first (reference) node order from the build topology.
-void dfs_layout(const ptd::BvhHostTopology& t, std::vector<float>& out) {
+// Depth-first (reference) node order from the device-order nodes (2 float4
+// per node, children adjacent): each inner node is followed by its left
+// subtree, then its right one; inner words become depth-first slots.
+void dfs_layout(const std::vector<float>& dev, std::vector<float>& out) {
     out.clear();
-    if (t.num_nodes == 0) return;
+    if (dev.empty()) return;
     struct Item {
         uint32_t node, parent_slot;
         int which;
     };
     std::vector<Item> st;
     st.push_back({0, 0xFFFFFFFFu, 0});
-    std::vector<uint32_t> words;
     while (!st.empty()) {
-        Item it = st.back();
+        const Item it = st.back();
         st.pop_back();
-        uint32_t slot = static_cast<uint32_t>(out.size() / 8);
-        if (it.parent_slot != 0xFFFFFFFFu) {
-            uint32_t w = slot;
-            std::memcpy(&out[it.parent_slot * 8 + (it.which == 0 ? 3 : 7)], &w, 4);
-        }
+        const uint32_t slot = static_cast<uint32_t>(out.size() / 8);
+        if (it.parent_slot != 0xFFFFFFFFu) std::memcpy(&out[it.parent_slot * 8 + (it.which == 0 ? 3 : 7)], &slot, 4);
         out.resize(out.size() + 8);
-        float* o = &out[slot * 8];
-        uint32_t k = it.node;
-        o[0] = t.lo[3 * k]; o[1] = t.lo[3 * k + 1]; o[2] = t.lo[3 * k + 2];
-        o[4] = t.hi[3 * k]; o[5] = t.hi[3 * k + 1]; o[6] = t.hi[3 * k + 2];
-        if (t.left[k] == 0) {
-            uint32_t d0 = 0x80000000u | t.start[k], d1 = t.count[k];
-            std::memcpy(&o[3], &d0, 4);
-            std::memcpy(&o[7], &d1, 4);
-        } else {
+        const float* src = &dev[8 * static_cast<size_t>(it.node)];
+        std::memcpy(&out[8 * static_cast<size_t>(slot)], src, 8 * sizeof(float));
+        uint32_t d0, d1;
+        std::memcpy(&d0, src + 3, 4);
+        std::memcpy(&d1, src + 7, 4);
+        if (!(d0 & 0x80000000u)) {
             // push right first so the left subtree is laid out next (left = slot + 1)
-            st.push_back({t.left[k] + 1, slot, 1});
-            st.push_back({t.left[k], slot, 0});
+            st.push_back({d1, slot, 1});
+            st.push_back({d0, slot, 0});
         }
     }
 }
@@ -560,7 +555,6 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
         s->tris.reset();
         s->faces_bvh.reset();
         s->prim_order.reset();
-        s->topo = ptd::BvhHostTopology();
         s->depth = s->max_leaf = 0;
         s->build_ms = 0.0f;
         if (nf > 0) {
@@ -570,7 +564,7 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
             bo.max_leaf = o.max_leaf;
             ptd::BvhDeviceResult r;
             std::string err;
-            if (!ptd::gpu_build_bvh(s->vpos.p, s->faces_orig.p, nf, bo, s->stream, &r, &s->topo, &err))
+            if (!ptd::gpu_build_bvh(s->vpos.p, s->faces_orig.p, nf, bo, s->stream, &r, nullptr, &err))
                 throw Error(err.find("supports at most") != std::string::npos ? PT_ERR_LIMIT : PT_ERR_DEVICE, err);
             s->nodes.adopt(r.nodes, static_cast<size_t>(r.num_nodes) * 2);
             s->prim_order.adopt(r.prim_order, nf);
@@ -582,20 +576,22 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
             hip_check(ptd::launch_emit_tris(s->vpos.p, s->faces_orig.p, s->prim_order.p, s->nodes.p, r.num_nodes, nf,
                                             s->tris.p, s->faces_bvh.p, s->stream),
                       "emit triangles");
-            s->prim_order_host = s->topo.prim_order;
-            // the 4-wide BVH, collapsed from the binary tree on the host
-            std::vector<float> ref_nodes;
-            dfs_layout(s->topo, ref_nodes);
-            // (a tree it cannot take, such as one with non-finite bounds, keeps
-            // the binary traversal only)
-            ptamd::WideBvh wb;
+            s->prim_order_host.resize(nf);
+            hip_check(hipMemcpyAsync(s->prim_order_host.data(), s->prim_order.p, sizeof(uint32_t) * nf,
+                                     hipMemcpyDeviceToHost, s->stream),
+                      "download face order");
+            // the 4-wide BVH, collapsed from the binary tree on the GPU (a tree
+            // with non-finite bounds, or one deeper than the traversal stack,
+            // keeps the binary traversal only)
+            ptd::WideDeviceResult wr;
             std::string werr;
-            const bool wide_ok =
-                ptamd::collapse_wide_bvh(ref_nodes.data(), static_cast<uint32_t>(ref_nodes.size() / 8), &wb, &werr);
-            s->wide_max_stack = wb.max_stack;
-            if (wide_ok && wb.num_nodes > 0 && wb.max_stack <= static_cast<uint32_t>(ptd::kWideStack))
-                s->wnodes.upload(reinterpret_cast<const float4*>(wb.words.data()), static_cast<size_t>(wb.num_nodes) * 4,
-                                 s->stream);
+            if (!ptd::gpu_collapse_wide(s->nodes.p, r.num_nodes, r.level_begin, r.level_count, s->stream, &wr, &werr))
+                throw Error(PT_ERR_DEVICE, werr);
+            s->wide_max_stack = wr.max_stack;
+            if (wr.nodes && wr.num_nodes > 0 && wr.max_stack <= static_cast<uint32_t>(ptd::kWideStack))
+                s->wnodes.adopt(wr.nodes, static_cast<size_t>(wr.num_nodes) * 4);
+            else if (wr.nodes)
+                (void)hipFree(wr.nodes);
         } else {
             s->prim_order_host.clear();
         }
@@ -806,16 +802,15 @@ pt_status pt_scene_get_bvh(const pt_scene* s, int layout, float* nodes8, uint32_
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
         if (!s->accel_built) throw Error(PT_ERR_STATE, "acceleration structure not built");
-        std::vector<float> out;
+        if (layout != 0 && layout != 1) throw Error(PT_ERR_INVALID_ARG, "layout must be 0 or 1");
+        std::vector<float> out(s->nodes.n * 4);
+        if (s->nodes.n)
+            hip_check(hipMemcpy(out.data(), s->nodes.p, sizeof(float4) * s->nodes.n, hipMemcpyDeviceToHost),
+                      "download nodes");
         if (layout == 0) {
-            dfs_layout(s->topo, out);
-        } else if (layout == 1) {
-            out.resize(s->nodes.n * 4);
-            if (s->nodes.n)
-                hip_check(hipMemcpy(out.data(), s->nodes.p, sizeof(float4) * s->nodes.n, hipMemcpyDeviceToHost),
-                          "download nodes");
-        } else {
-            throw Error(PT_ERR_INVALID_ARG, "layout must be 0 or 1");
+            std::vector<float> dev;
+            dev.swap(out);
+            dfs_layout(dev, out);
         }
         uint32_t nn = static_cast<uint32_t>(out.size() / 8);
         if (nodes8) {
@@ -829,6 +824,21 @@ pt_status pt_scene_get_bvh(const pt_scene* s, int layout, float* nodes8, uint32_
             std::memcpy(face_order, s->prim_order_host.data(), nf * sizeof(uint32_t));
         }
         if (num_faces) *num_faces = nf;
+    });
+}
+
+pt_status pt_scene_get_wide_bvh(const pt_scene* s, float* wide16, uint32_t* num_wide, uint32_t* max_stack) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (!s->accel_built) throw Error(PT_ERR_STATE, "acceleration structure not built");
+        const uint32_t nw = static_cast<uint32_t>(s->wnodes.n / 4);
+        if (wide16 && nw) {
+            if (!num_wide || *num_wide < nw) throw Error(PT_ERR_INVALID_ARG, "wide node buffer too small");
+            hip_check(hipMemcpy(wide16, s->wnodes.p, sizeof(float4) * s->wnodes.n, hipMemcpyDeviceToHost),
+                      "download wide nodes");
+        }
+        if (num_wide) *num_wide = nw;
+        if (max_stack) *max_stack = s->wide_max_stack;
     });
 }
 

@@ -31,6 +31,7 @@ struct BvhDeviceResult {
     uint32_t* prim_order = nullptr;  // BVH slot -> original face
     uint32_t depth = 0, max_leaf = 0;
     float build_ms = 0.0f;
+    std::vector<uint32_t> level_begin, level_count;  // nodes [begin, begin + count) per level, root first
 };
 
 // Builds over nfaces triangles given device vertex positions and original

@@ -620,6 +620,8 @@ bool gpu_build_bvh(const float4* d_vpos, const uint4* d_faces, uint32_t n, const
         hipLaunchKernelGGL(k_take_bounds, dim3(1), dim3(kBlock), 0, stream, root, cb, nlo, nhi, 1);
     }
     while (level_count > 0) {
+        res->level_begin.push_back(level_begin);
+        res->level_count.push_back(level_count);
         Level lv{level_begin, level_count};
         const uint32_t K = level_count;
         if (need_prim_bounds) {  // median-split children take their bounds from the primitives

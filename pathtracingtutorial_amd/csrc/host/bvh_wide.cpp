@@ -197,7 +197,9 @@ bool collapse_wide_bvh(const float* nodes8, uint32_t num_nodes, WideBvh* out, st
             for (int a = 0; a < 3; ++a) {
                 if (!std::isfinite(kids[i].lo[a]) || !std::isfinite(kids[i].hi[a]))
                     return fail("non-finite BVH bounds");
-                P[a] = std::fmin(P[a], kids[i].lo[a]), Q[a] = std::fmax(Q[a], kids[i].hi[a]);
+                // on equal values (+0 and -0) the later child's value is kept
+                P[a] = P[a] < kids[i].lo[a] ? P[a] : kids[i].lo[a];
+                Q[a] = Q[a] > kids[i].hi[a] ? Q[a] : kids[i].hi[a];
             }
         }
         for (int a = 0; a < 3; ++a) S[a] = axis_scale(P[a], Q[a]);

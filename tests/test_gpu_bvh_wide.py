@@ -1,0 +1,47 @@
+"""The 4-wide BVH that pt_scene_build_accel collapses on the GPU equals,
+bit for bit, the host collapse (pt_bvh_collapse_wide, whose invariants
+tests/test_bvh_wide.py pins) of the same binary tree in the reference's node
+layout: the same cut, breadth-first numbering, quantized boxes, child words
+and stack bound."""
+import numpy as np
+import pytest
+
+import pathtracingtutorial_amd as pt
+from test_gpu_parity import scene_from_bytes
+from test_oracle_golden import case_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(sm, label):
+    nodes, _ = sm.bvh(0)
+    host, host_ms = pt.collapse_wide(nodes)
+    dev, dev_ms = sm.wideBvh()
+    print(f"{label}: {nodes.shape[0]} binary nodes -> {host.shape[0]} wide nodes, stack {host_ms}", flush=True)
+    assert dev.shape == host.shape
+    assert dev_ms == host_ms
+    du, hu = dev.view(np.uint32), host.view(np.uint32)
+    bad = np.nonzero((du != hu).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} wide nodes differ, first {bad[:4]}: {du[bad[0]]} vs {hu[bad[0]]}"
+
+
+@pytest.mark.parametrize("which,bins,leaf,max_leaf", [(0, 24, 4, 0), (1, 32, 4, 4), (2, 24, 4, 0), (3, 24, 2, 0),
+                                                      (4, 24, 4, 0), (5, 24, 4, 4)])
+def test_gpu_collapse_equals_host_collapse(which, bins, leaf, max_leaf):
+    sm = pt.SceneManager()
+    sm.generate(which)
+    sm.buildAccel(bins, leaf, max_leaf)
+    _check(sm, f"generate({which}) bins {bins} leaf {leaf} max_leaf {max_leaf}")
+
+
+@pytest.mark.parametrize("name", ["row1", "row5", "row100", "soup200", "soup3000"])
+def test_gpu_collapse_on_golden_meshes(golden, name):
+    vb, fb, _ = case_arrays(golden, name)
+    nv, nf = len(vb) // 64, len(fb) // 16
+    sm = scene_from_bytes(vb, fb, nv, nf)
+    sm.buildAccel()
+    _check(sm, name)
+    if nf == 1:  # a leaf root: one wide node with the leaf as its only child
+        dev, ms = sm.wideBvh()
+        assert dev.shape == (1, 16) and ms == 0
+        assert dev.view(np.uint32)[0, 12] == 0x80000000 and np.all(dev.view(np.uint32)[0, 13:] == 0xFFFFFFFF)
