@@ -441,6 +441,7 @@ def main(argv=None) -> int:
         p_ach = algorithmic_bytes(pc) / a.steps / p_kernel_s / 1e9
         a_img, b_img = R["image"][..., :3].astype(np.float64), P["image"][..., :3].astype(np.float64)
         differ = float((np.abs(a_img - b_img) > 1e-4 * (1.0 + np.abs(b_img))).mean())
+        differ_bits = int((a_img != b_img).sum())
         out["plain"] = {"value": round(p_mrays, 2), "ms_per_step": round(P["wall"] * 1000.0 / a.steps, 2),
                         "speedup_of_wavefront": round(mrays / p_mrays, 3),
                         "roofline": {"kernel": "k_render_plain", "achieved": round(p_ach, 1),
@@ -448,6 +449,7 @@ def main(argv=None) -> int:
                                      "traffic": profiled_traffic("k_render_plain", WORKLOAD)},
                         # same random numbers per path: the images are the same estimate
                         "image_channels_differing_1e-4": round(differ, 6),
+                        "image_channels_differing_bits": differ_bits,
                         "rays_per_step": prays // a.steps}
     if not a.no_batched:
         try:

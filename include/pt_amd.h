@@ -215,7 +215,7 @@ typedef struct pt_wavefront_config {
     int32_t grab_shadow;      /* the same for the shadow-ray kernels */
     int32_t bvh_width;        /* 4: traverse the 4-wide BVH with quantized child boxes (built with the
                                  binary one; the same closest hits up to exact-distance ties), 2: the
-                                 binary BVH in the reference's traversal order (default 4) */
+                                 binary BVH, children entered by slab entry distance (default 4) */
 } pt_wavefront_config;
 
 /* GPU time (ms) and launch count per stage of the last wavefront batch

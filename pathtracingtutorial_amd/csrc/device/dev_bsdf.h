@@ -20,6 +20,13 @@
 
 namespace ptd {
 
+// No multiply-add contraction from here to the end of this file: every
+// product and sum rounds as written, so each kernel that inlines these
+// functions (the megakernel, the wavefront forward pass, the adjoint's
+// replay) computes a path's radiance to the same bits, whatever code
+// surrounds them.
+#pragma clang fp contract(off)
+
 // ---- scalar helpers usable with float and with dual numbers --------------
 __device__ __forceinline__ float sval(float x) { return x; }
 __device__ __forceinline__ float smaxf(float a, float b) { return fmaxf(a, b); }
@@ -554,5 +561,8 @@ __device__ __forceinline__ BsdfSample sample_bsdf(const MatEval<float>& m, F3 wo
     }
     return res;
 }
+
+// (the translation unit's default again)
+#pragma clang fp contract(fast)
 
 }  // namespace ptd

@@ -16,6 +16,13 @@
 
 namespace ptd {
 
+// No multiply-add contraction from here to the end of this file: every
+// product and sum rounds as written, so each kernel that inlines these
+// functions (the megakernel, the wavefront forward pass, the adjoint's
+// replay) computes a path's radiance to the same bits, whatever code
+// surrounds them.
+#pragma clang fp contract(off)
+
 // A per-path integrator's traversal context: the thread's stack, the tree it
 // walks (the 4-wide BVH when WIDE) and its ray counters.
 template <bool WIDE, class SK>
@@ -488,6 +495,14 @@ struct InlineNee {
 // current ray: returns true when the path continues with the new ray in st.
 // NEE contributions are premultiplied by the throughput and handed to the
 // visibility policy together with their shadow rays.
+// lo + (tp * e) * m with every product and sum rounded on its own: the
+// passes that sum a path's radiance (the forward pass from a zero start plus
+// the stored sum, the adjoint's replay from the stored sum) then round alike.
+__device__ __forceinline__ F3 add_radiance(F3 lo, F3 tp, F3 e, float m) {
+#pragma clang fp contract(off)
+    return f3(lo.x + (tp.x * e.x) * m, lo.y + (tp.y * e.y) * m, lo.z + (tp.z * e.z) * m);
+}
+
 // A ray that escaped (path.cu:71-82): the environment's radiance, MIS-weighted
 // against the environment sampling after a non-delta bounce; the path ends.
 template <class Nee>
@@ -499,7 +514,7 @@ __device__ __forceinline__ void escape(const DevScene& s, PathState& st, Nee& ne
         float mis = 1.0f;
         if (st.depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf_uv(s, st.d, eu, ev));
         nee.on_env_miss(s, st, mis);
-        st.lo += st.tp * env * mis;
+        st.lo = add_radiance(st.lo, st.tp, env, mis);
     }
 }
 
@@ -557,7 +572,8 @@ __device__ __forceinline__ bool bounce(const DevScene& s, const RenderParams& p,
     if (mat_emissive(mat)) {
         float mis = 1.0f;
         if (depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, emissive_pdf(s, sf.tri, sf.t, st.d));
-        st.lo += st.tp * (f3(mat.emissive.x, mat.emissive.y, mat.emissive.z) * mat.emissive_strength) * mis;
+        st.lo = add_radiance(st.lo, st.tp, f3(mat.emissive.x, mat.emissive.y, mat.emissive.z) * mat.emissive_strength,
+                             mis);
         return false;
     }
     if (mat.alpha_mode == 1) {  // path.cu:136-146 (raw uv)
@@ -679,5 +695,8 @@ __device__ F3 path_trace(const DevScene& s, const RenderParams& p, int x, int y,
     InlineNee<COUNT, Ctx> nee(c);
     return clamp_radiance(path_radiance<COUNT>(s, p, x, y, rng, c, nee));
 }
+
+// (the translation unit's default again)
+#pragma clang fp contract(fast)
 
 }  // namespace ptd

@@ -9,6 +9,13 @@
 
 namespace ptd {
 
+// No multiply-add contraction from here to the end of this file: every
+// product and sum rounds as written, so each kernel that inlines these
+// functions (the megakernel, the wavefront forward pass, the adjoint's
+// replay) computes a path's radiance to the same bits, whatever code
+// surrounds them.
+#pragma clang fp contract(off)
+
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.31830988618379067154f;
 constexpr float kInv2Pi = 0.15915494309189533577f;
@@ -83,5 +90,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+
+// (the translation unit's default again)
+#pragma clang fp contract(fast)
 
 }  // namespace ptd

@@ -6,9 +6,12 @@
 // (stack[level * stride + lane], conflict-free).  Leaves are ranges of
 // triangles ended by the "last" bit.  The slab test and the triangle test use
 // the reference's arithmetic (Part_2/src/accel/bvh_traversal.cuh:29-41,
-// geometry/triangle.h:13-76): the result is the same closest triangle (strict
-// '<' keeps the first found among exact ties, so only exact ties can resolve
-// differently from the reference's order).
+// geometry/triangle.h:13-76), so t is the reference's to the bit.  Children
+// are entered by slab entry distance (the reference orders them by child
+// centroid along the parent's widest axis, bvh_traversal.cuh:78-91), and the
+// strict '<' keeps the first triangle found among equal distances: the
+// closest triangle is the reference's except among exact-distance ties,
+// which may resolve the other way.
 #pragma once
 
 #include "dev_math.h"

@@ -29,7 +29,8 @@ struct WavefrontConfig {
     int grab_shade = 1024;
     int grab_shadow = 64;
     // 4: traverse the 4-wide BVH with quantized child boxes when every job's
-    // scene has one; 2: the binary BVH (the reference's traversal order)
+    // scene has one; 2: the binary BVH (both enter children nearest first;
+    // they can resolve an exact-distance tie differently)
     int bvh_width = 4;
 };
 

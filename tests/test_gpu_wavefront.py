@@ -71,6 +71,15 @@ def _same_paths(a, b, tol_frac=0.01):
     return frac_bad
 
 
+def _bit_identical(a, b, label):
+    """The two integrators trace the same paths (same tree, same random
+    numbers), round every radiance term alike (no multiply-add contraction
+    in the shading code) and sum a path's terms and a pixel's samples in the
+    same order: the images agree to the bit."""
+    d = np.abs(a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64))
+    assert np.array_equal(a[..., :3], b[..., :3]), f"{label}: {int((d > 0).sum())} channels differ, max {d.max():.3g}"
+
+
 def test_wavefront_matches_megakernel_cornell():
     sm = _cornell()
     mk, mk_acc = _render(sm, pt.PathIntegrator(), 2)
@@ -78,19 +87,20 @@ def test_wavefront_matches_megakernel_cornell():
     wf, wf_acc = _render(sm, pt.WavefrontIntegrator(), 2)
     assert np.isfinite(wf).all() and wf[..., :3].max() <= 100.0
     assert np.all(wf[..., 3] == 1.0)
-    bad = _same_paths(wf, mk)
-    print(f"cornell: {bad:.4%} of channels differ beyond 1e-4 relative")
-    _same_paths(wf_acc, mk_acc)
+    _bit_identical(wf, mk, "cornell batches")
+    _bit_identical(wf_acc, mk_acc, "cornell running mean")
 
 
-def test_wavefront_matches_megakernel_env_sobol():
+@pytest.mark.parametrize("width", [2, 4])
+def test_wavefront_matches_megakernel_env_sobol(width):
     sm = _env_scene()
+    sm.setWavefrontConfig(bvh_width=width)
+    sm.uploadToDevice()
     mk, _ = _render(sm, pt.PathIntegrator(), 2)
     sm.uploadToDevice()
     wf, _ = _render(sm, pt.WavefrontIntegrator(), 2)
     assert np.isfinite(wf).all()
-    bad = _same_paths(wf, mk, tol_frac=0.02)
-    print(f"env scene: {bad:.4%} of channels differ beyond 1e-4 relative")
+    _bit_identical(wf, mk, f"env scene, {width}-wide tree")
 
 
 def test_wavefront_matches_oracle():
@@ -200,7 +210,7 @@ def test_wavefront_depth_limits():
         wf, _ = _render(sm, pt.WavefrontIntegrator(), 1)
         if depth == 0:
             assert np.all(wf[..., :3] == 0.0)
-        _same_paths(wf, mk)
+        _bit_identical(wf, mk, f"depth {depth}")
 
 
 def test_wavefront_punctual_lights(tmp_path):
