@@ -386,7 +386,10 @@ def main(argv=None) -> int:
         sh_s = sh["ms"] / max(1, sh["launches"]) / 1000.0
         sh_bytes = shade_bytes(c) / max(1, sh["launches"] * a.steps)
         sh_ach = sh_bytes / sh_s / 1e9
-        shade_roofline = {"bound": "hbm", "kernel": "k_wf_shade", "achieved": round(sh_ach, 1),
+        # the shading stage: the hit paths' kernel and the escaped paths' kernel
+        # (bytes, time and profiled traffic all cover both)
+        shade_roofline = {"bound": "hbm", "kernel": "k_wf_shade", "kernels": ["k_wf_shade", "k_wf_shade_escape"],
+                          "achieved": round(sh_ach, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sh_ach / HBM_PEAK_GBS, 4),
                           "traffic": profiled_traffic("k_wf_shade", WORKLOAD), "kernel_ms": round(sh_s * 1000, 3),
                           "bytes_per_launch": int(sh_bytes),

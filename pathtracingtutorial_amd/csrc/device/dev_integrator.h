@@ -505,17 +505,57 @@ __device__ __forceinline__ F3 add_radiance(F3 lo, F3 tp, F3 e, float m) {
 
 // A ray that escaped (path.cu:71-82): the environment's radiance, MIS-weighted
 // against the environment sampling after a non-delta bounce; the path ends.
+// In three steps, so that a kernel can keep the gathers of several escaped
+// paths in flight at once: escape_taps (where to read: the bilinear taps of the
+// direction's equirect coordinates and its pmf entry), the reads (four texels,
+// and the pmf entry when env_pdf_wanted), env_escape (the radiance onto the
+// path's sum).  escape() chains them for one path.
+struct EscapeTaps {
+    TexTaps tex;
+    int pmf_idx;  // the direction's pmf entry, or -1: its pdf is 0 (no table, or a pole)
+    float st;     // sin(theta) of the direction
+};
+__device__ __forceinline__ EscapeTaps escape_taps(const DevScene& s, F3 d) {
+    EscapeTaps e;
+    float u = 0.0f, v = 0.0f;
+    dir_to_uv(d, u, v);
+    e.tex = tex_taps(s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
+    // environmentPdf (light_pdf.cuh:12-27), up to its table read
+    e.pmf_idx = -1;
+    e.st = 0.0f;
+    if (s.env_pmf != nullptr && s.num_env_texels != 0 && s.env_w > 0 && s.env_h > 0) {
+        int x = static_cast<int>(clampf(u * s.env_w, 0.0f, static_cast<float>(s.env_w - 1)));
+        int y = static_cast<int>(clampf(v * s.env_h, 0.0f, static_cast<float>(s.env_h - 1)));
+        e.st = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
+        if (e.st > 1e-5f) e.pmf_idx = y * s.env_w + x;
+    }
+    return e;
+}
+// The MIS weight needs the environment pdf after a non-delta bounce only.
+__device__ __forceinline__ bool env_pdf_wanted(const PathState& st) { return st.depth > 0 && !st.last_delta; }
 template <class Nee>
-__device__ __forceinline__ void escape(const DevScene& s, PathState& st, Nee& nee) {
-    float eu = 0.0f, ev = 0.0f;
-    if (s.env != nullptr) dir_to_uv(st.d, eu, ev);
-    F3 env = env_radiance_uv(s, eu, ev);
+__device__ __forceinline__ void env_escape(const DevScene& s, PathState& st, const EscapeTaps& e, const float4 (&t)[4],
+                                           float pmf, Nee& nee) {
+    const float4 c = texel_blend(e.tex.w[0], e.tex.w[1], e.tex.w[2], e.tex.w[3], t[0], t[1], t[2], t[3]);
+    const F3 env = f3(c.x, c.y, c.z) * kTwoPi;
     if (!is_zero(env)) {
         float mis = 1.0f;
-        if (st.depth > 0 && !st.last_delta) mis = power_heuristic(st.last_pdf, env_pdf_uv(s, st.d, eu, ev));
+        if (env_pdf_wanted(st)) {
+            const float pdf =
+                e.pmf_idx < 0 ? 0.0f : pmf * static_cast<float>(s.env_w * s.env_h) / (kTwoPi * kPi * e.st);
+            mis = power_heuristic(st.last_pdf, pdf);
+        }
         nee.on_env_miss(s, st, mis);
         st.lo = add_radiance(st.lo, st.tp, env, mis);
     }
+}
+template <class Nee>
+__device__ __forceinline__ void escape(const DevScene& s, PathState& st, Nee& nee) {
+    if (s.env == nullptr) return;  // no environment: no radiance
+    const EscapeTaps e = escape_taps(s, st.d);
+    const float4 t[4] = {s.env[e.tex.idx[0]], s.env[e.tex.idx[1]], s.env[e.tex.idx[2]], s.env[e.tex.idx[3]]};
+    const float pmf = env_pdf_wanted(st) && e.pmf_idx >= 0 ? s.env_pmf[e.pmf_idx] : 0.0f;
+    env_escape(s, st, e, t, pmf, nee);
 }
 
 template <class Nee>
