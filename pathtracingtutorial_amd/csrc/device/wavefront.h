@@ -24,9 +24,11 @@ struct WavefrontConfig {
     int stack_lds = 16;         // traversal stack entries in LDS (16 or 32); the rest live in device memory
     // queue items a wave takes per atomic (64 .. 1024, a power of two) in the
     // closest-hit, shading and shadow kernels: larger grabs cost fewer
-    // same-address atomics, smaller ones keep the rays in flight coherent
-    int grab_extend = 256;
-    int grab_shade = 1024;
+    // same-address atomics, smaller ones keep the rays in flight coherent and
+    // the waves' shares even (with each cursor on a line of its own, the
+    // smallest grabs were fastest on the 1080p icosphere frame)
+    int grab_extend = 64;
+    int grab_shade = 128;
     int grab_shadow = 64;
     // 4: traverse the 4-wide BVH with quantized child boxes when every job's
     // scene has one; 2: the binary BVH (both enter children nearest first;

@@ -732,10 +732,9 @@ struct EscapeItem {
     }
 };
 
-// Items per grab of the escape kernel: an escape is short work, so grabs of
-// 64 rows of 64 keep the cursor atomics rare (on the 1080p icosphere frame:
-// 0.85 ms per launch, against 0.90 ms with 16 rows and 1.8 ms with 4).
-constexpr uint32_t kEscapeGrab = 4096;
+// Items per grab of the escape kernel (rows of 64): on the 1080p icosphere
+// frame 0.56 ms per launch with 4 rows, 0.57 with 16, 0.67 with 64.
+constexpr uint32_t kEscapeGrab = 256;
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_wf_shade_escape(WfState w,
