@@ -42,10 +42,6 @@ constexpr int kSeg = 8;
 constexpr int kSortParts = 32;  // blocks per segment of the material sort
 constexpr uint32_t kMaxSortBins = 8192;
 // control words
-constexpr int C_QCNT = 0;    // [2][8] ray queue segment counts
-constexpr int C_SACNT = 16;  // [8] area shadow queue segment counts
-constexpr int C_SECNT = 24;  // [8] environment shadow queue segment counts
-constexpr int C_SP = 32;     // punctual shadow queue count (one flat queue)
 constexpr int C_CHUNK = 33;  // chunk index within the batch
 constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72, C_CUR_MISS = 80;
 constexpr int C_MISS0 = 88;  // [8] first item of each segment's escape bin in the material-sorted queue
@@ -58,7 +54,14 @@ constexpr int C_DRAIN = 128, C_DRAIN_STRIDE = 32;
 // eight segments' atomics would queue behind one another.
 constexpr int C_LINE = 32;
 constexpr int C_CURSORS = C_DRAIN + 6 * C_DRAIN_STRIDE;
-constexpr int C_WORDS = C_CURSORS + 6 * 8 * C_LINE;
+// The queues' segment counts, which every wave of the shading kernels adds
+// to, each on a line of its own too: count k of a set at base + k * C_LINE.
+constexpr int C_COUNTS = C_CURSORS + 6 * 8 * C_LINE;
+constexpr int C_QCNT = C_COUNTS;                      // [2][8] ray queue segment counts
+constexpr int C_SACNT = C_COUNTS + 2 * 8 * C_LINE;  // [8] area shadow queue segment counts
+constexpr int C_SECNT = C_COUNTS + 3 * 8 * C_LINE;  // [8] environment shadow queue segment counts
+constexpr int C_SP = C_COUNTS + 4 * 8 * C_LINE;     // punctual shadow queue count (one flat queue)
+constexpr int C_WORDS = C_SP + C_LINE;
 
 // A job as the kernels see it (device memory, one entry per job).
 struct JobDesc {
@@ -211,7 +214,7 @@ struct QueueWalk {
             if (tried >= kSeg) return false;
             const uint32_t b = seg_base(n, seg);
             const uint32_t s0 = starts ? starts[seg] : b;
-            const uint32_t s1 = stops ? stops[seg] : (counts ? b + counts[seg] : seg_base(n, seg + 1));
+            const uint32_t s1 = stops ? stops[seg] : (counts ? b + counts[seg * C_LINE] : seg_base(n, seg + 1));
             if (s1 <= s0) {
                 skip();
                 continue;
@@ -328,7 +331,7 @@ __global__ void k_wf_chunk_begin(WfState w) {
     const int k = static_cast<int>(threadIdx.x);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    if (k < kSeg) w.ctrl[C_QCNT + k] = seg_base(n, k + 1) - seg_base(n, k);
+    if (k < kSeg) w.ctrl[C_QCNT + k * C_LINE] = seg_base(n, k + 1) - seg_base(n, k);
 }
 
 __global__ void k_wf_chunk_end(uint32_t* ctrl) {
@@ -365,12 +368,14 @@ __global__ void k_wf_generate(WfState w) {
 __global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
     uint32_t t = threadIdx.x;
     if (t < kSeg) {
-        ctrl[C_QCNT + out_slot * kSeg + t] = 0;
-        ctrl[C_SACNT + t] = 0;
-        ctrl[C_SECNT + t] = 0;
+        ctrl[C_QCNT + (out_slot * kSeg + t) * C_LINE] = 0;
+        ctrl[C_SACNT + t * C_LINE] = 0;
+        ctrl[C_SECNT + t * C_LINE] = 0;
     }
     if (t == 0) ctrl[C_SP] = 0;
-    for (uint32_t k = C_CUR_EXT + t; k < C_WORDS; k += blockDim.x) ctrl[k] = 0;
+    // (the cursors, the drained masks and the escape-bin starts; not the
+    // counts, which hold the input queue)
+    for (uint32_t k = C_CUR_EXT + t; k < C_COUNTS; k += blockDim.x) ctrl[k] = 0;
 }
 
 // Traversal stack entries of the binary or the 4-wide BVH.
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     const auto sk = thread_stack<LDS, BLOCK, stack_total<WIDE>()>(w, stack);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
     const uint32_t* q = w.q[in_slot];
     QueueWalk qw(w.ctrl, C_CUR_EXT, counts, n, w.grab_extend);
     uint32_t i, end;
@@ -435,7 +440,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
 
 __device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* counts, int seg, int part, uint32_t& b0,
                                                 uint32_t& b1) {
-    const uint32_t base = seg_base(n, seg), cnt = counts[seg];
+    const uint32_t base = seg_base(n, seg), cnt = counts[seg * C_LINE];
     b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * part) / kSortParts);
     b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * (part + 1)) / kSortParts);
 }
@@ -448,7 +453,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, int in_slot, u
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
+    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg * C_LINE, seg, part, b0, b1);
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[w.qkey[i]], 1u);
     __syncthreads();
     uint32_t* out = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot,
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg, seg, part, b0, b1);
+    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg * C_LINE, seg, part, b0, b1);
     const uint32_t* q = w.q[in_slot];
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) w.sorted[atomicAdd(&off[w.qkey[i]], 1u)] = q[i];
 }
@@ -617,11 +622,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
     WfState w, int in_slot, int use_sorted, int split) {
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
     const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
-    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
+    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg * C_LINE;
     QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade, nullptr, split ? w.ctrl + C_MISS0 : nullptr);
     // Three items per lane in flight: the one being shaded (A), the next one
     // whose path record and hit are being loaded (B) and the one after it
@@ -652,8 +657,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
         bool cont = false;
         DeferNee nee;
         nee.base = seg_base(n, seg);
-        nee.sa_count = &w.ctrl[C_SACNT + seg];
-        nee.se_count = &w.ctrl[C_SECNT + seg];
+        nee.sa_count = &w.ctrl[C_SACNT + seg * C_LINE];
+        nee.se_count = &w.ctrl[C_SECNT + seg * C_LINE];
         nee.sa_o = w.sa_o;
         nee.sa_d = w.sa_d;
         nee.sa_c = w.sa_c;
@@ -695,7 +700,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0
             }
         }
         // the continuation goes to the segment its item came from
-        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg], cont);
+        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg * C_LINE], cont);
         if (cont) qout[ko] = p;
     }
     if (w.shade_ctr) ev.flush(w.shade_ctr);
@@ -741,7 +746,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
                                                                                                      int in_slot) {
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
     QueueWalk qw(w.ctrl, C_CUR_MISS, counts, n, kEscapeGrab, w.ctrl + C_MISS0, nullptr);
     ShadeEvents ev;
     EscapeItem b, c;
@@ -1149,11 +1154,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
     const DevScene& s0 = w.jobs[0].scene;
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg;
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
     const uint32_t* q = use_sorted ? w.sorted : w.q[in_slot];
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
-    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg;
+    uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg * C_LINE;
     QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
     uint32_t i, end;
     while (qw.next(i, end)) {
@@ -1165,8 +1170,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         bool cont = false;
         AdjDeferNee nee;
         nee.base = seg_base(n, seg);
-        nee.sa_count = &w.ctrl[C_SACNT + seg];
-        nee.se_count = &w.ctrl[C_SECNT + seg];
+        nee.sa_count = &w.ctrl[C_SACNT + seg * C_LINE];
+        nee.se_count = &w.ctrl[C_SECNT + seg * C_LINE];
         nee.sa_o = w.sa_o;
         nee.sa_d = w.sa_d;
         nee.sa_c = w.sa_c;
@@ -1232,7 +1237,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         }
         apply_vertex(s0, G, macc, pv, pmi, ptu, ptv, pgv);
         apply_env(s0, G, nee.miss, st.d, nee.miss_coef);
-        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg], cont);
+        const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg * C_LINE], cont);
         if (cont) qout[ko] = p;
     }
     flush_materials(macc, G, s0.num_materials);
