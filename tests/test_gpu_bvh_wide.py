@@ -25,12 +25,17 @@ def _check(sm, label):
     assert bad.size == 0, f"{bad.size} wide nodes differ, first {bad[:4]}: {du[bad[0]]} vs {hu[bad[0]]}"
 
 
-@pytest.mark.parametrize("which,bins,leaf,max_leaf", [(0, 24, 4, 0), (1, 32, 4, 4), (2, 24, 4, 0), (3, 24, 2, 0),
-                                                      (4, 24, 4, 0), (5, 24, 4, 4)])
+@pytest.mark.parametrize("which,bins,leaf,max_leaf", [(0, 24, 4, 0), (1, None, None, None), (1, 32, 4, 4),
+                                                      (2, 24, 4, 0), (3, 24, 2, 0), (4, 24, 4, 0), (5, None, None, None)])
 def test_gpu_collapse_equals_host_collapse(which, bins, leaf, max_leaf):
+    # (None: the workload's own build options, e.g. 32 bins and at most 4
+    # triangles per leaf for the 1080p icosphere frame)
     sm = pt.SceneManager()
     sm.generate(which)
-    sm.buildAccel(bins, leaf, max_leaf)
+    if bins is None:
+        sm.buildAccel()
+    else:
+        sm.buildAccel(bins, leaf, max_leaf)
     _check(sm, f"generate({which}) bins {bins} leaf {leaf} max_leaf {max_leaf}")
 
 
