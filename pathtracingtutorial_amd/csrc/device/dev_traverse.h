@@ -31,12 +31,14 @@ struct TraceCounters {
 
 struct RayInv {
     F3 o, inv;
+    F3 oi;  // 2^-20 |o inv| per axis: part of the wide node test's rounding slack
 };
 
 __device__ __forceinline__ RayInv make_ray_inv(F3 o, F3 d) {
     RayInv r;
     r.o = o;
     r.inv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    r.oi = f3(fabsf(o.x * r.inv.x) * 0x1p-20f, fabsf(o.y * r.inv.y) * 0x1p-20f, fabsf(o.z * r.inv.z) * 0x1p-20f);
     return r;
 }
 
@@ -94,9 +96,11 @@ __device__ __forceinline__ float slab_v(const RayInv& r, float4 lo, float4 hi, f
     else return slab(r, lo, hi, tmax);
 }
 
-// inv = 1 / d has no infinite (or NaN) component: slab_fast applies.
+// inv = 1 / d has no component above 2^64 (none infinite or NaN): slab_fast
+// applies, and the wide test's products S inv stay finite for any node
+// narrower than 2^64 per plane step.
 __device__ __forceinline__ bool finite_inv(const RayInv& r) {
-    return fabsf(r.inv.x) <= kInf && fabsf(r.inv.y) <= kInf && fabsf(r.inv.z) <= kInf;
+    return fabsf(r.inv.x) <= 0x1p64f && fabsf(r.inv.y) <= 0x1p64f && fabsf(r.inv.z) <= 0x1p64f;
 }
 
 // Moller-Trumbore (triangle.h:13-51 / :53-76).  Returns t, or -1 on a miss;
@@ -111,7 +115,18 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
     F3 pvec = cross_rn(d, e2);
     float det = dot_rn(e1, pvec);
     if (fabsf(det) < kEps) return -1.0f;
-    float inv_det = 1.0f / det;
+    float inv_det;
+    if (__builtin_expect(fabsf(det) < 0x1p126f, 1)) {
+        // the hardware reciprocal and one Newton step with fma: equal to the
+        // correctly rounded 1 / det for every float with |det| in [2^-126,
+        // 2^126) (compared on the GPU with IEEE division for every such
+        // float of either sign); three instructions instead of the
+        // division's scale / fma / fixup sequence
+        const float r0 = __builtin_amdgcn_rcpf(det);
+        inv_det = fmaf(fmaf(-det, r0, 1.0f), r0, r0);
+    } else {
+        inv_det = 1.0f / det;
+    }
     F3 tvec = o - f3(a.x, a.y, a.z);
     u = dot_rn(tvec, pvec) * inv_det;
     if (u < 0.0f || u > 1.0f) return -1.0f;
@@ -341,22 +356,38 @@ __device__ __forceinline__ int wide_children(const float4* nd, const RayInv& r, 
     w[3] = __float_as_uint(cw.w);
     // k: sort key, the entry distance's bits (t >= 0), or ~0 for a miss
     if constexpr (FAST) {
-        // slab_fast with each axis's near plane picked once per node: with
-        // inv >= 0 the low plane's distance is the smaller one (a decoded low
-        // plane never exceeds the high one, and the rounding of - o and * inv
-        // keeps the order), so min/max per axis become a choice of bytes
+        // A plane at byte q lies at distance (q S + P - o) inv = q A + B with
+        // A = S inv (exact: S is a power of two) and B = (P - o) inv, one fma
+        // per plane.  The fma form and the plane-by-plane form
+        // ((fma(q, S, P) - o) * inv, the decoded box under the reference's
+        // slab arithmetic) round differently, by at most
+        // u (1530 |A| + 5 |B| + 3 |P inv|) with u = 2^-24 (the decode's
+        // rounding, the two roundings of B, the fma's, and those of the
+        // plane-by-plane form).  With |P inv| <= |B| + |o inv| the slack e =
+        // 2^-20 (255 |A| + 2 |B| + |o inv|) exceeds that: near planes move
+        // closer by e and far planes farther, so every child box the
+        // plane-by-plane test accepts is accepted here (a superset of leaves
+        // is visited; the closest hit among them is the same triangle up to
+        // exact-distance ties).  With inv >= 0 the low plane is the near one,
+        // so each axis's near and far bytes are picked once per node.
         const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
         const uint32_t nx = px ? qlx : qhx, fx = px ? qhx : qlx;
         const uint32_t ny = py ? qly : qhy, fy = py ? qhy : qly;
         const uint32_t nz = pz ? qlz : qhz, fz = pz ? qhz : qlz;
+        const float Ax = a.w * r.inv.x, Ay = b.x * r.inv.y, Az = b.y * r.inv.z;
+        const float Bx = (a.x - r.o.x) * r.inv.x, By = (a.y - r.o.y) * r.inv.y, Bz = (a.z - r.o.z) * r.inv.z;
+        const float ex = fmaf(255.0f * 0x1p-20f, fabsf(Ax), fmaf(0x1p-19f, fabsf(Bx), r.oi.x));
+        const float ey = fmaf(255.0f * 0x1p-20f, fabsf(Ay), fmaf(0x1p-19f, fabsf(By), r.oi.y));
+        const float ez = fmaf(255.0f * 0x1p-20f, fabsf(Az), fmaf(0x1p-19f, fabsf(Bz), r.oi.z));
+        const float Bnx = Bx - ex, Bfx = Bx + ex, Bny = By - ey, Bfy = By + ey, Bnz = Bz - ez, Bfz = Bz + ez;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float tnx = (wide_plane(nx, i, a.w, a.x) - r.o.x) * r.inv.x;
-            const float tfx = (wide_plane(fx, i, a.w, a.x) - r.o.x) * r.inv.x;
-            const float tny = (wide_plane(ny, i, b.x, a.y) - r.o.y) * r.inv.y;
-            const float tfy = (wide_plane(fy, i, b.x, a.y) - r.o.y) * r.inv.y;
-            const float tnz = (wide_plane(nz, i, b.y, a.z) - r.o.z) * r.inv.z;
-            const float tfz = (wide_plane(fz, i, b.y, a.z) - r.o.z) * r.inv.z;
+            const float tnx = fmaf(static_cast<float>((nx >> (8 * i)) & 0xFFu), Ax, Bnx);
+            const float tfx = fmaf(static_cast<float>((fx >> (8 * i)) & 0xFFu), Ax, Bfx);
+            const float tny = fmaf(static_cast<float>((ny >> (8 * i)) & 0xFFu), Ay, Bny);
+            const float tfy = fmaf(static_cast<float>((fy >> (8 * i)) & 0xFFu), Ay, Bfy);
+            const float tnz = fmaf(static_cast<float>((nz >> (8 * i)) & 0xFFu), Az, Bnz);
+            const float tfz = fmaf(static_cast<float>((fz >> (8 * i)) & 0xFFu), Az, Bfz);
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
             const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
             k[i] = (w[i] != kWideEmpty && tn <= tf) ? (__float_as_uint(tn) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
