@@ -541,6 +541,14 @@ pt_status pt_trace_rays(pt_scene* scene, const float* rays8, int n, int any_hit,
  * traffic counters and states the achievable bandwidth of the box. */
 pt_status pt_probe_copy(size_t bytes, int iters, float* gbps);
 
+/* Compares the triangle test's reciprocal (hardware reciprocal plus one
+ * Newton step) with IEEE division, bit for bit, for every float of either
+ * sign with magnitude in [2^-126, 2^126); returns the number of differing
+ * results and the number compared (2 x 252 x 2^23).  No reference
+ * counterpart: it checks that the triangle test's 1/det equals the
+ * reference's 1.0f / det (geometry/triangle.h:27). */
+pt_status pt_probe_reciprocal(uint64_t* mismatches, uint64_t* compared);
+
 #ifdef __cplusplus
 }
 #endif

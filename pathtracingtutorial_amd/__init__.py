@@ -25,7 +25,7 @@ from ._capi import (BvhOptions, Camera, CameraFrame, Counters, PtError, SceneInf
 __all__ = [
     "SceneManager", "Renderer", "Integrator", "PathIntegrator", "WavefrontIntegrator", "PtError", "QualityMetrics",
     "render_jobs",
-    "compute_rmse", "compute_relmse", "probe_sampler", "trace_rays", "bvh_options", "REFERENCE_BVH", "lib",
+    "compute_rmse", "compute_relmse", "probe_sampler", "probe_reciprocal", "trace_rays", "bvh_options", "REFERENCE_BVH", "lib",
 ]
 
 # Reference builder settings (accel/bvh_builder.cpp:10-13).
@@ -462,3 +462,11 @@ def probe_copy(nbytes: int = 1 << 30, iters: int = 10) -> float:
     g = C.c_float(0.0)
     check(lib().pt_probe_copy(int(nbytes), int(iters), C.byref(g)))
     return float(g.value)
+
+
+def probe_reciprocal() -> tuple[int, int]:
+    """(mismatches, compared): the triangle test's reciprocal against IEEE
+    division over every float of magnitude in [2^-126, 2^126)."""
+    bad, n = C.c_uint64(0), C.c_uint64(0)
+    check(lib().pt_probe_reciprocal(C.byref(bad), C.byref(n)))
+    return int(bad.value), int(n.value)

@@ -1630,4 +1630,19 @@ pt_status pt_probe_copy(size_t bytes, int iters, float* gbps) {
     });
 }
 
+pt_status pt_probe_reciprocal(uint64_t* mismatches, uint64_t* compared) {
+    return guarded([&] {
+        if (!mismatches || !compared) throw Error(PT_ERR_INVALID_ARG, "bad argument");
+        ensure_device();
+        DBuf<unsigned long long> d;
+        d.alloc(1);
+        hip_check(hipMemset(d.p, 0, sizeof(unsigned long long)), "hipMemset");
+        hip_check(ptd::launch_probe_reciprocal(d.p, nullptr), "reciprocal probe launch");
+        unsigned long long bad = 0;
+        hip_check(hipMemcpy(&bad, d.p, sizeof(bad), hipMemcpyDeviceToHost), "reciprocal probe download");
+        *mismatches = bad;
+        *compared = 2ull * 252ull * (1ull << 23);
+    });
+}
+
 }  // extern "C"

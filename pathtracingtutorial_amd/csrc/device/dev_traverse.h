@@ -103,6 +103,16 @@ __device__ __forceinline__ bool finite_inv(const RayInv& r) {
     return fabsf(r.inv.x) <= 0x1p64f && fabsf(r.inv.y) <= 0x1p64f && fabsf(r.inv.z) <= 0x1p64f;
 }
 
+// 1 / x correctly rounded for |x| in [2^-126, 2^126): the hardware
+// reciprocal (within 1 ulp) and one Newton step with fma.  Equal to IEEE
+// division for every float of either sign in that range (pt_probe_reciprocal
+// compares all 2 x 252 x 2^23 of them); three instructions instead of the
+// division's scale / fma / fixup sequence.
+__device__ __forceinline__ float recip_rn(float x) {
+    const float r0 = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r0, 1.0f), r0, r0);
+}
+
 // Moller-Trumbore (triangle.h:13-51 / :53-76).  Returns t, or -1 on a miss;
 // u, v are the barycentrics of p1 and p2.
 __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& u, float& v, uint32_t& word) {
@@ -116,17 +126,8 @@ __device__ __forceinline__ float tri_test(F3 o, F3 d, const float4* tri, float& 
     float det = dot_rn(e1, pvec);
     if (fabsf(det) < kEps) return -1.0f;
     float inv_det;
-    if (__builtin_expect(fabsf(det) < 0x1p126f, 1)) {
-        // the hardware reciprocal and one Newton step with fma: equal to the
-        // correctly rounded 1 / det for every float with |det| in [2^-126,
-        // 2^126) (compared on the GPU with IEEE division for every such
-        // float of either sign); three instructions instead of the
-        // division's scale / fma / fixup sequence
-        const float r0 = __builtin_amdgcn_rcpf(det);
-        inv_det = fmaf(fmaf(-det, r0, 1.0f), r0, r0);
-    } else {
-        inv_det = 1.0f / det;
-    }
+    if (__builtin_expect(fabsf(det) < 0x1p126f, 1)) inv_det = recip_rn(det);
+    else inv_det = 1.0f / det;
     F3 tvec = o - f3(a.x, a.y, a.z);
     u = dot_rn(tvec, pvec) * inv_det;
     if (u < 0.0f || u > 1.0f) return -1.0f;

@@ -35,5 +35,8 @@ hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, con
                             uint4* d_faces_bvh, hipStream_t stream);
 
 hipError_t launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream);
+// Counts (into *d_out, which the caller zeroes) the floats of magnitude in
+// [2^-126, 2^126) whose recip_rn differs from IEEE 1.0f / x.
+hipError_t launch_probe_reciprocal(unsigned long long* d_out, hipStream_t stream);
 
 }  // namespace ptd

@@ -347,4 +347,27 @@ hipError_t launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream
     return hipGetLastError();
 }
 
+// ---- reciprocal probe ------------------------------------------------------------
+// Thread i takes significand i and walks every biased exponent 1..252 of both
+// signs (|x| in [2^-126, 2^126)), comparing recip_rn(x) with IEEE 1.0f / x
+// bit for bit.  The grid covers exactly 2^23 threads.
+
+__global__ __launch_bounds__(256) void k_probe_reciprocal(unsigned long long* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << 23)) return;
+    unsigned long long bad = 0;
+    for (uint32_t e = 1; e <= 252; ++e) {
+        for (uint32_t s = 0; s < 2; ++s) {
+            const float x = __uint_as_float((s << 31) | (e << 23) | i);
+            bad += __float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x);
+        }
+    }
+    if (bad) atomicAdd(out, bad);
+}
+
+hipError_t launch_probe_reciprocal(unsigned long long* d_out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_probe_reciprocal, dim3((1u << 23) / 256), dim3(256), 0, stream, d_out);
+    return hipGetLastError();
+}
+
 }  // namespace ptd

@@ -228,6 +228,15 @@ def test_gpu_bvh_config_settings_invariants():
     assert np.all(covered == 1)
 
 
+def test_triangle_reciprocal_equals_ieee_division():
+    """The triangle test's 1/det (hardware reciprocal + one Newton step) is
+    the correctly rounded reciprocal that the reference's 1.0f / det gives
+    (geometry/triangle.h:27), for every float of magnitude in [2^-126, 2^126)."""
+    bad, n = pt.probe_reciprocal()
+    assert n == 2 * 252 * (1 << 23)
+    assert bad == 0, f"{bad} of {n} reciprocals differ from IEEE division"
+
+
 # ------------------------------------------------------------- rendering
 def _paired_batches(sm, ora, spf, batches, rect=None):
     """GPU 'new' film per batch and oracle batch means with the same frames."""
