@@ -276,12 +276,13 @@ def views_leg(steps: int = 2) -> dict:
     want = (False, False, False, True, True)
     mv.forward_images(0)
     mv.adjoint(0, g, want)
-    f_ms, b_ms = [], []
+    f_ms, b_ms, reused = [], [], []
     for i in range(steps):
         mv.forward_images(i)
         f_ms.append(mv.last_forward_ms)
         mv.adjoint(i, g, want)
         b_ms.append(mv.last_backward_ms)
+        reused.append(mv.last_backward_reused)
     sm.enableCounters(True)
     sm.resetCounters()
     mv.forward_images(0)
@@ -295,7 +296,10 @@ def views_leg(steps: int = 2) -> dict:
     return {"workload": "16 views x 512x512, 2048002 tris, 32 spp, depth 10, grads: 1024^2 texture + 1024x512 env",
             "forward_ms": round(fm, 2), "backward_ms": round(bm, 2), "backward_over_forward": round(bm / fm, 3),
             "forward_mrays": round(rf / (fm / 1e3) / 1e6, 2), "value": round(rb / (bm / 1e3) / 1e6, 2),
-            "unit": "Mrays/s"}
+            "unit": "Mrays/s",
+            # the forward keeps each path's radiance (16 B per path), so the
+            # backward replays without a primal pass of its own
+            "backward_seeded_from_forward": all(reused)}
 
 
 def main(argv=None) -> int:

@@ -450,6 +450,19 @@ pt_status pt_render_views(pt_scene* scene, int n, const pt_camera* cameras, pt_f
 pt_status pt_render_adjoint_views(pt_scene* scene, int n, const pt_camera* cameras, const uint32_t* frames,
                                   const float* const* image_grads, const pt_adjoint_grads* grads,
                                   float* const* replay_images, float* ms_out);
+/* Keeping each path's radiance for the adjoint (no reference counterpart).
+ * With keep on, every wavefront forward of this scene (pt_render_batch with
+ * PT_INTEGRATOR_WAVEFRONT, pt_render_views) also stores each path's radiance
+ * sum in device memory (16 bytes per path of the batch, freed when keep is
+ * turned off); a wavefront adjoint (pt_render_adjoint, pt_render_adjoint_views)
+ * of the same batch (the same cameras, frames and settings, and no update of
+ * the scene's device data since) seeds its replay from it instead of running
+ * the primal pass again.  Gradients and replay images are those of the
+ * recomputing adjoint.  Off by default; without the memory for it a forward
+ * renders without keeping.  reused (may be NULL): 1 when the last adjoint
+ * seeded from kept radiance. */
+pt_status pt_scene_set_keep_radiance(pt_scene* scene, int on);
+pt_status pt_scene_get_keep_radiance(const pt_scene* scene, int* on, int* reused);
 /* Sets frame_count (the sample-index offset of the next batch). */
 pt_status pt_scene_set_frame(pt_scene* scene, uint32_t frame_count);
 /* First texel and texel count of texture tex in the device texel pool. */

@@ -252,6 +252,8 @@ SIGNATURES = {
     "pt_trace_rays": (C.c_int, [_P, _FP, C.c_int, C.c_int, _FP, _I32P, _FP]),
     "pt_probe_copy": (C.c_int, [C.c_size_t, C.c_int, _FP]),
     "pt_probe_reciprocal": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pt_scene_set_keep_radiance": (C.c_int, [_P, C.c_int]),
+    "pt_scene_get_keep_radiance": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "pt_probe_bsdf": (C.c_int, [C.c_void_p, C.c_int, _FP, _I32P, _FP, _FP]),
 }
 

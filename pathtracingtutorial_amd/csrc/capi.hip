@@ -144,6 +144,11 @@ struct pt_scene {
     bool env_tables_on_device = false;  // the device env tables were built on the GPU (host copy stale)
     bool emissive_tables_on_device = false;  // likewise for the emissive-triangle tables
     uint64_t generation = 0;  // bumped by every upload; keys the captured graph
+    // bumped by every call that changes the scene's device data (geometry,
+    // materials, textures, environment, light tables): keys the radiance a
+    // wavefront forward keeps for the adjoint of the same batch
+    uint64_t data_version = 1;
+    bool keep_radiance = false;
 
     ~pt_scene() {
         if (wf) ptd::wavefront_destroy(wf);
@@ -321,20 +326,26 @@ ptd::WavefrontJob make_job(const pt_scene* s, const pt_film* f, uint32_t frame_c
 // Renders the jobs with the wavefront engine of `owner` (its buffers, stream,
 // counters and graph cache).
 float render_wavefront_jobs(pt_scene* owner, const std::vector<ptd::WavefrontJob>& jobs,
-                            const ptd::WavefrontConfig& cfg) {
+                            const ptd::WavefrontConfig& cfg, uint64_t radiance_key = 0) {
     if (!owner->wf) owner->wf = ptd::wavefront_create();
     float ms = 0.0f;
     std::string err;
     hipError_t e =
-        ptd::wavefront_render(owner->wf, jobs, cfg, owner->counters.p, owner->counting, owner->stream, &ms, &err);
+        ptd::wavefront_render(owner->wf, jobs, cfg, owner->counters.p, owner->counting, owner->stream, &ms, &err,
+                              radiance_key);
     if (e == hipErrorOutOfMemory) throw Error(PT_ERR_OUT_OF_MEMORY, err.empty() ? "wavefront: out of memory" : err);
     if (e != hipSuccess && !err.empty()) throw Error(e == hipErrorInvalidValue ? PT_ERR_INVALID_ARG : PT_ERR_DEVICE, err);
     hip_check(e, "wavefront render");
     return ms;
 }
 
-float render_wavefront(pt_scene* s, pt_film* f, const ptd::WavefrontConfig& cfg, uint32_t frame_count) {
-    return render_wavefront_jobs(s, std::vector<ptd::WavefrontJob>{make_job(s, f, frame_count)}, cfg);
+// The key under which a wavefront forward of s keeps its paths' radiance for
+// the adjoint of the same batch (0: keep nothing).
+uint64_t radiance_key(const pt_scene* s) { return s->keep_radiance ? s->data_version : 0; }
+
+float render_wavefront(pt_scene* s, pt_film* f, const ptd::WavefrontConfig& cfg, uint32_t frame_count,
+                       uint64_t key = 0) {
+    return render_wavefront_jobs(s, std::vector<ptd::WavefrontJob>{make_job(s, f, frame_count)}, cfg, key);
 }
 
 void check_wavefront_config(const pt_wavefront_config& c) {
@@ -531,6 +542,7 @@ pt_status pt_scene_get_info(const pt_scene* s, pt_scene_info* info) {
 pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         scene_device(s);
         pt_bvh_options o = opts ? *opts : (s->host.has_bvh_opts ? s->host.bvh_opts : pt_bvh_options{24, 4, 0, 0});
         if (o.bins == 0) o.bins = 24;
@@ -625,6 +637,7 @@ void upload_emissive_tables(pt_scene* s) {
 pt_status pt_scene_upload(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         if (!s->accel_built) throw Error(PT_ERR_STATE, "pt_scene_build_accel must run before pt_scene_upload");
         const HostScene& h = s->host;
         const size_t nv = h.vertices.size();
@@ -1007,7 +1020,7 @@ pt_status pt_render_batch(pt_scene* s, pt_film* f, int integrator, float* ms_out
         check_renderable(s, f);
         float ms = 0.0f;
         if (integrator == PT_INTEGRATOR_WAVEFRONT) {
-            ms = render_wavefront(s, f, s->wf_cfg, s->frame_count);
+            ms = render_wavefront(s, f, s->wf_cfg, s->frame_count, radiance_key(s));
         } else if (integrator == PT_INTEGRATOR_PATH) {
             ptd::DevScene v = make_view(s);
             ptd::RenderParams p = make_params(s);
@@ -1100,7 +1113,7 @@ float run_adjoint(pt_scene* s, const std::vector<ptd::AdjointView>& views, const
         float ms = 0.0f;
         std::string err;
         hipError_t e = ptd::wavefront_adjoint(s->wf, jobs, to_grads(g), s->wf_cfg, s->counters.p, s->counting,
-                                              s->stream, &ms, &err);
+                                              s->stream, &ms, &err, radiance_key(s));
         if (e == hipErrorOutOfMemory) throw Error(PT_ERR_OUT_OF_MEMORY, err.empty() ? "adjoint: out of memory" : err);
         if (e != hipSuccess && !err.empty())
             throw Error(e == hipErrorInvalidValue ? PT_ERR_INVALID_ARG : PT_ERR_DEVICE, err);
@@ -1184,7 +1197,7 @@ pt_status pt_render_views(pt_scene* s, int n, const pt_camera* cameras, pt_film*
             jobs[i].film_new = f->film_new.p;
             jobs[i].film_acc = f->film_acc.p;
         }
-        float ms = render_wavefront_jobs(s, jobs, s->wf_cfg);
+        float ms = render_wavefront_jobs(s, jobs, s->wf_cfg, radiance_key(s));
         if (ms_out) *ms_out = ms;
     });
 }
@@ -1202,6 +1215,22 @@ pt_status pt_scene_get_adjoint_integrator(const pt_scene* s, int* integrator) {
     return guarded([&] {
         if (!s || !integrator) throw Error(PT_ERR_INVALID_ARG, "null argument");
         *integrator = s->adjoint_integrator;
+    });
+}
+
+pt_status pt_scene_set_keep_radiance(pt_scene* s, int on) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->keep_radiance = on != 0;
+        if (!s->keep_radiance && s->wf) ptd::wavefront_release_kept(s->wf);
+    });
+}
+
+pt_status pt_scene_get_keep_radiance(const pt_scene* s, int* on, int* reused) {
+    return guarded([&] {
+        if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        if (on) *on = s->keep_radiance ? 1 : 0;
+        if (reused) *reused = ptd::wavefront_adjoint_reused_radiance(s->wf) ? 1 : 0;
     });
 }
 
@@ -1229,6 +1258,7 @@ pt_status pt_scene_update_materials(pt_scene* s, const float* base_color, const 
                                     const float* metallic) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         auto& M = s->host.materials;
         for (size_t i = 0; i < M.size(); ++i) {
             if (base_color) {
@@ -1248,6 +1278,7 @@ pt_status pt_scene_update_materials(pt_scene* s, const float* base_color, const 
 pt_status pt_scene_update_texture(pt_scene* s, uint32_t tex, const float* rgba, size_t count, int on_device) {
     return guarded([&] {
         if (!s || !rgba) throw Error(PT_ERR_INVALID_ARG, "null argument");
+        s->data_version++;
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
         ptamd::Image* img = nullptr;
         float4* dst = nullptr;
@@ -1278,6 +1309,7 @@ pt_status pt_scene_update_texture(pt_scene* s, uint32_t tex, const float* rgba, 
 pt_status pt_scene_rebuild_env_tables(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
         ptamd::LightTables t;
         ptamd::build_env_tables(s->host.env, t);
@@ -1298,6 +1330,7 @@ pt_status pt_scene_rebuild_env_tables(pt_scene* s) {
 pt_status pt_scene_build_env_tables_device(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
         const size_t n = static_cast<size_t>(s->env_w) * s->env_h;
         if (n == 0 || s->env.p == nullptr) return;
@@ -1315,6 +1348,7 @@ pt_status pt_scene_build_env_tables_device(pt_scene* s) {
 pt_status pt_scene_update_emission(pt_scene* s, const float* emissive, const float* strength) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         auto& M = s->host.materials;
         for (size_t i = 0; i < M.size(); ++i) {
             if (emissive) {
@@ -1334,6 +1368,7 @@ pt_status pt_scene_update_emission(pt_scene* s, const float* emissive, const flo
 pt_status pt_scene_rebuild_emissive_tables(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
         const HostScene& h = s->host;
         const uint32_t nf = static_cast<uint32_t>(s->prim_order_host.size());
@@ -1348,6 +1383,7 @@ pt_status pt_scene_rebuild_emissive_tables(pt_scene* s) {
 pt_status pt_scene_build_emissive_tables_device(pt_scene* s) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
+        s->data_version++;
         if (!s->uploaded) throw Error(PT_ERR_STATE, "scene not uploaded (pt_scene_upload)");
         const uint32_t nf = static_cast<uint32_t>(s->faces_bvh.n);
         if (nf == 0) return;

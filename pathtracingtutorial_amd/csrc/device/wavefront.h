@@ -66,15 +66,26 @@ void wavefront_destroy(Wavefront* w);
 // Renders one batch of every job (all jobs' paths share the queues of one
 // persistent schedule) into their films exactly as the megakernel does
 // (batch mean + running mean).  ms = GPU time from HIP events on `stream`.
+// radiance_key != 0: also keep each path's radiance sum (16 B per path) for
+// an adjoint of the same batch; the key must change whenever the scene's
+// device data does.
 hipError_t wavefront_render(Wavefront* w, const std::vector<WavefrontJob>& jobs, const WavefrontConfig& cfg,
-                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err);
+                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err,
+                            uint64_t radiance_key = 0);
 // Path-replay adjoint of one batch of every job (views of one scene, each
 // with its img_grad) in the persistent schedule: per chunk the primal pass
 // (each path's radiance), then the replay from the same sampler state, which
-// accumulates d loss / d parameter into g.  Films are not touched.
+// accumulates d loss / d parameter into g.  Films are not touched.  When the
+// last forward kept its radiance under the same radiance_key (!= 0) for the
+// same jobs (scene views and batch parameters), the replay seeds from it and
+// the primal passes are not run.
 hipError_t wavefront_adjoint(Wavefront* w, const std::vector<WavefrontJob>& jobs, const AdjointGrads& g,
                              const WavefrontConfig& cfg, DevCounters* ctr, bool count, hipStream_t stream, float* ms,
-                             std::string* err);
+                             std::string* err, uint64_t radiance_key = 0);
+// Frees the kept radiance.
+void wavefront_release_kept(Wavefront* w);
+// Whether the last adjoint seeded from kept radiance.
+bool wavefront_adjoint_reused_radiance(const Wavefront* w);
 // GPU milliseconds and launch counts per stage of the last batch rendered
 // with time_stages set (zeros otherwise).
 void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches);

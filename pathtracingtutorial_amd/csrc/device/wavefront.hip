@@ -106,6 +106,12 @@ struct WfState {
     float4* adj_g;                // d loss / d path radiance (zero above the clamp)
     float4* pend;                 // kPend per path: the last continuation's sensitivities
     float4 *sa_x, *se_x, *sp_x;   // kPayload per shadow entry: its NEE sensitivities
+    // each path's radiance sum by global path index, kept from a forward batch
+    // for the adjoint of the same batch: kept_mode 1 = the forward writes it
+    // (per-chunk finalize), 2 = the adjoint's seed reads it and the chunk's
+    // primal pass is not run, 0 = neither
+    float4* kept;
+    uint32_t kept_mode;
     const AdjointGrads* grads;    // parameter gradient buffers (device copy)
     const JobDesc* jobs;
     DevCounters* shade_ctr;  // the shading kernel's event counters (counted runs only, else null)
@@ -875,7 +881,11 @@ __global__ void k_wf_finalize(WfState w, uint32_t gp0, uint32_t gp1) {
     const unsigned long long b = (first + spf) < (c0 + n) ? (first + spf) : (c0 + n);
     if (a >= b) return;
     F3 sum = first >= c0 ? f3(0.0f) : f3(w.sum[gp]);
-    for (unsigned long long g = a; g < b; ++g) sum += clamp_radiance(f3(w.lo[g - c0]));
+    for (unsigned long long g = a; g < b; ++g) {
+        const float4 l = w.lo[g - c0];
+        if (w.kept_mode == 1) w.kept[g] = l;
+        sum += clamp_radiance(f3(l));
+    }
     w.sum[gp] = make_float4(sum.x, sum.y, sum.z, 0.0f);
 }
 
@@ -1039,7 +1049,7 @@ __global__ void k_wfa_seed(WfState w) {
     const JobDesc& job = w.jobs[job_of_path(w, g)];
     const uint32_t spf = static_cast<uint32_t>(job.params.spf);
     const uint32_t pixel = static_cast<uint32_t>((g - job.path_offset) / spf);
-    const float4 L = w.lo[p];
+    const float4 L = w.kept_mode == 2 ? w.kept[g] : w.lo[p];
     const float4 gp = job.img_grad[pixel];
     const float inv = 1.0f / static_cast<float>(spf);
     w.adj_L[p] = L;
@@ -1432,7 +1442,13 @@ public:
     // forward and backward batches keep both)
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
-    uint64_t key[2][12] = {};
+    uint64_t key[2][14] = {};
+    // kept per-path radiance (WfState::kept) and what it was rendered from:
+    // the caller's key, then each job's scene view and parameters
+    float4* kept = nullptr;
+    unsigned long long kept_capacity = 0;
+    std::vector<unsigned char> kept_sig;
+    bool last_reused = false;
 
     ~Wavefront() { release(); }
 
@@ -1446,7 +1462,14 @@ public:
         release_graph(0);
         release_graph(1);
     }
+    void release_kept() {
+        if (kept) (void)hipFree(kept);
+        kept = nullptr;
+        kept_capacity = 0;
+        kept_sig.clear();
+    }
     void release() {
+        release_kept();
         release_graph();
         void* ps[] = {st.path, st.lo, st.hit, st.q[0], st.q[1], st.sorted, st.qkey, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
@@ -1575,6 +1598,7 @@ struct ChunkShape {
     bool sort, lights;
     bool area, env;  // some job samples emissive triangles / the environment (else that shadow queue stays empty)
     bool adjoint;    // primal pass, then the replay of the path-replay adjoint
+    bool kept;       // adjoint: the primal radiance is kept from the forward batch (no primal pass)
     bool wide;       // traverse the 4-wide BVH (every job's scene has one)
 };
 
@@ -1706,7 +1730,7 @@ template <bool COUNT, int EB, int SB>
 void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& cs, const Grids& g,
                    DevCounters* ctr, hipStream_t stream, StageTimer* tm, uint32_t gp0, uint32_t gp1) {
     const WfState& w = wf.st;
-    enqueue_paths<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, tm, false);
+    if (!(cs.adjoint && cs.kept)) enqueue_paths<COUNT, EB, SB>(wf, cfg, cs, g, ctr, stream, tm, false);
     if (cs.adjoint) {
         {
             Timed t(tm, WF_CONTROL);
@@ -1840,9 +1864,26 @@ struct CaptureEnqueuer {
 namespace {
 
 // One batch of every job: the forward render (G == null) or the adjoint.
+// What a kept radiance array was rendered from: the caller's key (which
+// changes with every update of the scene's device data), then each job's
+// scene view and batch parameters (camera, frame, samples, depth, sampler).
+std::vector<unsigned char> radiance_signature(uint64_t key, const std::vector<WavefrontJob>& jobs) {
+    std::vector<unsigned char> sig(sizeof(key) + jobs.size() * (sizeof(DevScene) + sizeof(RenderParams)));
+    unsigned char* o = sig.data();
+    std::memcpy(o, &key, sizeof(key));
+    o += sizeof(key);
+    for (const WavefrontJob& j : jobs) {
+        std::memcpy(o, &j.scene, sizeof(DevScene));
+        o += sizeof(DevScene);
+        std::memcpy(o, &j.params, sizeof(RenderParams));
+        o += sizeof(RenderParams);
+    }
+    return sig;
+}
+
 hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, const WavefrontConfig& cfg,
                      DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err,
-                     const AdjointGrads* G) {
+                     const AdjointGrads* G, uint64_t radiance_key) {
     if (jobs_in.empty()) return hipSuccess;
     // job table: global path and pixel offsets
     std::vector<JobDesc> jobs(jobs_in.size());
@@ -1933,6 +1974,32 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.adjoint = G != nullptr;
     cs.wide = cfg.bvh_width == 4 && all_wide;
     wf->st.key_bins = cs.sort ? cs.nbins : 0u;
+    // kept radiance: a forward with a key writes each path's radiance (16 B
+    // per path of the batch; without the memory for it the batch renders
+    // without keeping), an adjoint whose batch signature matches reads it
+    wf->st.kept_mode = 0;
+    wf->st.kept = nullptr;
+    wf->last_reused = false;
+    std::vector<unsigned char> sig;
+    if (radiance_key != 0) sig = radiance_signature(radiance_key, jobs_in);
+    if (radiance_key != 0 && !G) {
+        wf->kept_sig.clear();  // overwritten below (or not kept at all)
+        if (paths > wf->kept_capacity) {
+            wf->release_kept();
+            if (dalloc(wf->kept, paths)) {
+                wf->kept_capacity = paths;
+            } else {
+                wf->kept = nullptr;
+                (void)hipGetLastError();
+            }
+        }
+        if (wf->kept) wf->st.kept_mode = 1;
+    } else if (radiance_key != 0 && G && wf->kept && !wf->kept_sig.empty() && wf->kept_sig == sig) {
+        wf->st.kept_mode = 2;
+        wf->last_reused = true;
+    }
+    wf->st.kept = wf->st.kept_mode ? wf->kept : nullptr;
+    cs.kept = wf->st.kept_mode == 2;
     hipError_t e;
     if ((e = hipMemcpyAsync(wf->d_jobs, jobs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice, stream)) !=
         hipSuccess)
@@ -1945,7 +2012,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     (void)hipEventRecord(e0, stream);
     hipLaunchKernelGGL(k_wf_batch_begin, dim3(1), dim3(64), 0, stream, wf->st.ctrl);
     const bool use_graph = cfg.use_graph && !cfg.time_stages;
-    uint64_t key[12] = {static_cast<uint64_t>(C), static_cast<uint64_t>(paths), static_cast<uint64_t>(pixels),
+    uint64_t key[14] = {static_cast<uint64_t>(C), static_cast<uint64_t>(paths), static_cast<uint64_t>(pixels),
                         static_cast<uint64_t>(njobs), static_cast<uint64_t>(max_depth),
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
@@ -1954,7 +2021,8 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
                         static_cast<uint64_t>(cs.nbins),
                         static_cast<uint64_t>(cfg.stack_lds + (cs.adjoint ? 1000 : 0)) |
                             (static_cast<uint64_t>(cfg.grab_extend) << 16) | (static_cast<uint64_t>(cfg.grab_shade) << 32) |
-                            (static_cast<uint64_t>(cfg.grab_shadow) << 48)};
+                            (static_cast<uint64_t>(cfg.grab_shadow) << 48),
+                        reinterpret_cast<uint64_t>(wf->st.kept), static_cast<uint64_t>(wf->st.kept_mode)};
     StageTimer timer;
     timer.stream = stream;
     StageTimer* tm = cfg.time_stages ? &timer : nullptr;
@@ -2036,21 +2104,30 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
         }
     }
     if (e != hipSuccess) return e;
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess && wf->st.kept_mode == 1) wf->kept_sig = std::move(sig);
+    return e;
 }
 
 }  // namespace
 
 hipError_t wavefront_render(Wavefront* wf, const std::vector<WavefrontJob>& jobs, const WavefrontConfig& cfg,
-                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err) {
-    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, nullptr);
+                            DevCounters* ctr, bool count, hipStream_t stream, float* ms, std::string* err,
+                            uint64_t radiance_key) {
+    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, nullptr, radiance_key);
 }
 
 hipError_t wavefront_adjoint(Wavefront* wf, const std::vector<WavefrontJob>& jobs, const AdjointGrads& g,
                              const WavefrontConfig& cfg, DevCounters* ctr, bool count, hipStream_t stream, float* ms,
-                             std::string* err) {
-    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, &g);
+                             std::string* err, uint64_t radiance_key) {
+    return run_batch(wf, jobs, cfg, ctr, count, stream, ms, err, &g, radiance_key);
 }
+
+void wavefront_release_kept(Wavefront* w) {
+    if (w) w->release_kept();
+}
+
+bool wavefront_adjoint_reused_radiance(const Wavefront* w) { return w && w->last_reused; }
 
 void wavefront_stage_times(const Wavefront* w, float* ms, uint32_t* launches) {
     for (int k = 0; k < WF_STAGES; ++k) {

@@ -174,11 +174,15 @@ class _Render(torch.autograd.Function):
 class MultiViewRenderer:
     """Several cameras of one scene: forward in one wavefront schedule
     (pt_render_views), backward in one adjoint launch (pt_render_adjoint_views).
-    All cameras must have the same resolution (the views are stacked)."""
+    All cameras must have the same resolution (the views are stacked).
+    keep_radiance: the forward keeps each path's radiance (16 bytes per path)
+    so that the adjoint of the same batch skips its primal pass
+    (pt_scene_set_keep_radiance)."""
 
-    def __init__(self, scene, cameras, texture_id: int = 0):
+    def __init__(self, scene, cameras, texture_id: int = 0, keep_radiance: bool = True):
         self.base = DiffRenderer(scene, texture_id)
         self.scene = scene
+        check(lib().pt_scene_set_keep_radiance(scene.handle, 1 if keep_radiance else 0))
         n = len(cameras)
         assert n > 0 and len({(c.width, c.height) for c in cameras}) == 1
         self.n = n
@@ -192,6 +196,7 @@ class MultiViewRenderer:
         self._film_arr = (C.c_void_p * n)(*[f.value for f in self.films])
         self.last_forward_ms = 0.0
         self.last_backward_ms = 0.0
+        self.last_backward_reused = False
 
     def push(self, *args, **kw):
         self.base.push(*args, **kw)
@@ -229,6 +234,9 @@ class MultiViewRenderer:
         check(lib().pt_render_adjoint_views(self.scene.handle, self.n, self.cams, self._frames(frame), gp,
                                             C.byref(grads), None, C.byref(ms)))
         self.last_backward_ms = ms.value
+        reused = C.c_int(0)
+        check(lib().pt_scene_get_keep_radiance(self.scene.handle, None, C.byref(reused)))
+        self.last_backward_reused = bool(reused.value)
         d_tex = d_pool[b.tex_offset:b.tex_offset + b.tex_count] if d_pool is not None else None
         return {"base_color": d_base, "roughness": d_rough, "metallic": d_metal, "texture": d_tex, "env": d_env}
 

@@ -152,3 +152,74 @@ def test_ring_views_adjoint_matches_host_reference_adjoint():
     got = np.array([gt[b, c] if k == 3 else ge[b, c] for (k, a, b, c) in params])
     scale = np.abs(fd).max()
     assert np.all(np.abs(got - fd) <= 1e-3 * np.abs(fd) + 1e-5 * scale), list(zip(params, got, fd))
+
+
+def _reused(sm):
+    r = C.c_int(-1)
+    _capi.check(pt.lib().pt_scene_get_keep_radiance(sm.handle, None, C.byref(r)))
+    return bool(r.value)
+
+
+def test_adjoint_seeded_from_kept_forward_radiance_equals_recomputed():
+    # The forward keeps each path's radiance; the adjoint of the same batch
+    # seeds its replay from it (no primal pass) and gives the gradients of the
+    # adjoint that recomputes the primal pass.  A parameter update, another
+    # frame or keep turned off each make the adjoint recompute.
+    from pathtracingtutorial_amd.diff import MultiViewRenderer
+    sm = _scene()
+    cams = _cams(sm)
+    mv = MultiViewRenderer(sm, cams)
+    rng = np.random.default_rng(9)
+    g = torch.from_numpy(rng.uniform(-1, 1, (len(cams), 32, 32, 4)).astype(np.float32)).cuda()
+    mv.forward_images(1)
+    kept = mv.adjoint(1, g)
+    assert mv.last_backward_reused and _reused(sm)
+    _capi.check(pt.lib().pt_scene_set_keep_radiance(sm.handle, 0))
+    fresh = mv.adjoint(1, g)
+    assert not mv.last_backward_reused
+    for k in ("base_color", "roughness", "metallic", "texture", "env"):
+        a, b = kept[k], fresh[k]
+        scale = float(b.abs().max()) + 1e-12
+        assert float(b.abs().sum()) > 0, k
+        # the same terms summed by FP32 atomics in another order
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * scale), k
+    _capi.check(pt.lib().pt_scene_set_keep_radiance(sm.handle, 1))
+    mv.forward_images(1)
+    mv.adjoint(2, g)  # another batch
+    assert not mv.last_backward_reused
+    mv.forward_images(1)
+    mv.push(roughness=torch.full((sm.info().num_materials,), 0.4, device="cuda"))
+    mv.adjoint(1, g)  # parameters changed since the forward
+    assert not mv.last_backward_reused
+    mv.forward_images(1)
+    mv.adjoint(1, g)
+    assert mv.last_backward_reused
+
+
+def test_single_view_wavefront_forward_keeps_radiance_for_adjoint():
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    sm = _scene()
+    dr = DiffRenderer(sm)
+    _capi.check(pt.lib().pt_scene_set_keep_radiance(sm.handle, 1))
+    rng = np.random.default_rng(11)
+    g = torch.from_numpy(rng.uniform(-1, 1, (32, 32, 4)).astype(np.float32)).cuda()
+    _capi.check(pt.lib().pt_scene_set_frame(sm.handle, 3))
+    f = C.c_void_p()
+    _capi.check(pt.lib().pt_film_create(32, 32, C.byref(f)))
+    ms = C.c_float(0.0)
+    _capi.check(pt.lib().pt_render_batch(sm.handle, f, _capi.PT_INTEGRATOR_WAVEFRONT, C.byref(ms)))
+    img = np.zeros((32, 32, 4), np.float32)
+    _capi.check(pt.lib().pt_film_download(f, 1, img.ctypes.data_as(C.POINTER(C.c_float))))
+    kept = dr.adjoint(3, g, replay=True)
+    assert _reused(sm)
+    _capi.check(pt.lib().pt_scene_set_keep_radiance(sm.handle, 0))
+    fresh = dr.adjoint(3, g, replay=True)
+    assert not _reused(sm)
+    # the replay's image: the forward's batch image, both ways
+    assert np.array_equal(kept["replay"].cpu().numpy()[..., :3], img[..., :3])
+    assert torch.equal(kept["replay"], fresh["replay"])
+    for k in ("base_color", "roughness", "metallic", "texture", "env"):
+        a, b = kept[k], fresh[k]
+        scale = float(b.abs().max()) + 1e-12
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * scale), k
+    pt.lib().pt_film_destroy(f)
