@@ -171,7 +171,8 @@ def stage_times(sm, steps: int) -> dict:
 def backward_leg(steps: int = 3) -> dict:
     """Forward and path-replay backward of the differentiable Cornell box
     (256x256, 64 spp, depth 6): GPU ms of each pass and the rays the backward
-    pass traces (its primal replay plus the adjoint replay) per second."""
+    pass traces per second (the wavefront forward keeps each path's
+    radiance, so the backward traces only the adjoint replay)."""
     import torch
 
     import pathtracingtutorial_amd as pt
@@ -185,13 +186,15 @@ def backward_leg(steps: int = 3) -> dict:
     g = torch.ones((info.height, info.width, 4), dtype=torch.float32, device="cuda")
     dr.forward_image(0)
     dr.adjoint(0, g)
-    f_ms, b_ms = [], []
+    f_ms, b_ms, reused = [], [], []
     for i in range(steps):
         dr.forward_image(i)
         f_ms.append(dr.last_forward_ms)
         dr.adjoint(i, g)
         b_ms.append(dr.last_backward_ms)
+        reused.append(dr.last_backward_reused)
     sm.enableCounters(True)
+    dr.forward_image(0)
     sm.resetCounters()
     dr.adjoint(0, g)
     c = sm.counters()
@@ -201,7 +204,8 @@ def backward_leg(steps: int = 3) -> dict:
     return {"workload": "diff-cornell-36tri-256x256-64spp-depth6-pcg", "value": round(rays / (bm / 1e3) / 1e6, 2),
             "unit": "Mrays/s", "backward_ms": round(bm, 3), "forward_ms": round(fm, 3),
             "backward_over_forward": round(bm / fm, 3), "rays_per_backward": rays,
-            "params": "256x256 albedo texture, roughness + metallic of 8 materials, 512x256 env"}
+            "params": "256x256 albedo texture, roughness + metallic of 8 materials, 512x256 env",
+            "forward": "wavefront", "backward_seeded_from_forward": all(reused)}
 
 
 def batched_leg(njobs: int = 64, seed: int = 1000) -> dict:

@@ -42,10 +42,17 @@ except RuntimeError:  # no GPU here: the module still imports (for inspection)
 
 
 class DiffRenderer:
-    """Owns a film for `scene` and the gradient buffers of its parameters."""
+    """Owns a film for `scene` and the gradient buffers of its parameters.
+    integrator: the forward's form (the wavefront by default; the megakernel,
+    PT_INTEGRATOR_PATH, renders the same image).  keep_radiance: a wavefront
+    forward keeps each path's radiance (16 bytes per path) so that the adjoint
+    of the same batch skips its primal pass (pt_scene_set_keep_radiance)."""
 
-    def __init__(self, scene, texture_id: int = 0):
+    def __init__(self, scene, texture_id: int = 0, integrator: int = _capi.PT_INTEGRATOR_WAVEFRONT,
+                 keep_radiance: bool = True):
         self.scene = scene
+        self.integrator = int(integrator)
+        check(lib().pt_scene_set_keep_radiance(scene.handle, 1 if keep_radiance else 0))
         info = scene.info()
         self.num_materials = info.num_materials
         self.width, self.height = info.width, info.height
@@ -62,6 +69,7 @@ class DiffRenderer:
             self.pool_texels = max(self.pool_texels, off.value + cnt.value)
         self.last_forward_ms = 0.0
         self.last_backward_ms = 0.0
+        self.last_backward_reused = False
 
     # -- parameter transfer ------------------------------------------------------
     def push(self, base_color=None, roughness=None, metallic=None, texture=None, env=None) -> None:
@@ -103,7 +111,7 @@ class DiffRenderer:
     def forward_image(self, frame: int) -> torch.Tensor:
         check(lib().pt_scene_set_frame(self.scene.handle, int(frame)))
         ms = C.c_float(0.0)
-        check(lib().pt_render_batch(self.scene.handle, self._film, _capi.PT_INTEGRATOR_PATH, C.byref(ms)))
+        check(lib().pt_render_batch(self.scene.handle, self._film, self.integrator, C.byref(ms)))
         self.last_forward_ms = ms.value
         img = torch.empty((self.height, self.width, 4), dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
@@ -130,6 +138,9 @@ class DiffRenderer:
         check(lib().pt_render_adjoint(self.scene.handle, int(frame), C.c_void_p(g.data_ptr()), C.byref(grads),
                                       ptr(rep), C.byref(ms)))
         self.last_backward_ms = ms.value
+        reused = C.c_int(0)
+        check(lib().pt_scene_get_keep_radiance(self.scene.handle, None, C.byref(reused)))
+        self.last_backward_reused = bool(reused.value)
         d_tex = None
         if d_pool is not None:
             d_tex = d_pool[self.tex_offset:self.tex_offset + self.tex_count]
