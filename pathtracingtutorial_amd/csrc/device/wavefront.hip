@@ -1154,10 +1154,40 @@ struct AdjDeferNee {
 };
 
 // Built for 2 waves per SIMD (registers + accumulation registers within 256).
+// The previous vertex's pending continuation of path p at the start of a
+// bounce: its sensitivities (L - lo) . rho, material and texture coordinates
+// (pv false when the path has none).
+struct PendingVertex {
+    bool pv = false;
+    uint32_t mi = 0;
+    float tu = 0.0f, tv = 0.0f;
+    float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+};
+__device__ __forceinline__ void load_pending(const WfState& w, uint32_t p, const PathState& st, PendingVertex& v) {
+    if (st.depth <= 0) return;
+    const float4* pend = w.pend + kPend * static_cast<size_t>(p);
+    const float4 p3 = pend[3];
+    const uint32_t m = __float_as_uint(p3.w);
+    if (m == kNoVertex) return;
+    const float4 p0 = pend[0], p1 = pend[1], p2 = pend[2], p4 = pend[4];
+    const float rho[15] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z};
+    const F3 rem = f3(w.adj_L[p]) - st.lo;
+#pragma unroll
+    for (int j = 0; j < kDP; ++j) v.gv[j] = rem.x * rho[j] + rem.y * rho[kDP + j] + rem.z * rho[2 * kDP + j];
+    v.pv = true;
+    v.mi = m;
+    v.tu = p4.x;
+    v.tv = p4.y;
+}
+
+// Built for 2 waves per SIMD (registers + accumulation registers within 256).
+// split: the escaped paths (each segment's items from its C_MISS0 on) are
+// left to k_wfa_shade_escape.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_wfa_shade(WfState w,
                                                                                                  int in_slot,
-                                                                                                 int use_sorted) {
+                                                                                                 int use_sorted,
+                                                                                                 int split) {
     __shared__ float macc[kWfLdsMaterials * kDP];
     clear_materials(macc);
     const AdjointGrads G = *w.grads;
@@ -1169,7 +1199,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
     const int out_slot = in_slot ^ 1;
     uint32_t* qout = w.q[out_slot];
     uint32_t* out_counts = w.ctrl + C_QCNT + out_slot * kSeg * C_LINE;
-    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade);
+    QueueWalk qw(w.ctrl, C_CUR_SHADE, counts, n, w.grab_shade, nullptr, split ? w.ctrl + C_MISS0 : nullptr);
     uint32_t i, end;
     while (qw.next(i, end)) {
         const int seg = qw.seg;
@@ -1200,33 +1230,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         nee.miss = false;
         nee.miss_coef = f3(0.0f);
         // the previous vertex's continuation, applied with (L - lo)
-        bool pv = false;
-        uint32_t pmi = 0;
-        float ptu = 0.0f, ptv = 0.0f;
-        float pgv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        PendingVertex pv;
         if (active) {
             p = q[i];
             nee.path = p;
             nee.pend = w.pend + kPend * static_cast<size_t>(p);
             load_path(w, p, st, r);
             nee.g = f3(w.adj_g[p]);
-            if (st.depth > 0) {
-                const float4 p3 = nee.pend[3];
-                const uint32_t m = __float_as_uint(p3.w);
-                if (m != kNoVertex) {
-                    const float4 p0 = nee.pend[0], p1 = nee.pend[1], p2 = nee.pend[2], p4 = nee.pend[4];
-                    const float rho[15] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w,
-                                           p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z};
-                    const F3 rem = f3(w.adj_L[p]) - st.lo;
-#pragma unroll
-                    for (int j = 0; j < kDP; ++j)
-                        pgv[j] = rem.x * rho[j] + rem.y * rho[kDP + j] + rem.z * rho[2 * kDP + j];
-                    pv = true;
-                    pmi = m;
-                    ptu = p4.x;
-                    ptv = p4.y;
-                }
-            }
+            load_pending(w, p, st, pv);
             const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
             const RenderParams& rp = job.params;
             float4 hv = w.hit[p];
@@ -1245,10 +1256,61 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
                 if (cont && !nee.wrote_pend) nee.pend[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNoVertex));
             }
         }
-        apply_vertex(s0, G, macc, pv, pmi, ptu, ptv, pgv);
+        apply_vertex(s0, G, macc, pv.pv, pv.mi, pv.tu, pv.tv, pv.gv);
         apply_env(s0, G, nee.miss, st.d, nee.miss_coef);
         const uint32_t ko = seg_base(n, seg) + wave_append(&out_counts[seg * C_LINE], cont);
         if (cont) qout[ko] = p;
+    }
+    flush_materials(macc, G, s0.num_materials);
+}
+
+// The replay's escaped paths (the material-sorted queue's last bin): the
+// previous vertex's pending continuation, the environment's radiance onto the
+// path's sum and its gradient to the environment texels; the path ends.  The
+// same steps as k_wfa_shade takes for a miss, in a kernel light enough for
+// several waves per SIMD.
+struct MissNee {
+    F3 g;
+    bool miss = false;
+    F3 coef = f3(0.0f);
+    __device__ __forceinline__ void on_env_miss(const DevScene&, const PathState& st, float mis) {
+        miss = true;
+        coef = f3(g.x * st.tp.x * mis, g.y * st.tp.y * mis, g.z * st.tp.z * mis);
+    }
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_wfa_shade_escape(WfState w,
+                                                                                                        int in_slot) {
+    __shared__ float macc[kWfLdsMaterials * kDP];
+    clear_materials(macc);
+    const AdjointGrads G = *w.grads;
+    const DevScene& s0 = w.jobs[0].scene;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
+    QueueWalk qw(w.ctrl, C_CUR_MISS, counts, n, kEscapeGrab, w.ctrl + C_MISS0, nullptr);
+    uint32_t i, end;
+    while (qw.next(i, end)) {
+        PendingVertex pv;
+        MissNee nee;
+        F3 dir = f3(0.0f);
+        if (i < end) {
+            const uint32_t p = w.sorted[i];
+            PathState st;
+            uint4 r;
+            load_path(w, p, st, r);
+            nee.g = f3(w.adj_g[p]);
+            load_pending(w, p, st, pv);
+            const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
+            if (st.depth < job.params.max_depth) {
+                escape(job.scene, st, nee);
+                w.lo[p] = make_float4(st.lo.x, st.lo.y, st.lo.z, 0.0f);
+                dir = st.d;
+            }
+        }
+        apply_vertex(s0, G, macc, pv.pv, pv.mi, pv.tu, pv.tv, pv.gv);
+        apply_env(s0, G, nee.miss, dir, nee.coef);
     }
     flush_materials(macc, G, s0.num_materials);
 }
@@ -1606,7 +1668,7 @@ struct ChunkShape {
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
     int extend, shade, escape;
-    int ashade, ashadow;  // the adjoint's replay kernels
+    int ashade, ashadow, aescape;  // the adjoint's replay kernels
 };
 
 // The traversal kernels of one stack variant (LDS = stack entries in LDS)
@@ -1706,8 +1768,11 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
             Timed t(tm, WF_SHADE);
             const int us = cs.sort ? 1 : 0;
             const int split = cs.sort ? 1 : 0;  // escapes to k_wf_shade_escape
-            if (replay)
-                hipLaunchKernelGGL((k_wfa_shade<SB>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us);
+            if (replay) {
+                hipLaunchKernelGGL((k_wfa_shade<SB>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us, split);
+                if (split)
+                    hipLaunchKernelGGL((k_wfa_shade_escape<SB>), dim3(g.aescape), dim3(SB), 0, stream, w, in);
+            }
             else if (cfg.shade_waves == 4)
                 hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
             else if (cfg.shade_waves == 3)
@@ -1768,6 +1833,8 @@ Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg, bool wide) {
     g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
     g.escape = grid_for(reinterpret_cast<const void*>(&k_wf_shade_escape<SB>), SB, wf.num_cus, cfg.waves_per_simd);
     g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
+    g.aescape =
+        grid_for(reinterpret_cast<const void*>(&k_wfa_shade_escape<SB>), SB, wf.num_cus, cfg.waves_per_simd);
     g.ashadow = wide ? std::min(grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true, true>), EB,
                                          wf.num_cus, cfg.waves_per_simd),
                                 grid_for(reinterpret_cast<const void*>(&k_wfa_shadow_punctual<COUNT, EB, true>), EB,
