@@ -868,18 +868,63 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
 }
 
 // Per global pixel touched by the chunk: add its samples of this chunk to the
-// pixel's sum, one at a time in sample order (as the megakernel does).
-__global__ void k_wf_finalize(WfState w, uint32_t gp0, uint32_t gp1) {
-    const uint32_t gp = gp0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (gp >= gp1 || gp >= w.total_pixels) return;
+// pixel's sum, one at a time in sample order (as the megakernel does).  A
+// block whose 256 pixels belong to one job and have all their samples in
+// this chunk (the usual case) reads their samples, which are contiguous, in
+// slices of kFinSlice per pixel through LDS: the loads are coalesced rather
+// than one 16-byte stream per lane; each thread then sums its own pixel's
+// slice in order.
+constexpr int kFinBlock = 256;
+constexpr int kFinSlice = 16;
+constexpr int kFinRow = 3 * kFinSlice + 1;  // floats per pixel row in LDS (padded: no bank conflicts)
+
+__global__ __launch_bounds__(kFinBlock) void k_wf_finalize(WfState w, uint32_t gp0, uint32_t gp1) {
+    __shared__ float stage[kFinBlock * kFinRow];
+    const uint32_t t = threadIdx.x;
+    const uint32_t gp = gp0 + blockIdx.x * kFinBlock + t;
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const JobDesc& job = w.jobs[job_of_pixel(w, gp)];
-    const uint32_t spf = static_cast<uint32_t>(job.params.spf);
-    const unsigned long long first = job.path_offset + static_cast<unsigned long long>(gp - job.pixel_offset) * spf;
-    const unsigned long long a = first > c0 ? first : c0;
-    const unsigned long long b = (first + spf) < (c0 + n) ? (first + spf) : (c0 + n);
-    if (a >= b) return;
+    const bool in = gp < gp1 && gp < w.total_pixels;
+    uint32_t j = 0, spf = 0;
+    unsigned long long first = 0, a = 0, b = 0;
+    if (in) {
+        j = job_of_pixel(w, gp);
+        const JobDesc& job = w.jobs[j];
+        spf = static_cast<uint32_t>(job.params.spf);
+        first = job.path_offset + static_cast<unsigned long long>(gp - job.pixel_offset) * spf;
+        a = first > c0 ? first : c0;
+        b = (first + spf) < (c0 + n) ? (first + spf) : (c0 + n);
+    }
+    // block-uniform: every thread has a pixel of thread 0's job, all of whose samples are in the chunk
+    __shared__ uint32_t job0;
+    if (t == 0) job0 = in ? j : 0xFFFFFFFFu;
+    __syncthreads();
+    const bool whole = in && j == job0 && a == first && b == first + spf;
+    if (__syncthreads_and(whole ? 1 : 0)) {
+        const unsigned long long base = first - t * static_cast<unsigned long long>(spf) - c0;  // thread 0's first, chunk-local
+        F3 sum = f3(0.0f);
+        for (uint32_t s0 = 0; s0 < spf; s0 += kFinSlice) {
+            const uint32_t cnt = min(static_cast<uint32_t>(kFinSlice), spf - s0);
+            for (uint32_t e = t; e < kFinBlock * cnt; e += kFinBlock) {
+                const uint32_t px = e / cnt, sm = e % cnt;
+                const unsigned long long idx = base + static_cast<unsigned long long>(px) * spf + s0 + sm;
+                const float4 l = w.lo[idx];
+                if (w.kept_mode == 1) w.kept[c0 + idx] = l;
+                float* row = stage + px * kFinRow + 3 * sm;
+                row[0] = l.x;
+                row[1] = l.y;
+                row[2] = l.z;
+            }
+            __syncthreads();
+            const float* row = stage + t * kFinRow;
+            for (uint32_t sm = 0; sm < cnt; ++sm)
+                sum += clamp_radiance(f3(row[3 * sm], row[3 * sm + 1], row[3 * sm + 2]));
+            __syncthreads();
+        }
+        w.sum[gp] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+        return;
+    }
+    if (!in || a >= b) return;
     F3 sum = first >= c0 ? f3(0.0f) : f3(w.sum[gp]);
     for (unsigned long long g = a; g < b; ++g) {
         const float4 l = w.lo[g - c0];
@@ -1807,7 +1852,8 @@ void enqueue_chunk(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         Timed t(tm, WF_CONTROL);
         // pixels touched by the chunk: [gp0, gp1) when known (eager), else all
         const uint32_t np = gp1 - gp0;
-        hipLaunchKernelGGL(k_wf_finalize, dim3((np + 255) / 256), dim3(256), 0, stream, w, gp0, gp1);
+        hipLaunchKernelGGL(k_wf_finalize, dim3((np + kFinBlock - 1) / kFinBlock), dim3(kFinBlock), 0, stream, w, gp0,
+                           gp1);
         hipLaunchKernelGGL(k_wf_chunk_end, dim3(1), dim3(64), 0, stream, w.ctrl);
     }
 }
