@@ -323,6 +323,8 @@ def main(argv=None) -> int:
                          'e.g. \'{"chunk_spp": 32, "extend_block": 64}\'')
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (HBM calibration) and print its GB/s")
+    ap.add_argument("--only", default="", choices=["", "batched", "backward", "views"],
+                    help="only run this leg and print its JSON object (for profiling it on its own)")
     a = ap.parse_args(argv)
     if a.copy_probe:
         import pathtracingtutorial_amd as pt
@@ -335,6 +337,13 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
               "(one process renders on one GPU; frames would shard by sample index across GPUs)", file=sys.stderr)
         return 2
+    if a.only:
+        if a.only in ("backward", "views"):
+            import torch
+            torch.cuda.init()  # before the library's runtime (see below)
+        leg = {"batched": batched_leg, "backward": backward_leg, "views": views_leg}[a.only]
+        print(json.dumps({a.only: leg()}), flush=True)
+        return 0
 
     if not a.no_backward:
         # torch ships its own HIP runtime: it must initialise the GPU before
