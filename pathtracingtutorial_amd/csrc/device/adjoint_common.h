@@ -67,36 +67,40 @@ __device__ __forceinline__ void param_chain(const DevScene& s, const pt_material
 // pixel adjoint g: gv[] += sum_c g_c T_c Le_c k df_c/dtheta.  Returns false
 // when the BSDF pdf at the light direction is zero (no term).  env_coef =
 // g * T * f * k, the weight of d/dLe for environment samples.
+// (NP: the partials carried, see to_dual; gv[NP..kDP) are left alone.)
+template <int NP = kDP>
 __device__ __forceinline__ bool light_grad(const PathState& st, F3 g, const LightSample& ls, const ShadeCtx& x,
                                            float gv[kDP], F3& env_coef) {
-    BsdfEvalT<D5> bd = eval_bsdf(to_dual(*x.me), x.wo, ls.wi, x.n, true);
+    BsdfEvalT<Dual<NP>> bd = eval_bsdf(to_dual<NP>(*x.me), x.wo, ls.wi, x.n, true);
     const float pdf_b = bd.pdf.v;
     if (pdf_b <= 0.0f) return false;
     const float k = power_heuristic(ls.pdf, pdf_b) / (ls.pdf + 1e-7f);
     const F3 w = f3(g.x * st.tp.x * ls.radiance.x * k, g.y * st.tp.y * ls.radiance.y * k,
                     g.z * st.tp.z * ls.radiance.z * k);
 #pragma unroll
-    for (int j = 0; j < kDP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
+    for (int j = 0; j < NP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
     env_coef = f3(g.x * st.tp.x * bd.f.x.v * k, g.y * st.tp.y * bd.f.y.v * k, g.z * st.tp.z * bd.f.z.v * k);
     return true;
 }
 
 // d/dtheta of a punctual light's  T * I * f  (no MIS): gv[] += sum_c g_c T_c I_c df_c/dtheta.
+template <int NP = kDP>
 __device__ __forceinline__ void punctual_grad(const PathState& st, F3 g, F3 wi, F3 inten, const ShadeCtx& x,
                                               float gv[kDP]) {
-    BsdfEvalT<D5> bd = eval_bsdf(to_dual(*x.me), x.wo, wi, x.n, true);
+    BsdfEvalT<Dual<NP>> bd = eval_bsdf(to_dual<NP>(*x.me), x.wo, wi, x.n, true);
     const F3 w = f3(g.x * st.tp.x * inten.x, g.y * st.tp.y * inten.y, g.z * st.tp.z * inten.z);
 #pragma unroll
-    for (int j = 0; j < kDP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
+    for (int j = 0; j < NP; ++j) gv[j] += w.x * bd.f.x.d[j] + w.y * bd.f.y.d[j] + w.z * bd.f.z.d[j];
 }
 
 // f of the sampled continuation over dual numbers: the BSDF at bs.wi, or
 // for a delta lobe the analytic weight (disney_bsdf.cuh:391-403).
-__device__ __forceinline__ SV3<D5> continuation_f(const ShadeCtx& x, const BsdfSample& bs) {
-    const MatEval<D5> md = to_dual(*x.me);
+template <int NP = kDP>
+__device__ __forceinline__ SV3<Dual<NP>> continuation_f(const ShadeCtx& x, const BsdfSample& bs) {
+    const MatEval<Dual<NP>> md = to_dual<NP>(*x.me);
     if (bs.delta) {
         if (dot(x.n, bs.wi) < 0.0f) return md.base;
-        SV3<D5> f0 = compute_f0(md);
+        SV3<Dual<NP>> f0 = compute_f0(md);
         F3 H = normalize(x.wo + bs.wi);
         float t5 = schlick_t5(fmaxf(dot(H, bs.wi), 0.0f));
         return sv3(f0.x + (1.0f - f0.x) * t5, f0.y + (1.0f - f0.y) * t5, f0.z + (1.0f - f0.z) * t5);

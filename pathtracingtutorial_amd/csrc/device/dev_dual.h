@@ -163,23 +163,33 @@ constexpr int kDP = 5;
 constexpr int kDBase = 0, kDRough = 3, kDMetal = 4;
 using D5 = Dual<kDP>;
 
-__device__ __forceinline__ MatEval<D5> to_dual(const MatEval<float>& m) {
-    MatEval<D5> e;
-    e.base = sv3(D5::seed(m.base.x, kDBase + 0), D5::seed(m.base.y, kDBase + 1), D5::seed(m.base.z, kDBase + 2));
-    e.metallic = D5::seed(m.metallic, kDMetal);
-    e.roughness = D5::seed(m.roughness, kDRough);
-    e.spec_trans = D5(m.spec_trans);
-    e.ior = D5(m.ior);
-    e.clearcoat = D5(m.clearcoat);
-    e.clearcoat_roughness = D5(m.clearcoat_roughness);
-    e.sheen = D5(m.sheen);
-    e.sheen_tint = D5(m.sheen_tint);
-    e.subsurface = D5(m.subsurface);
-    e.anisotropic = D5(m.anisotropic);
-    e.specular_tint = D5(m.specular_tint);
-    e.specular_factor = D5(m.specular_factor);
-    e.specular_color = sv3(D5(m.specular_color.x), D5(m.specular_color.y), D5(m.specular_color.z));
-    e.sheen_color = sv3(D5(m.sheen_color.x), D5(m.sheen_color.y), D5(m.sheen_color.z));
+// NP = kDP carries all five partials; NP = 3 only the base colour's (the
+// first three), for batches that want no roughness or metallic gradient.
+template <int NP = kDP>
+__device__ __forceinline__ MatEval<Dual<NP>> to_dual(const MatEval<float>& m) {
+    static_assert(NP == kDP || NP == 3, "partials: all five, or the base colour's");
+    using D = Dual<NP>;
+    MatEval<D> e;
+    e.base = sv3(D::seed(m.base.x, kDBase + 0), D::seed(m.base.y, kDBase + 1), D::seed(m.base.z, kDBase + 2));
+    if constexpr (NP == kDP) {
+        e.metallic = D::seed(m.metallic, kDMetal);
+        e.roughness = D::seed(m.roughness, kDRough);
+    } else {
+        e.metallic = D(m.metallic);
+        e.roughness = D(m.roughness);
+    }
+    e.spec_trans = D(m.spec_trans);
+    e.ior = D(m.ior);
+    e.clearcoat = D(m.clearcoat);
+    e.clearcoat_roughness = D(m.clearcoat_roughness);
+    e.sheen = D(m.sheen);
+    e.sheen_tint = D(m.sheen_tint);
+    e.subsurface = D(m.subsurface);
+    e.anisotropic = D(m.anisotropic);
+    e.specular_tint = D(m.specular_tint);
+    e.specular_factor = D(m.specular_factor);
+    e.specular_color = sv3(D(m.specular_color.x), D(m.specular_color.y), D(m.specular_color.z));
+    e.sheen_color = sv3(D(m.sheen_color.x), D(m.sheen_color.y), D(m.sheen_color.z));
     e.emissive = m.emissive;
     e.emissive_strength = m.emissive_strength;
     e.anisotropy_rotation = m.anisotropy_rotation;

@@ -1105,6 +1105,9 @@ __global__ void k_wfa_seed(WfState w) {
 
 // Visibility policy of the replay: DeferNee's queues, each entry with its
 // sensitivities; the continuation's rho goes to the path's pending record.
+// NP: the partials the BSDF derivatives carry (to_dual): 3 when no
+// roughness or metallic gradient is wanted, the other two then stay zero.
+template <int NP>
 struct AdjDeferNee {
     static constexpr bool kDrawAhead = false;
     uint32_t path;
@@ -1138,7 +1141,7 @@ struct AdjDeferNee {
             sa_c[ka] = make_float4(xa.x, xa.y, xa.z, 0.0f);
             float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             F3 ec = f3(0.0f);
-            light_grad(st, g, la, x, gv, ec);
+            light_grad<NP>(st, g, la, x, gv, ec);
             put_payload(sa_x + kPayload * static_cast<size_t>(ka), gv, mi, x, f3(0.0f));
         }
         const uint32_t ke = base + wave_append(se_count, e);
@@ -1148,7 +1151,7 @@ struct AdjDeferNee {
             se_c[ke] = make_float4(xe.x, xe.y, xe.z, 0.0f);
             float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             F3 ec = f3(0.0f);
-            if (!light_grad(st, g, le, x, gv, ec)) ec = f3(0.0f);
+            if (!light_grad<NP>(st, g, le, x, gv, ec)) ec = f3(0.0f);
             put_payload(se_x + kPayload * static_cast<size_t>(ke), gv, mi, x, ec);
         }
     }
@@ -1157,16 +1160,16 @@ struct AdjDeferNee {
         miss_coef = f3(g.x * st.tp.x * mis, g.y * st.tp.y * mis, g.z * st.tp.z * mis);
     }
     __device__ void on_bsdf_sample(const DevScene& s, const PathState&, const BsdfSample& bs, const ShadeCtx& x) {
-        const SV3<D5> f = continuation_f(x, bs);
+        const SV3<Dual<NP>> f = continuation_f<NP>(x, bs);
         const float rx = f.x.v != 0.0f ? g.x / f.x.v : 0.0f;
         const float ry = f.y.v != 0.0f ? g.y / f.y.v : 0.0f;
         const float rz = f.z.v != 0.0f ? g.z / f.z.v : 0.0f;
         float r[16];
 #pragma unroll
         for (int j = 0; j < kDP; ++j) {
-            r[j] = rx * f.x.d[j];
-            r[kDP + j] = ry * f.y.d[j];
-            r[2 * kDP + j] = rz * f.z.d[j];
+            r[j] = j < NP ? rx * f.x.d[j] : 0.0f;
+            r[kDP + j] = j < NP ? ry * f.y.d[j] : 0.0f;
+            r[2 * kDP + j] = j < NP ? rz * f.z.d[j] : 0.0f;
         }
         r[15] = __uint_as_float(static_cast<uint32_t>(x.mat - s.materials));
         pend[0] = make_float4(r[0], r[1], r[2], r[3]);
@@ -1192,7 +1195,7 @@ struct AdjDeferNee {
             sp_d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(path));
             sp_c[k] = make_float4(c.x, c.y, c.z, 0.0f);
             float gv[kDP] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            punctual_grad(st, g, wi, inten, x, gv);
+            punctual_grad<NP>(st, g, wi, inten, x, gv);
             put_payload(sp_x + kPayload * static_cast<size_t>(k), gv, mi, x, f3(0.0f));
         }
     }
@@ -1228,7 +1231,7 @@ __device__ __forceinline__ void load_pending(const WfState& w, uint32_t p, const
 // Built for 2 waves per SIMD (registers + accumulation registers within 256).
 // split: the escaped paths (each segment's items from its C_MISS0 on) are
 // left to k_wfa_shade_escape.
-template <int BLOCK>
+template <int BLOCK, int NP>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_wfa_shade(WfState w,
                                                                                                  int in_slot,
                                                                                                  int use_sorted,
@@ -1253,7 +1256,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         PathState st{};
         uint4 r = make_uint4(0, 0, 0, 0);
         bool cont = false;
-        AdjDeferNee nee;
+        AdjDeferNee<NP> nee;
         nee.base = seg_base(n, seg);
         nee.sa_count = &w.ctrl[C_SACNT + seg * C_LINE];
         nee.se_count = &w.ctrl[C_SECNT + seg * C_LINE];
@@ -1706,6 +1709,7 @@ struct ChunkShape {
     bool area, env;  // some job samples emissive triangles / the environment (else that shadow queue stays empty)
     bool adjoint;    // primal pass, then the replay of the path-replay adjoint
     bool kept;       // adjoint: the primal radiance is kept from the forward batch (no primal pass)
+    int adj_np;      // adjoint: partials of the BSDF derivatives (3 without roughness/metallic gradients, else 5)
     bool wide;       // traverse the 4-wide BVH (every job's scene has one)
 };
 
@@ -1713,7 +1717,7 @@ struct ChunkShape {
 // computed beforehand (no runtime queries while a graph is being captured).
 struct Grids {
     int extend, shade, escape;
-    int ashade, ashadow, aescape;  // the adjoint's replay kernels
+    int ashade, ashadow, aescape, ashade3;  // the adjoint's replay kernels (ashade3: base colour only)
 };
 
 // The traversal kernels of one stack variant (LDS = stack entries in LDS)
@@ -1814,7 +1818,11 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
             const int us = cs.sort ? 1 : 0;
             const int split = cs.sort ? 1 : 0;  // escapes to k_wf_shade_escape
             if (replay) {
-                hipLaunchKernelGGL((k_wfa_shade<SB>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us, split);
+                if (cs.adj_np == 3)
+                    hipLaunchKernelGGL((k_wfa_shade<SB, 3>), dim3(g.ashade3), dim3(SB), 0, stream, w, in, us, split);
+                else
+                    hipLaunchKernelGGL((k_wfa_shade<SB, kDP>), dim3(g.ashade), dim3(SB), 0, stream, w, in, us,
+                                       split);
                 if (split)
                     hipLaunchKernelGGL((k_wfa_shade_escape<SB>), dim3(g.aescape), dim3(SB), 0, stream, w, in);
             }
@@ -1878,7 +1886,8 @@ Grids grids_for(const Wavefront& wf, const WavefrontConfig& cfg, bool wide) {
                                                : reinterpret_cast<const void*>(&k_wf_shade<SB, 0>);
     g.shade = grid_for(shade, SB, wf.num_cus, cfg.waves_per_simd);
     g.escape = grid_for(reinterpret_cast<const void*>(&k_wf_shade_escape<SB>), SB, wf.num_cus, cfg.waves_per_simd);
-    g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB>), SB, wf.num_cus, cfg.waves_per_simd);
+    g.ashade = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB, kDP>), SB, wf.num_cus, cfg.waves_per_simd);
+    g.ashade3 = grid_for(reinterpret_cast<const void*>(&k_wfa_shade<SB, 3>), SB, wf.num_cus, cfg.waves_per_simd);
     g.aescape =
         grid_for(reinterpret_cast<const void*>(&k_wfa_shade_escape<SB>), SB, wf.num_cus, cfg.waves_per_simd);
     g.ashadow = wide ? std::min(grid_for(reinterpret_cast<const void*>(&k_wfa_shadow<COUNT, EB, true, true>), EB,
@@ -2085,6 +2094,7 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.area = any_area;
     cs.env = any_env;
     cs.adjoint = G != nullptr;
+    cs.adj_np = G && !G->rough && !G->metal ? 3 : kDP;
     cs.wide = cfg.bvh_width == 4 && all_wide;
     wf->st.key_bins = cs.sort ? cs.nbins : 0u;
     // kept radiance: a forward with a key writes each path's radiance (16 B
@@ -2128,7 +2138,8 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     uint64_t key[14] = {static_cast<uint64_t>(C), static_cast<uint64_t>(paths), static_cast<uint64_t>(pixels),
                         static_cast<uint64_t>(njobs), static_cast<uint64_t>(max_depth),
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
-                        static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves),
+                        static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves +
+                                              100000 * cs.adj_np),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
                         static_cast<uint64_t>(cs.lights + 2 * cs.area + 4 * cs.env + 8 * cs.wide),
                         static_cast<uint64_t>(cs.nbins),
