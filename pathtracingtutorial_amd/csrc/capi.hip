@@ -116,6 +116,7 @@ struct pt_scene {
     DBuf<float4> vpos;
     DBuf<uint4> faces_orig, faces_bvh;
     DBuf<float4> tris, nodes;
+    DBuf<float4> shrec;  // per-triangle shading records (DevScene::shade_rec), built at upload
     DBuf<float4> wnodes;  // 4-wide BVH (empty when the tree could overflow its traversal stack)
     uint32_t wide_max_stack = 0;
     DBuf<uint32_t> prim_order;
@@ -227,6 +228,7 @@ ptd::DevScene make_view(const pt_scene* s) {
     v.env_alias = s->env_alias.p;
     v.env = s->env.p;
     v.lights = s->plights.p;
+    v.shade_rec = s->shrec.p;
     v.num_nodes = s->nodes.p ? static_cast<uint32_t>(s->nodes.n / 2) : 0;
     v.wnodes = s->wnodes.p;
     v.num_wnodes = s->wnodes.p ? static_cast<uint32_t>(s->wnodes.n / 4) : 0;
@@ -539,6 +541,10 @@ pt_status pt_scene_get_info(const pt_scene* s, pt_scene_info* info) {
     });
 }
 
+namespace {
+void build_shade_records(pt_scene* s);
+}  // namespace
+
 pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
@@ -614,6 +620,10 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
         ptamd::build_emissive_tables(h.vertices, faces_bvh, h.materials, s->lights);
         ptamd::build_env_tables(h.env, s->lights);
         s->accel_built = true;
+        if (s->uploaded) {  // the faces' order changed: the shading records follow it
+            build_shade_records(s);
+            hip_check(hipStreamSynchronize(s->stream), "shading records");
+        }
     });
 }
 
@@ -631,6 +641,22 @@ void upload_emissive_tables(pt_scene* s) {
     s->tri_pmf.upload(L.tri_pmf.data(), L.tri_pmf.size(), s->stream);
     s->tri_alias.upload(reinterpret_cast<const float2*>(L.tri_alias.data()), L.tri_alias.size(), s->stream);
     s->emissive_tables_on_device = false;
+}
+}  // namespace
+
+namespace {
+// The shading records of the current BVH order and vertex data (after an
+// upload, and after a rebuild of the tree of an uploaded scene).
+void build_shade_records(pt_scene* s) {
+    const size_t nf = s->faces_bvh.n;
+    if (nf == 0 || !s->vnrm.p || !s->vuv.p) {
+        s->shrec.reset();
+        return;
+    }
+    if (s->shrec.n != 4 * nf) s->shrec.alloc(4 * nf);
+    hip_check(ptd::launch_shade_records(s->faces_bvh.p, s->vnrm.p, s->vuv.p, static_cast<uint32_t>(nf), s->shrec.p,
+                                        s->stream),
+              "shading records");
 }
 }  // namespace
 
@@ -703,6 +729,7 @@ pt_status pt_scene_upload(pt_scene* s) {
         if (L.env_pmf.empty()) s->env_pmf.reset(), s->env_alias.reset();
         s->env_tables_on_device = false;
         s->plights.upload(h.lights.data(), h.lights.size(), s->stream);
+        build_shade_records(s);
         hip_check(hipStreamSynchronize(s->stream), "upload");
         s->frame_count = 0;
         s->uploaded = true;

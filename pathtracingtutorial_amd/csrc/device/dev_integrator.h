@@ -333,9 +333,10 @@ __device__ __forceinline__ float bary_rn(float a, float b, float c, float w, flo
 // multiply-add fusion): a texture's filter weights are quantized to 1/256
 // of a texel, and a last-bit difference in uv can move one step between
 // neighbouring texels.
+// The vertex data come from the triangle's shading record when the scene
+// has them (one 64 B read after the hit), else through the face.
 __device__ __forceinline__ Surface make_surface(const DevScene& s, F3 o, F3 d, const Hit& h) {
     Surface sf;
-    uint4 f = s.faces[h.tri];
     float w;
     {
 #pragma clang fp contract(off)
@@ -343,16 +344,35 @@ __device__ __forceinline__ Surface make_surface(const DevScene& s, F3 o, F3 d, c
         sf.p = f3(o.x + h.t * d.x, o.y + h.t * d.y, o.z + h.t * d.z);
     }
     sf.t = h.t;
-    const float4 n0 = s.vnrm[f.x], n1 = s.vnrm[f.y], n2 = s.vnrm[f.z];
+    F3 n0, n1, n2;
+    float2 t0, t1, t2;
+    if (s.shade_rec != nullptr) {
+        const float4* r = s.shade_rec + 4 * static_cast<size_t>(h.tri);
+        const float4 a = gld(r), b = gld(r + 1), c = gld(r + 2), e = gld(r + 3);
+        n0 = f3(a.x, a.y, a.z);
+        n1 = f3(a.w, b.x, b.y);
+        n2 = f3(b.z, b.w, c.x);
+        t0 = make_float2(c.y, c.z);
+        t1 = make_float2(c.w, e.x);
+        t2 = make_float2(e.y, e.z);
+        sf.mat = __float_as_uint(e.w);
+    } else {
+        const uint4 f = s.faces[h.tri];
+        n0 = f3(s.vnrm[f.x]);
+        n1 = f3(s.vnrm[f.y]);
+        n2 = f3(s.vnrm[f.z]);
+        t0 = s.vuv[f.x];
+        t1 = s.vuv[f.y];
+        t2 = s.vuv[f.z];
+        sf.mat = f.w;
+    }
     sf.n = normalize_rn(f3(bary_rn(n0.x, n1.x, n2.x, w, h.u, h.v), bary_rn(n0.y, n1.y, n2.y, w, h.u, h.v),
                            bary_rn(n0.z, n1.z, n2.z, w, h.u, h.v)));
-    float2 t0 = s.vuv[f.x], t1 = s.vuv[f.y], t2 = s.vuv[f.z];
     sf.u = bary_rn(t0.x, t1.x, t2.x, w, h.u, h.v);
     sf.v = bary_rn(t0.y, t1.y, t2.y, w, h.u, h.v);
     sf.bu = h.u;
     sf.bv = h.v;
     sf.tri = h.tri;
-    sf.mat = f.w;
     return sf;
 }
 

@@ -55,6 +55,10 @@ struct DevScene {
     const float2* env_alias;
     const float4* env;           // env texels, RGBA
     const pt_punctual_light* lights;
+    // per BVH-ordered triangle, the vertex normals, uvs and material in one
+    // 64 B record: {n0.xyz, n1.x}, {n1.yz, n2.xy}, {n2.z, t0.xy, t1.x},
+    // {t1.y, t2.xy, material bits} (null: read the face and the vertices)
+    const float4* shade_rec;
     uint32_t num_nodes, num_faces, num_materials, num_textures;
     uint32_t num_emissive, num_env_texels, num_lights, num_wnodes;
     int32_t env_w, env_h, env_wrap_s, env_wrap_t;

@@ -35,6 +35,10 @@ hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, con
                             uint4* d_faces_bvh, hipStream_t stream);
 
 hipError_t launch_copy_probe(const float4* src, float4* dst, size_t n, hipStream_t stream);
+// DevScene::shade_rec (4 float4 per triangle) from the BVH-ordered faces and
+// the vertex normals and uvs.
+hipError_t launch_shade_records(const uint4* d_faces_bvh, const float4* d_vnrm, const float2* d_vuv, uint32_t nfaces,
+                                float4* d_out, hipStream_t stream);
 // Counts (into *d_out, which the caller zeroes) the floats of magnitude in
 // [2^-126, 2^126) whose recip_rn differs from IEEE 1.0f / x.
 hipError_t launch_probe_reciprocal(unsigned long long* d_out, hipStream_t stream);

@@ -332,6 +332,33 @@ hipError_t launch_emit_tris(const float4* d_vpos, const uint4* d_faces_orig, con
     return hipGetLastError();
 }
 
+// ---- shading records ---------------------------------------------------------------
+// DevScene::shade_rec from the BVH-ordered faces and the vertex normals and
+// uvs (the same values make_surface would gather through the face).
+
+__global__ __launch_bounds__(256) void k_shade_records(const uint4* __restrict__ faces,
+                                                       const float4* __restrict__ vnrm, const float2* __restrict__ vuv,
+                                                       uint32_t n, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 f = faces[i];
+    const float4 n0 = vnrm[f.x], n1 = vnrm[f.y], n2 = vnrm[f.z];
+    const float2 t0 = vuv[f.x], t1 = vuv[f.y], t2 = vuv[f.z];
+    float4* r = out + 4 * static_cast<size_t>(i);
+    r[0] = make_float4(n0.x, n0.y, n0.z, n1.x);
+    r[1] = make_float4(n1.y, n1.z, n2.x, n2.y);
+    r[2] = make_float4(n2.z, t0.x, t0.y, t1.x);
+    r[3] = make_float4(t1.y, t2.x, t2.y, __uint_as_float(f.w));
+}
+
+hipError_t launch_shade_records(const uint4* d_faces_bvh, const float4* d_vnrm, const float2* d_vuv, uint32_t nfaces,
+                                float4* d_out, hipStream_t stream) {
+    if (nfaces == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_records, dim3((nfaces + 255) / 256), dim3(256), 0, stream, d_faces_bvh, d_vnrm, d_vuv,
+                       nfaces, d_out);
+    return hipGetLastError();
+}
+
 // ---- copy probe ------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
