@@ -52,12 +52,13 @@ def algorithmic_bytes(c: dict) -> int:
 # and hit (16); a continuing path's record is written back (64, counted with
 # the continuation); an escape reads 4 bilinear
 # environment texels (64) and the texel's pmf (4, the MIS weight); a surface
-# hit reads the face (16), three vertex normals (48) and uvs (24) and the
-# 256-byte material record; a light sample reads the alias entry (8), the
+# hit reads the triangle's shading record (64: three vertex normals and uvs,
+# the material index) and the 256-byte material record; a light sample reads
+# the alias entry (8), the
 # pmf (4) and 4 texels (64); a shadow-queue entry is 48 bytes written, a
 # queued continuation 4 + its record 64, a radiance update 16 read + 16
 # written.
-SHADE_EVENT_BYTES = {"shade_items": 4 + 64 + 16, "shade_env_lookups": 64 + 4, "shade_hits": 16 + 48 + 24 + 256,
+SHADE_EVENT_BYTES = {"shade_items": 4 + 64 + 16, "shade_env_lookups": 64 + 4, "shade_hits": 64 + 256,
                      "shade_light_samples": 8 + 4 + 64, "shade_shadow_entries": 48, "shade_continuations": 4 + 64,
                      "shade_radiance_adds": 32}
 
