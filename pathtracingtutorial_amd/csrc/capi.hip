@@ -541,10 +541,6 @@ pt_status pt_scene_get_info(const pt_scene* s, pt_scene_info* info) {
     });
 }
 
-namespace {
-void build_shade_records(pt_scene* s);
-}  // namespace
-
 pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
     return guarded([&] {
         if (!s) throw Error(PT_ERR_INVALID_ARG, "null scene");
@@ -620,10 +616,6 @@ pt_status pt_scene_build_accel(pt_scene* s, const pt_bvh_options* opts) {
         ptamd::build_emissive_tables(h.vertices, faces_bvh, h.materials, s->lights);
         ptamd::build_env_tables(h.env, s->lights);
         s->accel_built = true;
-        if (s->uploaded) {  // the faces' order changed: the shading records follow it
-            build_shade_records(s);
-            hip_check(hipStreamSynchronize(s->stream), "shading records");
-        }
     });
 }
 

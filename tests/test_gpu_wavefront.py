@@ -298,3 +298,28 @@ def test_wavefront_split_stack_deep_bvh_bit_identical(width):
     assert np.array_equal(out[16][0], out[32][0])
     for k in ("node_visits", "tri_tests", "shadow_node_visits", "shadow_tri_tests", "stack_overflows"):
         assert out[16][1][k] == out[32][1][k], k
+
+
+@pytest.mark.parametrize("which", ["env", "cornell"])
+def test_tree_rebuilt_after_upload_renders_like_a_fresh_scene(which):
+    # a rebuilt tree reorders the triangles: the scene must be uploaded again
+    # (rendering before that is refused), and the upload rebuilds everything
+    # indexed in tree order (shading records, emissive-triangle tables)
+    make = _env_scene if which == "env" else _cornell
+    a = make()
+    a.buildAccel(32, 4, 4)
+    with pytest.raises(pt.PtError):
+        _render(a, pt.WavefrontIntegrator())
+    a.uploadToDevice()
+    b = pt.SceneManager()
+    b.generate(2, 5) if which == "env" else b.generate(0)
+    b.updateCamera(a.camera())
+    b.setSamplePerFrame(8)
+    b.setMaxDepth(6 if which == "env" else 8)
+    b.buildAccel(32, 4, 4)
+    b.uploadToDevice()
+    for integ in (pt.WavefrontIntegrator, pt.PathIntegrator):
+        ia, _ = _render(a, integ())
+        ib, _ = _render(b, integ())
+        assert np.isfinite(ia).all() and ia.mean() > 0
+        assert np.array_equal(ia, ib), integ.__name__
