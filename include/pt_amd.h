@@ -395,7 +395,7 @@ pt_status pt_render_jobs(const pt_job* jobs, int n, float* ms_out);
 /* Renderer::renderAll (renderer.cu:87-111): batches until target_samples. */
 pt_status pt_render_all(pt_scene* scene, pt_film* film, int integrator, float* total_ms);
 /* Wavefront configuration of a scene (defaults: chunk_spp 64, blocks 64 and
- * 256, waves_per_simd 0, sort 1, graph 1, time_stages 0, shade_waves 0,
+ * 256, waves_per_simd 0, sort 1, graph 1, time_stages 0, shade_waves 3,
  * stack_lds 16, grabs 64 / 128 / 64, bvh_width 4). */
 pt_status pt_scene_set_wavefront_config(pt_scene* scene, const pt_wavefront_config* cfg);
 pt_status pt_scene_get_wavefront_config(const pt_scene* scene, pt_wavefront_config* cfg);

@@ -20,7 +20,7 @@ struct WavefrontConfig {
     int sort_by_material = 1;   // counting-sort the shading queue by material
     int use_graph = 1;          // capture the bounce loop of a chunk in a hipGraph
     int time_stages = 0;        // bracket every launch with HIP events (no graph); see wavefront_stage_times
-    int shade_waves = 0;        // shading kernel built for at least this many waves per SIMD (0, 3 or 4)
+    int shade_waves = 3;        // shading kernel built for at least this many waves per SIMD (0, 3 or 4)
     int stack_lds = 16;         // traversal stack entries in LDS (16 or 32); the rest live in device memory
     // queue items a wave takes per atomic (64 .. 1024, a power of two) in the
     // closest-hit, shading and shadow kernels: larger grabs cost fewer
