@@ -1228,11 +1228,13 @@ __device__ __forceinline__ void load_pending(const WfState& w, uint32_t p, const
     v.tv = p4.y;
 }
 
-// Built for 2 waves per SIMD (registers + accumulation registers within 256).
+// Built for 2 waves per SIMD with five partials (registers + accumulation
+// registers within 256), for 3 with the base colour's three (168 registers,
+// the rest spilled: faster than 2 waves, slower than 4).
 // split: the escaped paths (each segment's items from its C_MISS0 on) are
 // left to k_wfa_shade_escape.
 template <int BLOCK, int NP>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_wfa_shade(WfState w,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(NP == 3 ? 3 : 2, 8))) void k_wfa_shade(WfState w,
                                                                                                  int in_slot,
                                                                                                  int use_sorted,
                                                                                                  int split) {
