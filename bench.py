@@ -84,6 +84,26 @@ def profiled_traffic(kernel: str, workload: str):
     return None
 
 
+def profiled_issue(kernel: str):
+    """Issue and memory counters of `kernel` (its non-counting instantiation)
+    from the committed rocprofv3 summary (profiles/wavefront_issue_counters.md):
+    VALU issue share of all SIMD cycles, share of lanes active per VALU
+    instruction, share of wave time waiting on memory, L2 hit rate; or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for block in re.findall(r"```json\n(.*?)\n```", open(path).read(), re.S):
+            try:
+                d = json.loads(block)
+            except ValueError:
+                continue
+            for name, v in d.items():
+                if isinstance(v, dict) and name.startswith(kernel + "<") and "valu_issue_share" in v:
+                    return {"kernel": name, **{k: v[k] for k in ("valu_issue_share", "active_lanes", "wait_share",
+                                                                  "l2_hit")}}
+    return None
+
+
 def cpu_baseline(sm, target_s: float = 12.0) -> dict:
     """The oracle on a bounded sample: whole rows of the same frame, 1 spp."""
     import ctypes as C
@@ -409,7 +429,8 @@ def main(argv=None) -> int:
         shade_roofline = {"bound": "hbm", "kernel": "k_wf_shade", "kernels": ["k_wf_shade", "k_wf_shade_escape"],
                           "achieved": round(sh_ach, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sh_ach / HBM_PEAK_GBS, 4),
-                          "traffic": profiled_traffic("k_wf_shade", WORKLOAD), "kernel_ms": round(sh_s * 1000, 3),
+                          "traffic": profiled_traffic("k_wf_shade", WORKLOAD),
+                          "issue": profiled_issue("k_wf_shade"), "kernel_ms": round(sh_s * 1000, 3),
                           "bytes_per_launch": int(sh_bytes),
                           "events_per_frame": {k: c[k] // a.steps for k in SHADE_EVENT_BYTES}}
     else:
@@ -446,6 +467,7 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(kernel, WORKLOAD),
+                     "issue": profiled_issue(kernel) if head == "wavefront" else None,
                      "kernel": kernel, "kernel_ms": round(kernel_s * 1000, 3),
                      "bytes_per_launch": int(bytes_per_launch), **per_ray, **extra},
     }
