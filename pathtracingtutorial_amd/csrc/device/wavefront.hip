@@ -41,6 +41,15 @@ namespace {
 constexpr int kSeg = 8;
 constexpr int kSortParts = 32;  // blocks per segment of the material sort
 constexpr uint32_t kMaxSortBins = 8192;
+// The replay's escaped paths hand their environment gradients to a splat
+// stage instead of adding them with device atomics one texel tap at a time
+// (a bounce's escapes after the first look in random directions, so a wave's
+// taps rarely share a texel): each escape's taps are recorded, the records
+// are counting-sorted by the kSplatTileW x kSplatTileH tile of their first
+// tap, and one block per (segment, tile) sums the tile's records in LDS and
+// adds the sums to the gradient with one atomic per touched texel channel.
+constexpr int kSplatTileW = 64, kSplatTileH = 32;
+constexpr uint32_t kMaxSplatTiles = 4096;
 // control words
 constexpr int C_CHUNK = 33;  // chunk index within the batch
 constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72, C_CUR_MISS = 80;
@@ -61,7 +70,8 @@ constexpr int C_QCNT = C_COUNTS;                      // [2][8] ray queue segmen
 constexpr int C_SACNT = C_COUNTS + 2 * 8 * C_LINE;  // [8] area shadow queue segment counts
 constexpr int C_SECNT = C_COUNTS + 3 * 8 * C_LINE;  // [8] environment shadow queue segment counts
 constexpr int C_SP = C_COUNTS + 4 * 8 * C_LINE;     // punctual shadow queue count (one flat queue)
-constexpr int C_WORDS = C_SP + C_LINE;
+constexpr int C_ESCNT = C_SP + C_LINE;              // [8] replay: escape records per segment
+constexpr int C_WORDS = C_ESCNT + 8 * C_LINE;
 
 // A job as the kernels see it (device memory, one entry per job).
 struct JobDesc {
@@ -106,6 +116,12 @@ struct WfState {
     float4* adj_g;                // d loss / d path radiance (zero above the clamp)
     float4* pend;                 // kPend per path: the last continuation's sensitivities
     float4 *sa_x, *se_x, *sp_x;   // kPayload per shadow entry: its NEE sensitivities
+    float4* esc_rec;              // replay: 3 per escape record (tap indices, tap weights, coefficient)
+    uint32_t* esc_key;            // its splat tile
+    uint32_t* esc_perm;           // the records of each segment in tile order
+    uint32_t* esc_hist;           // [segment][part][tile] counts, then offsets, of that order
+    uint32_t env_tiles_x;         // splat tiles across the environment
+    uint32_t env_tiles;           // splat tiles in all (0: escapes add their gradients directly)
     // each path's radiance sum by global path index, kept from a forward batch
     // for the adjoint of the same batch: kept_mode 1 = the forward writes it
     // (per-chunk finalize), 2 = the adjoint's seed reads it and the chunk's
@@ -377,6 +393,7 @@ __global__ void k_wf_bounce_begin(uint32_t* ctrl, int out_slot) {
         ctrl[C_QCNT + (out_slot * kSeg + t) * C_LINE] = 0;
         ctrl[C_SACNT + t * C_LINE] = 0;
         ctrl[C_SECNT + t * C_LINE] = 0;
+        ctrl[C_ESCNT + t * C_LINE] = 0;
     }
     if (t == 0) ctrl[C_SP] = 0;
     // (the cursors, the drained masks and the escape-bin starts; not the
@@ -451,7 +468,20 @@ __device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* coun
     b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * (part + 1)) / kSortParts);
 }
 
-__global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, int in_slot, uint32_t nbins) {
+// What a counting sort within the segments works on: keys in queue order,
+// the per-(segment, part) histograms, the output, the items it moves (null:
+// the entries' own indices), the segment counts and, for the material sort,
+// where each segment's escape bin starts.
+struct SortSpec {
+    const uint32_t* key;
+    uint32_t* hist;
+    uint32_t* out;
+    const uint32_t* src;
+    const uint32_t* counts;
+    uint32_t* miss0;
+};
+
+__global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, SortSpec a, uint32_t nbins) {
     extern __shared__ uint32_t hist[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
@@ -459,61 +489,60 @@ __global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, int in_slot, u
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg * C_LINE, seg, part, b0, b1);
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[w.qkey[i]], 1u);
+    sort_part_range(n, a.counts, seg, part, b0, b1);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[a.key[i]], 1u);
     __syncthreads();
-    uint32_t* out = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    uint32_t* out = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
 }
 
-// One block per segment: offsets in (bin, part) order, starting at the
-// segment's base, written over the counts.
-__global__ __launch_bounds__(256) void k_wf_sort_scan(WfState w, uint32_t nbins) {
-    __shared__ uint32_t part_sum[256];
+// One block of T threads per segment: offsets in (bin, part) order, starting
+// at the segment's base, written over the counts.  Each thread sums a run of
+// entries; the runs' sums are scanned in LDS (log2 T steps).
+template <int T>
+__global__ __launch_bounds__(T) void k_wf_sort_scan(WfState w, SortSpec a, uint32_t nbins) {
+    __shared__ uint32_t part_sum[T];
     const int seg = static_cast<int>(blockIdx.x);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    uint32_t* h = w.blockhist + static_cast<size_t>(seg) * kSortParts * nbins;
+    uint32_t* h = a.hist + static_cast<size_t>(seg) * kSortParts * nbins;
     const uint32_t E = nbins * kSortParts;  // entry e = bin * parts + part
     const uint32_t t = threadIdx.x;
-    const uint32_t e0 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * t) / 256);
-    const uint32_t e1 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * (t + 1)) / 256);
+    const uint32_t e0 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * t) / T);
+    const uint32_t e1 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * (t + 1)) / T);
     auto at = [&](uint32_t e) -> uint32_t& { return h[(e % kSortParts) * nbins + e / kSortParts]; };
     uint32_t sum = 0;
     for (uint32_t e = e0; e < e1; ++e) sum += at(e);
     part_sum[t] = sum;
     __syncthreads();
-    if (t == 0) {
-        uint32_t acc = seg_base(n, seg);
-        for (int k = 0; k < 256; ++k) {
-            uint32_t v = part_sum[k];
-            part_sum[k] = acc;
-            acc += v;
-        }
+    for (uint32_t off = 1; off < static_cast<uint32_t>(T); off <<= 1) {  // inclusive scan
+        const uint32_t v = t >= off ? part_sum[t - off] : 0u;
+        __syncthreads();
+        part_sum[t] += v;
+        __syncthreads();
     }
-    __syncthreads();
-    uint32_t acc = part_sum[t];
+    uint32_t acc = seg_base(n, seg) + (t > 0 ? part_sum[t - 1] : 0u);
     const uint32_t e_miss = (nbins - 1) * kSortParts;  // the escape bin (the last), part 0
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t v = at(e);
-        if (e == e_miss) w.ctrl[C_MISS0 + seg] = acc;
+        if (e == e_miss && a.miss0) a.miss0[seg] = acc;
         at(e) = acc;
         acc += v;
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, int in_slot, uint32_t nbins) {
+__global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, SortSpec a, uint32_t nbins) {
     extern __shared__ uint32_t off[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
-    const uint32_t* src = w.blockhist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
     __syncthreads();
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, w.ctrl + C_QCNT + in_slot * kSeg * C_LINE, seg, part, b0, b1);
-    const uint32_t* q = w.q[in_slot];
-    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) w.sorted[atomicAdd(&off[w.qkey[i]], 1u)] = q[i];
+    sort_part_range(n, a.counts, seg, part, b0, b1);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+        a.out[atomicAdd(&off[a.key[i]], 1u)] = a.src ? a.src[i] : i;
 }
 
 // Visibility policy of the wavefront: shadow rays go to the queues at once.
@@ -1085,6 +1114,82 @@ __device__ void apply_env(const DevScene& s, const AdjointGrads& G, bool valid, 
     }
 }
 
+// The splat stage's form of apply_env: lanes with `valid` record their taps
+// and coefficient in their segment's part of the escape records, keyed by
+// the splat tile of the first tap.  Every lane of the wave calls it.
+__device__ void env_record(const WfState& w, const DevScene& s, const AdjointGrads& G, bool valid, F3 d, F3 coef,
+                           int seg, uint32_t n) {
+    valid = valid && G.env != nullptr && s.env != nullptr && !is_zero(coef);
+    TexTaps t{};
+    if (valid) t = env_taps(s, d);
+    const uint32_t k = seg_base(n, seg) + wave_append(&w.ctrl[C_ESCNT + seg * C_LINE], valid);
+    if (!valid) return;
+    w.esc_rec[3 * static_cast<size_t>(k)] =
+        make_float4(__uint_as_float(t.idx[0]), __uint_as_float(t.idx[1]), __uint_as_float(t.idx[2]),
+                    __uint_as_float(t.idx[3]));
+    w.esc_rec[3 * static_cast<size_t>(k) + 1] = make_float4(t.w[0], t.w[1], t.w[2], t.w[3]);
+    w.esc_rec[3 * static_cast<size_t>(k) + 2] = make_float4(coef.x, coef.y, coef.z, 0.0f);
+    const uint32_t ew = static_cast<uint32_t>(s.env_w);
+    const uint32_t x0 = t.idx[0] % ew, y0 = t.idx[0] / ew;
+    w.esc_key[k] = min((y0 / kSplatTileH) * w.env_tiles_x + x0 / kSplatTileW, w.env_tiles - 1);
+}
+
+// The splat of one (segment, tile): block blockIdx.x = tile * kSeg + segment
+// sums the records of its tile (a contiguous run of the segment's sorted
+// records) in LDS over the tile and its one-texel apron on the right and
+// bottom (a first tap in the tile has its other taps there, except where the
+// lookup wraps; such taps are added directly), then adds each nonzero texel
+// channel to the environment gradient.
+constexpr int kSplatW = kSplatTileW + 1, kSplatH = kSplatTileH + 1;
+__global__ __launch_bounds__(256) void k_wfa_env_splat(WfState w) {
+    __shared__ float acc[kSplatW * kSplatH * 3];
+    const uint32_t seg = blockIdx.x % kSeg, tile = blockIdx.x / kSeg;
+    const uint32_t ntiles = w.env_tiles;
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    const uint32_t* h = w.esc_hist + static_cast<size_t>(seg) * kSortParts * ntiles;  // part 0: the tiles' starts
+    const uint32_t r0 = h[tile];
+    const uint32_t r1 = tile + 1 < ntiles ? h[tile + 1] : seg_base(n, static_cast<int>(seg)) + w.ctrl[C_ESCNT + seg * C_LINE];
+    if (r0 >= r1) return;  // the whole block
+    const AdjointGrads G = *w.grads;
+    const DevScene& s = w.jobs[0].scene;
+    const uint32_t ew = static_cast<uint32_t>(s.env_w), eh = static_cast<uint32_t>(s.env_h);
+    const uint32_t tx0 = (tile % w.env_tiles_x) * kSplatTileW, ty0 = (tile / w.env_tiles_x) * kSplatTileH;
+    for (int j = threadIdx.x; j < kSplatW * kSplatH * 3; j += blockDim.x) acc[j] = 0.0f;
+    __syncthreads();
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const size_t e = w.esc_perm[r];
+        const float4 a = w.esc_rec[3 * e], b = w.esc_rec[3 * e + 1], c = w.esc_rec[3 * e + 2];
+        const uint32_t idx[4] = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z),
+                                 __float_as_uint(a.w)};
+        const float wt[4] = {b.x, b.y, b.z, b.w};
+        const F3 coef = f3(c.x, c.y, c.z);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (wt[k] == 0.0f) continue;
+            const F3 v = coef * (wt[k] * kTwoPi);
+            const uint32_t x = idx[k] % ew, y = idx[k] / ew;
+            if (x >= tx0 && x - tx0 < static_cast<uint32_t>(kSplatW) && y >= ty0 &&
+                y - ty0 < static_cast<uint32_t>(kSplatH)) {
+                float* dst = acc + 3 * ((y - ty0) * kSplatW + (x - tx0));
+                if (v.x != 0.0f) atomicAdd(dst + 0, v.x);
+                if (v.y != 0.0f) atomicAdd(dst + 1, v.y);
+                if (v.z != 0.0f) atomicAdd(dst + 2, v.z);
+            } else {
+                atomic_add3(G.env + 4 * static_cast<size_t>(idx[k]), v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kSplatW * kSplatH * 3; j += blockDim.x) {
+        const float v = acc[j];
+        if (v == 0.0f) continue;
+        const uint32_t cell = static_cast<uint32_t>(j) / 3, ch = static_cast<uint32_t>(j) % 3;
+        const uint32_t x = tx0 + cell % kSplatW, y = ty0 + cell / kSplatW;
+        if (x < ew && y < eh) atomicAdd(G.env + 4 * (static_cast<size_t>(y) * ew + x) + ch, v);
+    }
+}
+
 __global__ void k_wfa_seed(WfState w) {
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
@@ -1340,16 +1445,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* counts = w.ctrl + C_QCNT + in_slot * kSeg * C_LINE;
     QueueWalk qw(w.ctrl, C_CUR_MISS, counts, n, kEscapeGrab, w.ctrl + C_MISS0, nullptr);
+    const bool splat = w.env_tiles != 0;
     uint32_t i, end;
     while (qw.next(i, end)) {
+        const int seg = qw.seg;
         PendingVertex pv;
         MissNee nee;
         F3 dir = f3(0.0f);
+        bool first = false;  // an escape of the camera ray
         if (i < end) {
             const uint32_t p = w.sorted[i];
             PathState st;
             uint4 r;
             load_path(w, p, st, r);
+            first = st.depth == 0;
             nee.g = f3(w.adj_g[p]);
             load_pending(w, p, st, pv);
             const JobDesc& job = w.jobs[w.njobs == 1 ? 0u : r.x];
@@ -1360,7 +1469,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
             }
         }
         apply_vertex(s0, G, macc, pv.pv, pv.mi, pv.tu, pv.tv, pv.gv);
-        apply_env(s0, G, nee.miss, dir, nee.coef);
+        // camera rays that escape are neighbours in the queue and mostly share
+        // texels (merged within the wave); later escapes go to the splat
+        apply_env(s0, G, nee.miss && (first || !splat), dir, nee.coef);
+        if (splat) env_record(w, s0, G, nee.miss && !first, dir, nee.coef, seg, n);
     }
     flush_materials(macc, G, s0.num_materials);
 }
@@ -1585,7 +1697,8 @@ public:
         release_graph();
         void* ps[] = {st.path, st.lo, st.hit, st.q[0], st.q[1], st.sorted, st.qkey, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
-                      d_jobs, st.spill, st.adj_L, st.adj_g, st.pend, st.sa_x, st.se_x, st.sp_x, d_grads};
+                      d_jobs, st.spill, st.adj_L, st.adj_g, st.pend, st.sa_x, st.se_x, st.sp_x, st.esc_rec,
+                      st.esc_key, st.esc_perm, st.esc_hist, d_grads};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         st = WfState{};
@@ -1603,7 +1716,9 @@ public:
             ok = ok && dalloc(st.adj_L, npaths) && dalloc(st.adj_g, npaths) &&
                  dalloc(st.pend, static_cast<size_t>(kPend) * npaths) &&
                  dalloc(st.sa_x, static_cast<size_t>(kPayload) * npaths) &&
-                 dalloc(st.se_x, static_cast<size_t>(kPayload) * npaths);
+                 dalloc(st.se_x, static_cast<size_t>(kPayload) * npaths) &&
+                 dalloc(st.esc_rec, static_cast<size_t>(3) * npaths) && dalloc(st.esc_key, npaths) &&
+                 dalloc(st.esc_perm, npaths);
             adj_capacity = npaths;
             release_graph();
         }
@@ -1614,6 +1729,10 @@ public:
         }
         if (ok && !d_grads) {
             ok = dalloc(d_grads, 1);
+            release_graph();
+        }
+        if (ok && !st.esc_hist) {
+            ok = dalloc(st.esc_hist, static_cast<size_t>(kSeg) * kSortParts * kMaxSplatTiles);
             release_graph();
         }
         if (!ok) {
@@ -1713,6 +1832,7 @@ struct ChunkShape {
     bool kept;       // adjoint: the primal radiance is kept from the forward batch (no primal pass)
     int adj_np;      // adjoint: partials of the BSDF derivatives (3 without roughness/metallic gradients, else 5)
     bool wide;       // traverse the 4-wide BVH (every job's scene has one)
+    uint32_t env_tiles;  // adjoint with an environment gradient: splat tiles of the escapes' gradients (else 0)
 };
 
 // Enqueues one chunk: generate, max_depth bounces, finalize.  Grid sizes are
@@ -1809,10 +1929,11 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         if (cs.sort) {
             Timed t(tm, WF_SORT);
-            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, in,
-                               cs.nbins);
-            hipLaunchKernelGGL(k_wf_sort_scan, dim3(kSeg), dim3(256), 0, stream, w, cs.nbins);
-            hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, in,
+            const SortSpec m{w.qkey, w.blockhist, w.sorted, w.q[in], w.ctrl + C_QCNT + in * kSeg * C_LINE,
+                             w.ctrl + C_MISS0};
+            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, m, cs.nbins);
+            hipLaunchKernelGGL(k_wf_sort_scan<256>, dim3(kSeg), dim3(256), 0, stream, w, m, cs.nbins);
+            hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, m,
                                cs.nbins);
         }
         {
@@ -1827,6 +1948,16 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
                                        split);
                 if (split)
                     hipLaunchKernelGGL((k_wfa_shade_escape<SB>), dim3(g.aescape), dim3(SB), 0, stream, w, in);
+                if (split && cs.env_tiles) {
+                    const SortSpec e{w.esc_key, w.esc_hist, w.esc_perm, nullptr, w.ctrl + C_ESCNT, nullptr};
+                    const size_t tile_lds = sizeof(uint32_t) * cs.env_tiles;
+                    hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
+                                       cs.env_tiles);
+                    hipLaunchKernelGGL(k_wf_sort_scan<1024>, dim3(kSeg), dim3(1024), 0, stream, w, e, cs.env_tiles);
+                    hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w,
+                                       e, cs.env_tiles);
+                    hipLaunchKernelGGL(k_wfa_env_splat, dim3(kSeg * cs.env_tiles), dim3(256), 0, stream, w);
+                }
             }
             else if (cfg.shade_waves == 4)
                 hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
@@ -2098,6 +2229,17 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
     cs.adjoint = G != nullptr;
     cs.adj_np = G && !G->rough && !G->metal ? 3 : kDP;
     cs.wide = cfg.bvh_width == 4 && all_wide;
+    {
+        // the adjoint's jobs are views of one scene: its environment's tiles
+        const DevScene& s0 = jobs_in[0].scene;
+        const uint32_t tx = s0.env_w > 0 ? (static_cast<uint32_t>(s0.env_w) + kSplatTileW - 1) / kSplatTileW : 0u;
+        const uint32_t ty = s0.env_h > 0 ? (static_cast<uint32_t>(s0.env_h) + kSplatTileH - 1) / kSplatTileH : 0u;
+        const bool splat = G != nullptr && G->env != nullptr && s0.env != nullptr && cs.sort && tx * ty > 0 &&
+                           tx * ty <= kMaxSplatTiles;
+        cs.env_tiles = splat ? tx * ty : 0u;
+        wf->st.env_tiles_x = tx;
+        wf->st.env_tiles = cs.env_tiles;
+    }
     wf->st.key_bins = cs.sort ? cs.nbins : 0u;
     // kept radiance: a forward with a key writes each path's radiance (16 B
     // per path of the batch; without the memory for it the batch renders
@@ -2141,7 +2283,8 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
                         static_cast<uint64_t>(njobs), static_cast<uint64_t>(max_depth),
                         static_cast<uint64_t>(cfg.extend_block * 1000 + cfg.shade_block),
                         static_cast<uint64_t>(cs.sort * 100 + cfg.waves_per_simd + 1000 * cfg.shade_waves +
-                                              100000 * cs.adj_np),
+                                              100000 * cs.adj_np) |
+                            (static_cast<uint64_t>(cs.env_tiles) << 32) | (static_cast<uint64_t>(wf->st.env_tiles_x) << 48),
                         static_cast<uint64_t>(count ? 1 : 0), reinterpret_cast<uint64_t>(ctr),
                         static_cast<uint64_t>(cs.lights + 2 * cs.area + 4 * cs.env + 8 * cs.wide),
                         static_cast<uint64_t>(cs.nbins),
