@@ -41,15 +41,18 @@ namespace {
 constexpr int kSeg = 8;
 constexpr int kSortParts = 32;  // blocks per segment of the material sort
 constexpr uint32_t kMaxSortBins = 8192;
-// The replay's escaped paths hand their environment gradients to a splat
-// stage instead of adding them with device atomics one texel tap at a time
-// (a bounce's escapes after the first look in random directions, so a wave's
-// taps rarely share a texel): each escape's taps are recorded, the records
-// are counting-sorted by the kSplatTileW x kSplatTileH tile of their first
-// tap, and one block per (segment, tile) sums the tile's records in LDS and
-// adds the sums to the gradient with one atomic per touched texel channel.
-constexpr int kSplatTileW = 64, kSplatTileH = 32;
+// The replay hands the environment gradients of its escapes after the first
+// bounce and of its unoccluded environment shadow rays to a splat stage
+// instead of adding them with device atomics one texel tap at a time (their
+// directions are random, so a wave's taps rarely share a texel): each one's
+// taps are recorded (a path makes at most one such record per bounce), the
+// bounce's records are counting-sorted by the kSplatTileW x kSplatTileH tile
+// of their first tap, and one block per (segment, tile) sums the tile's
+// records in LDS and adds the sums to the gradient with one atomic per
+// touched texel channel.
+constexpr int kSplatTileW = 32, kSplatTileH = 16;
 constexpr uint32_t kMaxSplatTiles = 4096;
+constexpr uint32_t kSplatParts = 4;  // blocks per (segment, tile), each a quarter of its records
 // control words
 constexpr int C_CHUNK = 33;  // chunk index within the batch
 constexpr int C_CUR_EXT = 40, C_CUR_SHADE = 48, C_CUR_SA = 56, C_CUR_SE = 64, C_CUR_SP = 72, C_CUR_MISS = 80;
@@ -116,9 +119,11 @@ struct WfState {
     float4* adj_g;                // d loss / d path radiance (zero above the clamp)
     float4* pend;                 // kPend per path: the last continuation's sensitivities
     float4 *sa_x, *se_x, *sp_x;   // kPayload per shadow entry: its NEE sensitivities
-    float4* esc_rec;              // replay: 3 per escape record (tap indices, tap weights, coefficient)
-    uint32_t* esc_key;            // its splat tile
-    uint32_t* esc_perm;           // the records of each segment in tile order
+    float4* esc_rec;              // replay: environment-gradient records (u, v, coefficient x, y)
+    float* esc_cz;                // their coefficient's z
+    uint32_t* esc_key;            // their splat tile
+    float4* esc_srec;             // the records of each segment in tile order
+    float* esc_scz;
     uint32_t* esc_hist;           // [segment][part][tile] counts, then offsets, of that order
     uint32_t env_tiles_x;         // splat tiles across the environment
     uint32_t env_tiles;           // splat tiles in all (0: escapes add their gradients directly)
@@ -1120,36 +1125,63 @@ __device__ void apply_env(const DevScene& s, const AdjointGrads& G, bool valid, 
 __device__ void env_record(const WfState& w, const DevScene& s, const AdjointGrads& G, bool valid, F3 d, F3 coef,
                            int seg, uint32_t n) {
     valid = valid && G.env != nullptr && s.env != nullptr && !is_zero(coef);
+    float u = 0.0f, v = 0.0f;
     TexTaps t{};
-    if (valid) t = env_taps(s, d);
+    if (valid) {
+        dir_to_uv(d, u, v);  // env_taps()'s steps: the splat repeats tex_taps() on (u, v)
+        t = tex_taps(s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, u, v);
+    }
     const uint32_t k = seg_base(n, seg) + wave_append(&w.ctrl[C_ESCNT + seg * C_LINE], valid);
     if (!valid) return;
-    w.esc_rec[3 * static_cast<size_t>(k)] =
-        make_float4(__uint_as_float(t.idx[0]), __uint_as_float(t.idx[1]), __uint_as_float(t.idx[2]),
-                    __uint_as_float(t.idx[3]));
-    w.esc_rec[3 * static_cast<size_t>(k) + 1] = make_float4(t.w[0], t.w[1], t.w[2], t.w[3]);
-    w.esc_rec[3 * static_cast<size_t>(k) + 2] = make_float4(coef.x, coef.y, coef.z, 0.0f);
+    w.esc_rec[k] = make_float4(u, v, coef.x, coef.y);
+    w.esc_cz[k] = coef.z;
     const uint32_t ew = static_cast<uint32_t>(s.env_w);
     const uint32_t x0 = t.idx[0] % ew, y0 = t.idx[0] / ew;
     w.esc_key[k] = min((y0 / kSplatTileH) * w.env_tiles_x + x0 / kSplatTileW, w.env_tiles - 1);
 }
 
-// The splat of one (segment, tile): block blockIdx.x = tile * kSeg + segment
-// sums the records of its tile (a contiguous run of the segment's sorted
-// records) in LDS over the tile and its one-texel apron on the right and
-// bottom (a first tap in the tile has its other taps there, except where the
-// lookup wraps; such taps are added directly), then adds each nonzero texel
-// channel to the environment gradient.
+// The counting sort's scatter for the environment-gradient records: moves
+// each record to its place in tile order (the splat then reads them in order).
+__global__ __launch_bounds__(256) void k_wfa_env_scatter(WfState w, SortSpec a, uint32_t nbins) {
+    extern __shared__ uint32_t off[];
+    const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
+    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
+    __syncthreads();
+    unsigned long long c0;
+    const uint32_t n = chunk_n(w, c0);
+    uint32_t b0, b1;
+    sort_part_range(n, a.counts, seg, part, b0, b1);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        const uint32_t o = atomicAdd(&off[a.key[i]], 1u);
+        w.esc_srec[o] = w.esc_rec[i];
+        w.esc_scz[o] = w.esc_cz[i];
+    }
+}
+
+// The splat of one (segment, tile), in kSplatParts blocks: block blockIdx.x
+// = (tile * kSeg + segment) * kSplatParts + part sums its part of the tile's
+// records (a contiguous run of the segment's sorted records) in LDS over the
+// tile and its one-texel apron on the right and bottom (a first tap in the
+// tile has its other taps there, except where the lookup wraps; such taps
+// are added directly), then adds each nonzero texel channel to the
+// environment gradient.
 constexpr int kSplatW = kSplatTileW + 1, kSplatH = kSplatTileH + 1;
 __global__ __launch_bounds__(256) void k_wfa_env_splat(WfState w) {
     __shared__ float acc[kSplatW * kSplatH * 3];
-    const uint32_t seg = blockIdx.x % kSeg, tile = blockIdx.x / kSeg;
+    const uint32_t part = blockIdx.x % kSplatParts, st = blockIdx.x / kSplatParts;
+    const uint32_t seg = st % kSeg, tile = st / kSeg;
     const uint32_t ntiles = w.env_tiles;
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     const uint32_t* h = w.esc_hist + static_cast<size_t>(seg) * kSortParts * ntiles;  // part 0: the tiles' starts
-    const uint32_t r0 = h[tile];
-    const uint32_t r1 = tile + 1 < ntiles ? h[tile + 1] : seg_base(n, static_cast<int>(seg)) + w.ctrl[C_ESCNT + seg * C_LINE];
+    const uint32_t t0 = h[tile];
+    const uint32_t t1 =
+        tile + 1 < ntiles ? h[tile + 1] : seg_base(n, static_cast<int>(seg)) + w.ctrl[C_ESCNT + seg * C_LINE];
+    if (t0 >= t1) return;  // the whole block
+    const uint32_t len = t1 - t0;
+    const uint32_t r0 = t0 + static_cast<uint32_t>((static_cast<unsigned long long>(len) * part) / kSplatParts);
+    const uint32_t r1 = t0 + static_cast<uint32_t>((static_cast<unsigned long long>(len) * (part + 1)) / kSplatParts);
     if (r0 >= r1) return;  // the whole block
     const AdjointGrads G = *w.grads;
     const DevScene& s = w.jobs[0].scene;
@@ -1158,17 +1190,14 @@ __global__ __launch_bounds__(256) void k_wfa_env_splat(WfState w) {
     for (int j = threadIdx.x; j < kSplatW * kSplatH * 3; j += blockDim.x) acc[j] = 0.0f;
     __syncthreads();
     for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const size_t e = w.esc_perm[r];
-        const float4 a = w.esc_rec[3 * e], b = w.esc_rec[3 * e + 1], c = w.esc_rec[3 * e + 2];
-        const uint32_t idx[4] = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z),
-                                 __float_as_uint(a.w)};
-        const float wt[4] = {b.x, b.y, b.z, b.w};
-        const F3 coef = f3(c.x, c.y, c.z);
+        const float4 a = w.esc_srec[r];
+        const F3 coef = f3(a.z, a.w, w.esc_scz[r]);
+        const TexTaps t = tex_taps(s.env_w, s.env_h, s.env_wrap_s, s.env_wrap_t, a.x, a.y);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (wt[k] == 0.0f) continue;
-            const F3 v = coef * (wt[k] * kTwoPi);
-            const uint32_t x = idx[k] % ew, y = idx[k] / ew;
+            if (t.w[k] == 0.0f) continue;
+            const F3 v = coef * (t.w[k] * kTwoPi);
+            const uint32_t x = t.idx[k] % ew, y = t.idx[k] / ew;
             if (x >= tx0 && x - tx0 < static_cast<uint32_t>(kSplatW) && y >= ty0 &&
                 y - ty0 < static_cast<uint32_t>(kSplatH)) {
                 float* dst = acc + 3 * ((y - ty0) * kSplatW + (x - tx0));
@@ -1176,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_wfa_env_splat(WfState w) {
                 if (v.y != 0.0f) atomicAdd(dst + 1, v.y);
                 if (v.z != 0.0f) atomicAdd(dst + 2, v.z);
             } else {
-                atomic_add3(G.env + 4 * static_cast<size_t>(idx[k]), v);
+                atomic_add3(G.env + 4 * static_cast<size_t>(t.idx[k]), v);
             }
         }
     }
@@ -1526,7 +1555,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             }
         }
         apply_vertex(s0, G, macc, vis, mi, tu, tv, gv);
-        if (ENV) apply_env(s0, G, vis, dir, ec);
+        if (ENV && w.env_tiles) env_record(w, s0, G, vis, dir, ec, qw.seg, n);
+        else if (ENV) apply_env(s0, G, vis, dir, ec);
     }
     flush<COUNT, true>(ctr, rays, cnt.nodes, cnt.tris, ovf);
     flush_materials(macc, G, s0.num_materials);
@@ -1698,7 +1728,7 @@ public:
         void* ps[] = {st.path, st.lo, st.hit, st.q[0], st.q[1], st.sorted, st.qkey, st.sa_o, st.sa_d,
                       st.sa_c, st.se_o, st.se_d, st.se_c, st.sp_o, st.sp_d, st.sp_c, st.blockhist, st.sum, st.ctrl,
                       d_jobs, st.spill, st.adj_L, st.adj_g, st.pend, st.sa_x, st.se_x, st.sp_x, st.esc_rec,
-                      st.esc_key, st.esc_perm, st.esc_hist, d_grads};
+                      st.esc_cz, st.esc_key, st.esc_srec, st.esc_scz, st.esc_hist, d_grads};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         st = WfState{};
@@ -1717,8 +1747,8 @@ public:
                  dalloc(st.pend, static_cast<size_t>(kPend) * npaths) &&
                  dalloc(st.sa_x, static_cast<size_t>(kPayload) * npaths) &&
                  dalloc(st.se_x, static_cast<size_t>(kPayload) * npaths) &&
-                 dalloc(st.esc_rec, static_cast<size_t>(3) * npaths) && dalloc(st.esc_key, npaths) &&
-                 dalloc(st.esc_perm, npaths);
+                 dalloc(st.esc_rec, npaths) && dalloc(st.esc_cz, npaths) && dalloc(st.esc_key, npaths) &&
+                 dalloc(st.esc_srec, npaths) && dalloc(st.esc_scz, npaths);
             adj_capacity = npaths;
             release_graph();
         }
@@ -1948,16 +1978,6 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
                                        split);
                 if (split)
                     hipLaunchKernelGGL((k_wfa_shade_escape<SB>), dim3(g.aescape), dim3(SB), 0, stream, w, in);
-                if (split && cs.env_tiles) {
-                    const SortSpec e{w.esc_key, w.esc_hist, w.esc_perm, nullptr, w.ctrl + C_ESCNT, nullptr};
-                    const size_t tile_lds = sizeof(uint32_t) * cs.env_tiles;
-                    hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
-                                       cs.env_tiles);
-                    hipLaunchKernelGGL(k_wf_sort_scan<1024>, dim3(kSeg), dim3(1024), 0, stream, w, e, cs.env_tiles);
-                    hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w,
-                                       e, cs.env_tiles);
-                    hipLaunchKernelGGL(k_wfa_env_splat, dim3(kSeg * cs.env_tiles), dim3(256), 0, stream, w);
-                }
             }
             else if (cfg.shade_waves == 4)
                 hipLaunchKernelGGL((k_wf_shade<SB, 4>), dim3(g.shade), dim3(SB), 0, stream, w, in, us, split);
@@ -1971,6 +1991,20 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         {
             Timed t(tm, WF_SHADOW);
             launch_shadow<COUNT, EB>(cfg, cs, g, stream, w, replay, ctr);
+        }
+        if (replay && cs.sort && cs.env_tiles) {
+            // the bounce's environment-gradient records (escapes after the
+            // first bounce, unoccluded environment shadow rays: at most one
+            // per path) to the gradient
+            Timed t(tm, WF_SHADE);
+            const SortSpec e{w.esc_key, w.esc_hist, nullptr, nullptr, w.ctrl + C_ESCNT, nullptr};
+            const size_t tile_lds = sizeof(uint32_t) * cs.env_tiles;
+            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
+                               cs.env_tiles);
+            hipLaunchKernelGGL(k_wf_sort_scan<1024>, dim3(kSeg), dim3(1024), 0, stream, w, e, cs.env_tiles);
+            hipLaunchKernelGGL(k_wfa_env_scatter, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
+                               cs.env_tiles);
+            hipLaunchKernelGGL(k_wfa_env_splat, dim3(kSeg * cs.env_tiles * kSplatParts), dim3(256), 0, stream, w);
         }
     }
 }
