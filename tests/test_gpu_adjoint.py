@@ -353,3 +353,67 @@ def test_adjoint_full_arrays_match_host_reference_adjoint(form):
     compare_arrays("metallic", out["metallic"].cpu().numpy(), host["metallic"], frac=1.0)
     compare_arrays("texture", out["texture"].cpu().numpy()[:, :3], host["textures"][0].reshape(-1, 4)[:, :3])
     compare_arrays("env", out["env"].cpu().numpy()[..., :3], host["env"][..., :3])
+
+
+def _quads_under_sky(ew, eh, seed=4):
+    """A diffuse ground quad and a tilted glossy quad under a random ew x eh
+    environment, seen from above with the sky behind them."""
+    from test_oracle_furnace import _material
+    rng = np.random.default_rng(seed)
+    pts = [(-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, 4), (-1, 0.3, -1), (1, 1.3, -1), (1, 1.3, 1), (-1, 0.3, 1)]
+    verts = (_capi.Vertex * 8)()
+    for i, p in enumerate(pts):
+        verts[i].position = _capi.Vec3p.of(p)
+        n = np.array([0.0, 1.0, 0.0]) if i < 4 else np.array([-1.0, 2.0, 0.0]) / np.sqrt(5.0)
+        verts[i].normal = _capi.Vec3p.of(tuple(n))
+    faces = (_capi.Face * 4)()
+    faces[0], faces[1] = _capi.Face(0, 2, 1, 0), _capi.Face(0, 3, 2, 0)
+    faces[2], faces[3] = _capi.Face(4, 6, 5, 1), _capi.Face(4, 7, 6, 1)
+    mats = (_capi.Material * 2)(_material(roughness=0.7), _material(metallic=0.8, roughness=0.3))
+    env = np.ones((eh, ew, 4), np.float32)
+    env[..., :3] = rng.uniform(0.2, 3.0, (eh, ew, 3)).astype(np.float32)
+    cam = _capi.Camera()
+    cam.position = _capi.Vec3p.of((0.0, 3.0, 6.0))
+    cam.look_at = _capi.Vec3p.of((0.0, 0.5, 0.0))
+    cam.up = _capi.Vec3p.of((0.0, 1.0, 0.0))
+    cam.fov, cam.focal_distance = 60.0, 1.0
+    cam.width, cam.height = 48, 32
+    st = _capi.RenderSettings()
+    st.sample_per_frame, st.target_samples, st.max_depth, st.sampler = 8, 8, 5, 0
+    d = _capi.HostSceneDesc()
+    d.vertices, d.num_vertices = verts, 8
+    d.faces, d.num_faces = faces, 4
+    d.materials, d.num_materials = mats, 2
+    d.camera = C.pointer(cam)
+    d.env_rgba = env.ctypes.data_as(C.POINTER(C.c_float))
+    d.env_width, d.env_height = ew, eh
+    d.settings = C.pointer(st)
+    sm = pt.SceneManager()
+    sm.create(d)
+    sm.buildAccel()
+    sm.uploadToDevice()
+    sm.setWavefrontConfig(bvh_width=2)
+    return sm
+
+
+@pytest.mark.parametrize("size", [(90, 45), (33, 17)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_environment_gradient_splat_on_odd_sizes(size):
+    # The wavefront adjoint adds the environment gradients of later escapes and
+    # of environment shadow rays through per-tile LDS sums; an environment
+    # whose sides are not multiples of the tile has partial tiles, and
+    # bilinear taps that wrap round the seam fall outside their tile.  The
+    # gradients equal those of the per-pixel adjoint, which adds every tap
+    # directly, and of the wavefront without the material sort (no splat).
+    sm = _quads_under_sky(*size)
+    rng = np.random.default_rng(8)
+    g = torch.from_numpy(rng.uniform(-1, 1, (32, 48, 4)).astype(np.float32)).cuda()
+    out = _both_forms(sm, 2, g)
+    env = out["wavefront"]["env"]
+    assert env is not None and float(env.abs().sum()) > 0
+    assert int((env.reshape(-1, 4)[:, :3] != 0).any(1).sum()) > size[0] * size[1] // 4  # spread over the map
+    _assert_same_grads(out["path"], out["wavefront"], keys=("base_color", "roughness", "metallic", "env"))
+    from pathtracingtutorial_amd.diff import DiffRenderer
+    sm.setWavefrontConfig(sort_by_material=0)
+    sm.setAdjointIntegrator(pt.WavefrontIntegrator())
+    unsorted = DiffRenderer(sm).adjoint(2, g)
+    _assert_same_grads(out["wavefront"], unsorted, keys=("base_color", "roughness", "metallic", "env"))
