@@ -396,12 +396,13 @@ def _quads_under_sky(ew, eh, seed=4):
     return sm
 
 
-@pytest.mark.parametrize("size", [(90, 45), (33, 17)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("size", [(90, 45), (33, 17), (2080, 1040)], ids=lambda s: f"{s[0]}x{s[1]}")
 def test_environment_gradient_splat_on_odd_sizes(size):
     # The wavefront adjoint adds the environment gradients of later escapes and
     # of environment shadow rays through per-tile LDS sums; an environment
     # whose sides are not multiples of the tile has partial tiles, and
-    # bilinear taps that wrap round the seam fall outside their tile.  The
+    # bilinear taps that wrap round the seam fall outside their tile; one of
+    # more than 4096 tiles (2080 x 1040) adds its taps directly.  The
     # gradients equal those of the per-pixel adjoint, which adds every tap
     # directly, and of the wavefront without the material sort (no splat).
     sm = _quads_under_sky(*size)
@@ -410,7 +411,8 @@ def test_environment_gradient_splat_on_odd_sizes(size):
     out = _both_forms(sm, 2, g)
     env = out["wavefront"]["env"]
     assert env is not None and float(env.abs().sum()) > 0
-    assert int((env.reshape(-1, 4)[:, :3] != 0).any(1).sum()) > size[0] * size[1] // 4  # spread over the map
+    touched = int((env.reshape(-1, 4)[:, :3] != 0).any(1).sum())
+    assert touched > min(size[0] * size[1] // 4, 2000)  # spread over the map
     _assert_same_grads(out["path"], out["wavefront"], keys=("base_color", "roughness", "metallic", "env"))
     from pathtracingtutorial_amd.diff import DiffRenderer
     sm.setWavefrontConfig(sort_by_material=0)
