@@ -2226,11 +2226,12 @@ hipError_t run_batch(Wavefront* wf, const std::vector<WavefrontJob>& jobs_in, co
             wf->num_cus = 256;
     }
     const uint32_t njobs = static_cast<uint32_t>(jobs.size());
-    // At most 2^28 paths in flight (~56 GB of path state and queues, the
-    // adjoint's replay state ~30 GB more): a 1080p frame of 64 samples runs as
-    // one chunk, whose bounce loop drains once (every chunk ends in a tail of
-    // nearly empty launches).  A device that cannot hold that many gets half
-    // as many, down to 2^22.
+    // At most 2^28 paths in flight (~56 GB of path state and queues, 208 B
+    // per path; the adjoint's replay state, pending records, shadow payloads
+    // and splat records ~68 GB more, 252 B per path): a 1080p frame of 64
+    // samples runs as one chunk, whose bounce loop drains once (every chunk
+    // ends in a tail of nearly empty launches).  A device that cannot hold
+    // that many gets half as many, down to 2^22.
     unsigned long long cap = std::min<unsigned long long>(1ull << 28, want);
     uint32_t C = 0;
     while (true) {
