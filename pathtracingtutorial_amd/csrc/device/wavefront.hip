@@ -1119,9 +1119,10 @@ __device__ void apply_env(const DevScene& s, const AdjointGrads& G, bool valid, 
     }
 }
 
-// The splat stage's form of apply_env: lanes with `valid` record their taps
-// and coefficient in their segment's part of the escape records, keyed by
-// the splat tile of the first tap.  Every lane of the wave calls it.
+// The splat stage's form of apply_env: lanes with `valid` record the lookup's
+// (u, v) and their coefficient in their segment's part of the records, keyed
+// by the splat tile of the lookup's first tap.  Every lane of the wave calls
+// it.
 __device__ void env_record(const WfState& w, const DevScene& s, const AdjointGrads& G, bool valid, F3 d, F3 coef,
                            int seg, uint32_t n) {
     valid = valid && G.env != nullptr && s.env != nullptr && !is_zero(coef);
