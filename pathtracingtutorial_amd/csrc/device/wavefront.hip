@@ -39,7 +39,11 @@ namespace ptd {
 namespace {
 
 constexpr int kSeg = 8;
-constexpr int kSortParts = 32;  // blocks per segment of the material sort
+// Blocks per segment of the counting sorts (their histograms are per block):
+// 128 for the material sort (1080p frame: sort 4.8 -> 2.4 ms with 32), 64 for
+// the replay's splat records (whose scan covers up to 4096 tiles per part).
+constexpr int kSortParts = 128;
+constexpr int kSplatSortParts = 64;
 constexpr uint32_t kMaxSortBins = 8192;
 // The replay hands the environment gradients of its escapes after the first
 // bounce and of its unoccluded environment shadow rays to a splat stage
@@ -466,17 +470,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
 // material index (of whichever job the path belongs to), written per queue
 // item by the closest-hit kernel (qkey).
 
-__device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* counts, int seg, int part, uint32_t& b0,
-                                                uint32_t& b1) {
+__device__ __forceinline__ void sort_part_range(uint32_t n, const uint32_t* counts, int seg, int part,
+                                                uint32_t parts, uint32_t& b0, uint32_t& b1) {
     const uint32_t base = seg_base(n, seg), cnt = counts[seg * C_LINE];
-    b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * part) / kSortParts);
-    b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * (part + 1)) / kSortParts);
+    b0 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * part) / parts);
+    b1 = base + static_cast<uint32_t>((static_cast<unsigned long long>(cnt) * (part + 1)) / parts);
 }
 
 // What a counting sort within the segments works on: keys in queue order,
 // the per-(segment, part) histograms, the output, the items it moves (null:
-// the entries' own indices), the segment counts and, for the material sort,
-// where each segment's escape bin starts.
+// the entries' own indices), the segment counts, for the material sort where
+// each segment's escape bin starts, and the blocks (parts) per segment.
 struct SortSpec {
     const uint32_t* key;
     uint32_t* hist;
@@ -484,6 +488,7 @@ struct SortSpec {
     const uint32_t* src;
     const uint32_t* counts;
     uint32_t* miss0;
+    uint32_t parts;
 };
 
 __global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, SortSpec a, uint32_t nbins) {
@@ -494,10 +499,10 @@ __global__ __launch_bounds__(256) void k_wf_sort_count(WfState w, SortSpec a, ui
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, a.counts, seg, part, b0, b1);
+    sort_part_range(n, a.counts, seg, part, a.parts, b0, b1);
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&hist[a.key[i]], 1u);
     __syncthreads();
-    uint32_t* out = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    uint32_t* out = a.hist + (static_cast<size_t>(seg) * a.parts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) out[b] = hist[b];
 }
 
@@ -510,12 +515,13 @@ __global__ __launch_bounds__(T) void k_wf_sort_scan(WfState w, SortSpec a, uint3
     const int seg = static_cast<int>(blockIdx.x);
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    uint32_t* h = a.hist + static_cast<size_t>(seg) * kSortParts * nbins;
-    const uint32_t E = nbins * kSortParts;  // entry e = bin * parts + part
+    const uint32_t P = a.parts;
+    uint32_t* h = a.hist + static_cast<size_t>(seg) * P * nbins;
+    const uint32_t E = nbins * P;  // entry e = bin * parts + part
     const uint32_t t = threadIdx.x;
     const uint32_t e0 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * t) / T);
     const uint32_t e1 = static_cast<uint32_t>((static_cast<unsigned long long>(E) * (t + 1)) / T);
-    auto at = [&](uint32_t e) -> uint32_t& { return h[(e % kSortParts) * nbins + e / kSortParts]; };
+    auto at = [&](uint32_t e) -> uint32_t& { return h[(e % P) * nbins + e / P]; };
     uint32_t sum = 0;
     for (uint32_t e = e0; e < e1; ++e) sum += at(e);
     part_sum[t] = sum;
@@ -527,7 +533,7 @@ __global__ __launch_bounds__(T) void k_wf_sort_scan(WfState w, SortSpec a, uint3
         __syncthreads();
     }
     uint32_t acc = seg_base(n, seg) + (t > 0 ? part_sum[t - 1] : 0u);
-    const uint32_t e_miss = (nbins - 1) * kSortParts;  // the escape bin (the last), part 0
+    const uint32_t e_miss = (nbins - 1) * P;  // the escape bin (the last), part 0
     for (uint32_t e = e0; e < e1; ++e) {
         uint32_t v = at(e);
         if (e == e_miss && a.miss0) a.miss0[seg] = acc;
@@ -539,13 +545,13 @@ __global__ __launch_bounds__(T) void k_wf_sort_scan(WfState w, SortSpec a, uint3
 __global__ __launch_bounds__(256) void k_wf_sort_scatter(WfState w, SortSpec a, uint32_t nbins) {
     extern __shared__ uint32_t off[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
-    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * a.parts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
     __syncthreads();
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, a.counts, seg, part, b0, b1);
+    sort_part_range(n, a.counts, seg, part, a.parts, b0, b1);
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x)
         a.out[atomicAdd(&off[a.key[i]], 1u)] = a.src ? a.src[i] : i;
 }
@@ -1146,13 +1152,13 @@ __device__ void env_record(const WfState& w, const DevScene& s, const AdjointGra
 __global__ __launch_bounds__(256) void k_wfa_env_scatter(WfState w, SortSpec a, uint32_t nbins) {
     extern __shared__ uint32_t off[];
     const int seg = static_cast<int>(blockIdx.x % kSeg), part = static_cast<int>(blockIdx.x / kSeg);
-    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * kSortParts + part) * nbins;
+    const uint32_t* src = a.hist + (static_cast<size_t>(seg) * a.parts + part) * nbins;
     for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) off[b] = src[b];
     __syncthreads();
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
     uint32_t b0, b1;
-    sort_part_range(n, a.counts, seg, part, b0, b1);
+    sort_part_range(n, a.counts, seg, part, a.parts, b0, b1);
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
         const uint32_t o = atomicAdd(&off[a.key[i]], 1u);
         w.esc_srec[o] = w.esc_rec[i];
@@ -1175,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_wfa_env_splat(WfState w) {
     const uint32_t ntiles = w.env_tiles;
     unsigned long long c0;
     const uint32_t n = chunk_n(w, c0);
-    const uint32_t* h = w.esc_hist + static_cast<size_t>(seg) * kSortParts * ntiles;  // part 0: the tiles' starts
+    const uint32_t* h = w.esc_hist + static_cast<size_t>(seg) * kSplatSortParts * ntiles;  // part 0: the tiles' starts
     const uint32_t t0 = h[tile];
     const uint32_t t1 =
         tile + 1 < ntiles ? h[tile + 1] : seg_base(n, static_cast<int>(seg)) + w.ctrl[C_ESCNT + seg * C_LINE];
@@ -1763,7 +1769,7 @@ public:
             release_graph();
         }
         if (ok && !st.esc_hist) {
-            ok = dalloc(st.esc_hist, static_cast<size_t>(kSeg) * kSortParts * kMaxSplatTiles);
+            ok = dalloc(st.esc_hist, static_cast<size_t>(kSeg) * kSplatSortParts * kMaxSplatTiles);
             release_graph();
         }
         if (!ok) {
@@ -1960,10 +1966,13 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
         }
         if (cs.sort) {
             Timed t(tm, WF_SORT);
-            const SortSpec m{w.qkey, w.blockhist, w.sorted, w.q[in], w.ctrl + C_QCNT + in * kSeg * C_LINE,
-                             w.ctrl + C_MISS0};
+            const SortSpec m{w.qkey,      w.blockhist, w.sorted, w.q[in], w.ctrl + C_QCNT + in * kSeg * C_LINE,
+                             w.ctrl + C_MISS0, kSortParts};
             hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, m, cs.nbins);
-            hipLaunchKernelGGL(k_wf_sort_scan<256>, dim3(kSeg), dim3(256), 0, stream, w, m, cs.nbins);
+            if (static_cast<size_t>(cs.nbins) * kSortParts > 65536)  // many materials: more scan threads
+                hipLaunchKernelGGL(k_wf_sort_scan<1024>, dim3(kSeg), dim3(1024), 0, stream, w, m, cs.nbins);
+            else
+                hipLaunchKernelGGL(k_wf_sort_scan<256>, dim3(kSeg), dim3(256), 0, stream, w, m, cs.nbins);
             hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSeg * kSortParts), dim3(256), sort_lds, stream, w, m,
                                cs.nbins);
         }
@@ -1998,12 +2007,12 @@ void enqueue_paths(Wavefront& wf, const WavefrontConfig& cfg, const ChunkShape& 
             // first bounce, unoccluded environment shadow rays: at most one
             // per path) to the gradient
             Timed t(tm, WF_SHADE);
-            const SortSpec e{w.esc_key, w.esc_hist, nullptr, nullptr, w.ctrl + C_ESCNT, nullptr};
+            const SortSpec e{w.esc_key, w.esc_hist, nullptr, nullptr, w.ctrl + C_ESCNT, nullptr, kSplatSortParts};
             const size_t tile_lds = sizeof(uint32_t) * cs.env_tiles;
-            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
+            hipLaunchKernelGGL(k_wf_sort_count, dim3(kSeg * kSplatSortParts), dim3(256), tile_lds, stream, w, e,
                                cs.env_tiles);
             hipLaunchKernelGGL(k_wf_sort_scan<1024>, dim3(kSeg), dim3(1024), 0, stream, w, e, cs.env_tiles);
-            hipLaunchKernelGGL(k_wfa_env_scatter, dim3(kSeg * kSortParts), dim3(256), tile_lds, stream, w, e,
+            hipLaunchKernelGGL(k_wfa_env_scatter, dim3(kSeg * kSplatSortParts), dim3(256), tile_lds, stream, w, e,
                                cs.env_tiles);
             hipLaunchKernelGGL(k_wfa_env_splat, dim3(kSeg * cs.env_tiles * kSplatParts), dim3(256), 0, stream, w);
         }
